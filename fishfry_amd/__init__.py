@@ -1,0 +1,286 @@
+"""fishfry_amd -- MI355X (gfx950) periodic Poisson solver behind fishfry's
+ParisPencil interface.
+
+The compute path is libfishfry.so (hand-written HIP kernels, built in-tree by
+``python -c "import __graft_entry__; __graft_entry__.build()"`` or
+``make -C fishfry_amd/csrc``).  This module binds its C-ABI (include/fishfry.h)
+with ctypes and mirrors the reference's plugin interface:
+
+* ``ParisPencil(n, lo, hi, m, id)`` with ``bytes()`` and
+  ``solve(bytes, density, potential, stamps)`` -- ParisPencil.hpp:23-44;
+* ``Field`` / ``field_init`` / ``field_diff`` -- FishFry.hpp:14-101;
+* ``FishFry(n_tasks, n_points).run(n_iters)`` -- FishFry.hpp:103-229.
+
+Device buffers are torch CUDA tensors (PyTorch is used only for device memory
+and streams) or raw device pointers.  There is no CPU fallback: if the library
+is missing or fails, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from typing import List, Optional, Sequence, Tuple
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfishfry.so")
+
+FF_OK = 0
+FF_ERR_INVALID = 1
+FF_ERR_UNSUPPORTED = 2
+FF_ERR_BYTES = 3
+FF_ERR_HIP = 4
+FF_ERR_NOMEM = 5
+FF_ERR_REJECTED = 6
+FF_ERR_TIMEOUT = 7
+FF_ERR_NOT_FOUND = 8
+
+FF_FP64 = 0
+FF_FP32_REFINE = 1
+FF_PRESERVE_INPUT = 1
+FF_SOLVE_STAMPS = 14
+FF_MAX_MODES = 32
+
+STATUS_NAMES = {
+    0: "FF_OK", 1: "FF_ERR_INVALID", 2: "FF_ERR_UNSUPPORTED", 3: "FF_ERR_BYTES", 4: "FF_ERR_HIP",
+    5: "FF_ERR_NOMEM", 6: "FF_ERR_REJECTED", 7: "FF_ERR_TIMEOUT", 8: "FF_ERR_NOT_FOUND",
+}
+
+
+class FishFryError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("refine_steps", ctypes.c_int), ("flags", ctypes.c_int)]
+
+
+class Stamp(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64), ("seconds", ctypes.c_double)]
+
+
+class Field(ctypes.Structure):
+    """ff_field: grid, phase shift, amplitude and extra modes of f."""
+    _fields_ = [
+        ("n", ctypes.c_int * 3),
+        ("lo", ctypes.c_double * 3),
+        ("d", ctypes.c_double * 3),
+        ("shift", ctypes.c_double * 3),
+        ("scale", ctypes.c_double),
+        ("dd", ctypes.c_double * 3),
+        ("nmodes", ctypes.c_int),
+        ("mode_k", (ctypes.c_int * 3) * FF_MAX_MODES),
+        ("mode_a", ctypes.c_double * FF_MAX_MODES),
+        ("mode_b", ctypes.c_double * FF_MAX_MODES),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libfishfry.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FishFryError(FF_ERR_INVALID, f"{LIB_PATH} is missing: build it first (make -C fishfry_amd/csrc)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    i3 = ctypes.POINTER(ctypes.c_int)
+    d3 = ctypes.POINTER(ctypes.c_double)
+    L.ff_last_error.restype = ctypes.c_char_p
+    L.ff_last_error.argtypes = []
+    L.ff_reference_bytes.restype = ctypes.c_size_t
+    L.ff_reference_bytes.argtypes = [i3, i3]
+    L.ff_plan_create.restype = ctypes.c_int
+    L.ff_plan_create.argtypes = [i3, d3, d3, i3, i3, ctypes.POINTER(Options), ctypes.POINTER(P)]
+    L.ff_plan_bytes.restype = ctypes.c_size_t
+    L.ff_plan_bytes.argtypes = [P]
+    L.ff_plan_destroy.restype = ctypes.c_int
+    L.ff_plan_destroy.argtypes = [P]
+    L.ff_solve.restype = ctypes.c_int
+    L.ff_solve.argtypes = [P, ctypes.c_size_t, P, P, P, ctypes.POINTER(Stamp), ctypes.POINTER(ctypes.c_int)]
+    L.ff_solve_async.restype = ctypes.c_int
+    L.ff_solve_async.argtypes = [P, ctypes.c_size_t, P, P, P]
+    L.ff_plan_describe.restype = ctypes.c_int
+    L.ff_plan_describe.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
+    L.ff_solve_pass.restype = ctypes.c_int
+    L.ff_solve_pass.argtypes = [P, ctypes.c_int, P, P, P]
+    L.ff_field_default.restype = None
+    L.ff_field_default.argtypes = [i3, ctypes.POINTER(Field)]
+    L.ff_field_init.restype = ctypes.c_int
+    L.ff_field_init.argtypes = [ctypes.POINTER(Field), P, P]
+    L.ff_field_diff.restype = ctypes.c_int
+    L.ff_field_diff.argtypes = [ctypes.POINTER(Field), P, d3, P]
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return lib().ff_last_error().decode()
+
+
+def check(status: int, what: str = "") -> None:
+    if status != FF_OK:
+        raise FishFryError(status, f"{what}: {last_error()}" if what else last_error())
+
+
+def _i3(v):
+    return (ctypes.c_int * 3)(*[int(x) for x in v])
+
+
+def _d3(v):
+    return (ctypes.c_double * 3)(*[float(x) for x in v])
+
+
+def _ptr(buf) -> int:
+    """Device address of a torch tensor or an int."""
+    if buf is None:
+        return 0
+    if isinstance(buf, int):
+        return buf
+    return int(buf.data_ptr())
+
+
+def _stream(stream) -> int:
+    if stream is None:
+        try:
+            import torch
+
+            return int(torch.cuda.current_stream().cuda_stream)
+        except Exception:
+            return 0
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)
+
+
+def reference_bytes(n: Sequence[int], m: Sequence[int] = (1, 1, 1)) -> int:
+    """The reference's per-array byte formula (ParisPencil.cpp:108-134)."""
+    return int(lib().ff_reference_bytes(_i3(n), _i3(m)))
+
+
+def benchmark_grid(n: Sequence[int]) -> Tuple[List[float], List[float], List[float]]:
+    """lo, hi, dx of FishFry.hpp:109-111,133-134 at 1x1x1 tasks."""
+    dx = [2.0 * math.pi / n[a] for a in range(3)]
+    return [-math.pi] * 3, [math.pi - dx[a] for a in range(3)], dx
+
+
+class ParisPencil:
+    """Mirror of the reference's solver plugin (ParisPencil.hpp:11-59).
+
+    ``ParisPencil(n, lo, hi, m, id)`` plans the solve on the current device;
+    ``bytes()`` is the per-array allocation the caller must make;
+    ``solve(bytes, density, potential, stamps)`` overwrites ``potential`` with
+    phi (``density`` is clobbered in FP64 mode, as in the reference).
+    """
+
+    def __init__(self, n: Sequence[int], lo: Sequence[float], hi: Sequence[float],
+                 m: Sequence[int] = (1, 1, 1), id: Sequence[int] = (0, 0, 0),
+                 precision: int = FF_FP64, refine_steps: int = 0, flags: int = 0):
+        self.n = tuple(int(v) for v in n)
+        self.precision = precision
+        self.refine_steps = refine_steps
+        h = ctypes.c_void_p()
+        opts = Options(precision, refine_steps, flags)
+        check(lib().ff_plan_create(_i3(n), _d3(lo), _d3(hi), _i3(m), _i3(id), ctypes.byref(opts), ctypes.byref(h)),
+              "ParisPencil")
+        self._h = h
+
+    @property
+    def handle(self) -> int:
+        return self._h.value or 0
+
+    def bytes(self) -> int:
+        return int(lib().ff_plan_bytes(self._h))
+
+    def solve(self, nbytes: int, density, potential, stamps: Optional[list] = None, stream=None) -> None:
+        L = lib()
+        if stamps is None:
+            check(L.ff_solve(self._h, nbytes, _ptr(density), _ptr(potential), _stream(stream), None, None), "solve")
+            return
+        arr = (Stamp * FF_SOLVE_STAMPS)()
+        cnt = ctypes.c_int(FF_SOLVE_STAMPS)
+        check(L.ff_solve(self._h, nbytes, _ptr(density), _ptr(potential), _stream(stream), arr, ctypes.byref(cnt)),
+              "solve")
+        stamps.clear()
+        for i in range(cnt.value):
+            stamps.append((arr[i].name.decode(), arr[i].seconds))
+
+    def solve_async(self, nbytes: int, density, potential, stream=None) -> None:
+        check(lib().ff_solve_async(self._h, nbytes, _ptr(density), _ptr(potential), _stream(stream)), "solve_async")
+
+    def solve_pass(self, p: int, src, dst, stream=None) -> None:
+        check(lib().ff_solve_pass(self._h, p, _ptr(src), _ptr(dst), _stream(stream)), f"pass {p}")
+
+    def describe(self) -> str:
+        buf = ctypes.create_string_buffer(4096)
+        check(lib().ff_plan_describe(self._h, buf, len(buf)), "describe")
+        return buf.value.decode()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().ff_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def field_default(n: Sequence[int]) -> Field:
+    """The benchmark's f on the 1x1x1 grid (FishFry.hpp:108-139)."""
+    f = Field()
+    lib().ff_field_default(_i3(n), ctypes.byref(f))
+    return f
+
+
+def field_set_modes(f: Field, modes) -> Field:
+    """modes: iterable of ((kx, ky, kz), a, b) -> f += a cos(k.x) + b sin(k.x)."""
+    modes = list(modes)
+    if len(modes) > FF_MAX_MODES:
+        raise FishFryError(FF_ERR_INVALID, "too many modes")
+    f.nmodes = len(modes)
+    for m, (k, a, b) in enumerate(modes):
+        for c in range(3):
+            f.mode_k[m][c] = int(k[c])
+        f.mode_a[m] = float(a)
+        f.mode_b[m] = float(b)
+    return f
+
+
+def field_init(field: Field, f, stream=None) -> None:
+    check(lib().ff_field_init(ctypes.byref(field), _ptr(f), _stream(stream)), "field_init")
+
+
+def field_diff(field: Field, phi, stream=None) -> Tuple[float, float, float]:
+    out = (ctypes.c_double * 3)()
+    check(lib().ff_field_diff(ctypes.byref(field), _ptr(phi), out, _stream(stream)), "field_diff")
+    return out[0], out[1], out[2]
+
+
+def field_norms(field: Field, phi, stream=None) -> Tuple[float, float, float]:
+    """Relative L1, L2, Linf of FishFry.hpp:146-159,198."""
+    s = field_diff(field, None, stream)
+    e = field_diff(field, phi, stream)
+    return e[0] / s[0], math.sqrt(e[1] / s[1]), e[2] / s[2]
+
+
+from .driver import FishFry  # noqa: E402  (needs the names above)
+
+__all__ = [
+    "ParisPencil", "FishFry", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
+    "field_norms", "reference_bytes", "benchmark_grid", "FishFryError", "lib", "LIB_PATH",
+    "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT",
+]

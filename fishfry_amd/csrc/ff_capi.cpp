@@ -1,0 +1,204 @@
+// extern "C" entry points of libfishfry.so (declared in include/fishfry.h).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "ff_solver.hpp"
+
+namespace ff {
+const char* last_error_text();
+}
+
+using namespace ff;
+
+static double now_seconds() {
+  using clk = std::chrono::steady_clock;
+  return std::chrono::duration<double>(clk::now().time_since_epoch()).count();
+}
+
+extern "C" {
+
+const char* ff_last_error(void) { return last_error_text(); }
+
+size_t ff_reference_bytes(const int n[3], const int m[3]) {
+  if (!n || !m) return 0;
+  for (int a = 0; a < 3; ++a)
+    if (n[a] < 1 || m[a] < 1) return 0;
+  return reference_bytes(n, m);
+}
+
+ff_status ff_plan_create(const int n[3], const double lo[3], const double hi[3], const int m[3], const int id[3],
+                         const ff_options* opts, ff_plan** out) {
+  clear_error();
+  if (!out) return set_error(FF_ERR_INVALID, "ff_plan_create: null output handle");
+  *out = nullptr;
+  if (!n || !lo || !hi) return set_error(FF_ERR_INVALID, "ff_plan_create: null n/lo/hi");
+  if (m && (m[0] != 1 || m[1] != 1 || m[2] != 1))
+    return set_error(FF_ERR_UNSUPPORTED, "ff_plan_create: only 1x1x1 tasks (one GPU) are supported");
+  if (id && (id[0] != 0 || id[1] != 0 || id[2] != 0))
+    return set_error(FF_ERR_INVALID, "ff_plan_create: id must be {0,0,0} at 1x1x1 tasks");
+  return plan_create(n, lo, hi, opts, out);
+}
+
+size_t ff_plan_bytes(const ff_plan* plan) { return plan ? plan->bytes : 0; }
+
+ff_status ff_plan_destroy(ff_plan* plan) {
+  plan_destroy(plan);
+  return FF_OK;
+}
+
+static ff_status check_solve(const ff_plan* plan, size_t bytes, const double* density, const double* potential) {
+  if (!plan) return set_error(FF_ERR_INVALID, "ff_solve: null plan");
+  if (!density || !potential) return set_error(FF_ERR_INVALID, "ff_solve: null density or potential");
+  if (density == potential) return set_error(FF_ERR_INVALID, "ff_solve: density and potential must differ");
+  if (bytes < plan->bytes) {
+    char buf[160];
+    snprintf(buf, sizeof buf, "ff_solve: bytes %zu < required %zu (ParisPencil::bytes)", bytes, plan->bytes);
+    return set_error(FF_ERR_BYTES, buf);
+  }
+  return FF_OK;
+}
+
+ff_status ff_solve_async(const ff_plan* plan, size_t bytes, double* density, double* potential, void* stream) {
+  clear_error();
+  ff_status st = check_solve(plan, bytes, density, potential);
+  if (st != FF_OK) return st;
+  return solve_enqueue(plan, density, potential, (hipStream_t)stream, nullptr);
+}
+
+ff_status ff_solve(const ff_plan* plan, size_t bytes, double* density, double* potential, void* stream,
+                   ff_stamp* stamps, int* n_stamps) {
+  clear_error();
+  ff_status st = check_solve(plan, bytes, density, potential);
+  if (st != FF_OK) return st;
+  if (stamps && (!n_stamps || *n_stamps < 0)) return set_error(FF_ERR_INVALID, "ff_solve: n_stamps required");
+  hipStream_t s = (hipStream_t)stream;
+  const double t0 = now_seconds();
+  const bool timed = stamps && plan->opt.precision == FF_FP64;
+  hipEvent_t ev[6] = {};
+  if (timed) {
+    for (int e = 0; e < 6; ++e)
+      if ((st = hip_check(hipEventCreate(&ev[e]), "hipEventCreate"))) return st;
+    st = hip_check(hipEventRecord(ev[5], s), "event");
+  }
+  if (st == FF_OK) st = solve_enqueue(plan, density, potential, s, timed ? ev : nullptr);
+  if (st == FF_OK) st = hip_check(hipStreamSynchronize(s), "solve synchronize");
+  const double t1 = now_seconds();
+  if (st == FF_OK && stamps) {
+    // the reference's 14 stamps (ParisPencil.cpp:157,201,203,247,249,294,296,
+    // 370,372,416,418,461,463,483); its reorders and self all-to-alls are
+    // fused away here, so they take zero time at the end of their pass
+    static const char* names[FF_SOLVE_STAMPS] = {"Total",
+                                                 "Reorder 3D block into sub-pencils",
+                                                 "Redistribute into Z pencils",
+                                                 "Real-to-complex FFT in Z",
+                                                 "Redistribute for Y pencils",
+                                                 "Forward FFT in Y",
+                                                 "Redistribute for X pencils",
+                                                 "X FFTs and filter",
+                                                 "Redistribute for Y pencils",
+                                                 "Backward FFT in Y",
+                                                 "Redistribute in Z pencils",
+                                                 "Complex-to-real FFT in Z",
+                                                 "Redistribute for 3D blocks",
+                                                 "Rearrange into 3D blocks and normalize"};
+    // pass boundary (0..5) reached by each stamp
+    static const int at[FF_SOLVE_STAMPS] = {0, 0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 5};
+    double pass_t[6] = {t0, t0, t0, t0, t0, t1};
+    if (timed) {
+      for (int e = 0; e < 5; ++e) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev[5], ev[e]) == hipSuccess) pass_t[e + 1] = t0 + 1e-3 * ms;
+      }
+      pass_t[5] = t1;
+    }
+    const int cnt = *n_stamps < FF_SOLVE_STAMPS ? *n_stamps : FF_SOLVE_STAMPS;
+    for (int i = 0; i < cnt; ++i) {
+      snprintf(stamps[i].name, sizeof stamps[i].name, "%s", names[i]);
+      stamps[i].seconds = i == FF_SOLVE_STAMPS - 1 ? t1 : pass_t[at[i]];
+    }
+    *n_stamps = cnt;
+  } else if (n_stamps) {
+    *n_stamps = 0;
+  }
+  if (timed)
+    for (int e = 0; e < 6; ++e)
+      if (ev[e]) (void)hipEventDestroy(ev[e]);
+  return st;
+}
+
+ff_status ff_plan_describe(const ff_plan* p, char* buf, size_t len) {
+  clear_error();
+  if (!p || !buf || !len) return set_error(FF_ERR_INVALID, "ff_plan_describe: bad arguments");
+  std::string s = "{";
+  char tmp[512];
+  snprintf(tmp, sizeof tmp, "\"n\":[%d,%d,%d],\"nh\":%d,\"bytes\":%zu,\"precision\":%d,\"refine_steps\":%d,", p->n[0],
+           p->n[1], p->n[2], p->nh, p->bytes, p->opt.precision, p->opt.refine_steps);
+  s += tmp;
+  snprintf(tmp, sizeof tmp, "\"dd\":[%.17g,%.17g,%.17g],", p->ddi, p->ddj, p->ddk);
+  s += tmp;
+  auto line = [&](const char* key, const LineCfg& c, int T, size_t lds, int threads) {
+    std::string r = "\"";
+    r += key;
+    r += "\":{";
+    snprintf(tmp, sizeof tmp, "\"n\":%d,\"E\":%d,\"L\":%d,\"T\":%d,\"threads\":%d,\"lds\":%zu,\"radices\":[", c.n, c.E,
+             c.lp.L, T, threads, lds);
+    r += tmp;
+    for (int i = 0; i < c.lp.np; ++i) {
+      snprintf(tmp, sizeof tmp, "%s%d", i ? "," : "", c.lp.radix[i]);
+      r += tmp;
+    }
+    r += "]}";
+    return r;
+  };
+  s += line("z", p->zr[0].line, p->zr[0].T, p->zr[0].lds, p->zr[0].threads) + ",";
+  s += line("y", p->cy[0].line, p->cy[0].T, p->cy[0].lds, p->cy[0].threads) + ",";
+  s += line("x", p->cx[0].line, p->cx[0].T, p->cx[0].lds, p->cx[0].threads);
+  s += "}";
+  if (s.size() + 1 > len) return set_error(FF_ERR_INVALID, "ff_plan_describe: buffer too small");
+  memcpy(buf, s.c_str(), s.size() + 1);
+  return FF_OK;
+}
+
+ff_status ff_solve_pass(const ff_plan* p, int pass, const double* src, double* dst, void* stream) {
+  clear_error();
+  if (!p || !src || !dst) return set_error(FF_ERR_INVALID, "ff_solve_pass: null argument");
+  if (p->opt.precision != FF_FP64) return set_error(FF_ERR_INVALID, "ff_solve_pass: FP64 plans only");
+  hipStream_t s = (hipStream_t)stream;
+  switch (pass) {
+    case 0: return run_zfwd(p, 0, 0, src, dst, s);
+    case 1: return run_col(p, 0, 1, COL_FWD, src, dst, s);
+    case 2: return run_col(p, 0, 0, COL_FILTER, src, dst, s);
+    case 3: return run_col(p, 0, 1, COL_INV, src, dst, s);
+    case 4: return run_zinv(p, 0, 0, EPI_STORE, src, dst, nullptr, nullptr, s);
+    default: return set_error(FF_ERR_INVALID, "ff_solve_pass: pass must be 0..4");
+  }
+}
+
+void ff_field_default(const int n[3], ff_field* out) {
+  if (!out) return;
+  memset(out, 0, sizeof *out);
+  for (int a = 0; a < 3; ++a) {
+    out->n[a] = n ? n[a] : 0;
+    out->d[a] = n && n[a] ? 2.0 * M_PI / n[a] : 0.0;  // FishFry.hpp:109-111
+    out->lo[a] = -M_PI;                               // FishFry.hpp:133,137-139 at id = 0
+    const double hi = M_PI - out->d[a];               // FishFry.hpp:134
+    out->dd[a] = n && n[a] ? 2.0 * M_PI * double(n[a] - 1) / (double(n[a]) * (hi - out->lo[a])) : 0.0;
+  }
+  out->scale = 1.0;
+}
+
+ff_status ff_field_init(const ff_field* field, double* f, void* stream) {
+  clear_error();
+  return field_init(field, f, (hipStream_t)stream);
+}
+
+ff_status ff_field_diff(const ff_field* field, const double* phi, double out[3], void* stream) {
+  clear_error();
+  std::lock_guard<std::mutex> g(scratch_mutex());
+  return field_diff(field, phi, out, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,332 @@
+// Device FFT engine for the Poisson passes (gfx950, wave64).
+//
+// A "line" is one 1D transform of length n.  Each line is owned by L = n/E
+// threads; thread r0 of a line holds the E elements r0 + i*L (i = 0..E-1) in
+// registers.  A transform is a sequence of Stockham autosort passes with radix
+// R (every R divides E, the product of the radices is n).  In every pass a
+// thread performs E/R butterflies j = r0 + q*L, q = 0..E/R-1, whose inputs are
+// the elements j + m*n/R = r0 + (q + m*E/R)*L -- registers q + m*E/R -- so the
+// input side of a pass never needs data from another thread.  The outputs go
+// to their autosorted positions ((j - j%ns)*R + j%ns + m*ns) in LDS, and the
+// next pass reads them back in natural order.  After the last pass the
+// registers hold the result in natural order again, so a kernel can chain a
+// forward transform, a pointwise multiply and an inverse transform without an
+// extra exchange.
+//
+// Twiddles come from a per-length table W[t] = exp(-2*pi*i*t/n), t < n,
+// generated on the host in long double with exact integer argument reduction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ff_consts.hpp"
+
+#define FF_MAXP 16
+#define FF_MAX_THREADS 256
+
+namespace ff {
+
+template <typename F>
+struct alignas(2 * sizeof(F)) cx {
+  F x, y;
+};
+
+template <typename F>
+__device__ __forceinline__ cx<F> operator+(cx<F> a, cx<F> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename F>
+__device__ __forceinline__ cx<F> operator-(cx<F> a, cx<F> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename F>
+__device__ __forceinline__ cx<F> scale(cx<F> a, F s) { return {a.x * s, a.y * s}; }
+template <typename F>
+__device__ __forceinline__ cx<F> conj(cx<F> a) { return {a.x, -a.y}; }
+
+// a * w for the forward direction (DIR < 0), a * conj(w) for the inverse.
+template <int DIR, typename F>
+__device__ __forceinline__ cx<F> twmul(cx<F> a, cx<F> w) {
+  if (DIR < 0) return {a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x};
+  return {a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y};
+}
+
+// a * (-i) forward, a * (+i) inverse.
+template <int DIR, typename F>
+__device__ __forceinline__ cx<F> mul_mi(cx<F> a) {
+  if (DIR < 0) return {a.y, -a.x};
+  return {-a.y, a.x};
+}
+
+// a * exp(DIR * 2*pi*i*t/R) with t, R compile-time.
+template <int R, int t_, int DIR, typename F>
+__device__ __forceinline__ cx<F> rot(cx<F> a) {
+  constexpr int t = ((t_ % R) + R) % R;
+  if constexpr (t == 0) {
+    return a;
+  } else if constexpr ((4 * t) % R == 0) {
+    constexpr int q = (4 * t) / R;
+    if constexpr (q == 2) return {-a.x, -a.y};
+    else if constexpr (q == 1) return mul_mi<DIR>(a);
+    else return mul_mi<-DIR>(a);
+  } else if constexpr ((8 * t) % R == 0) {
+    constexpr F h = F(0.7071067811865475244008444);
+    constexpr int o = (8 * t) / R;  // odd multiple of pi/4
+    // exp(DIR*i*o*pi/4) = (cx, DIR*sx)
+    constexpr F cxv = (o == 1 || o == 7) ? h : -h;
+    constexpr F sxv = (o == 1 || o == 3) ? h : -h;
+    constexpr F s = DIR < 0 ? -sxv : sxv;
+    return {cxv * a.x - s * a.y, cxv * a.y + s * a.x};
+  } else {
+    constexpr F c = F(kcos(R, t));
+    constexpr F s0 = F(ksin(R, t));
+    constexpr F s = DIR < 0 ? -s0 : s0;
+    return {c * a.x - s * a.y, c * a.y + s * a.x};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// In-register DFTs of length R, in place on a[0..R-1].
+
+template <int R, int DIR, typename F>
+struct Dft;
+
+template <int DIR, typename F>
+struct Dft<1, DIR, F> {
+  static __device__ __forceinline__ void run(cx<F>*) {}
+};
+
+template <int DIR, typename F>
+struct Dft<2, DIR, F> {
+  static __device__ __forceinline__ void run(cx<F>* a) {
+    const cx<F> t = a[0];
+    a[0] = t + a[1];
+    a[1] = t - a[1];
+  }
+};
+
+template <int DIR, typename F>
+struct Dft<3, DIR, F> {
+  static __device__ __forceinline__ void run(cx<F>* a) {
+    constexpr F h = F(0.8660254037844386467637232);
+    const cx<F> t1 = a[1] + a[2];
+    const cx<F> t2 = {a[0].x - F(0.5) * t1.x, a[0].y - F(0.5) * t1.y};
+    const cx<F> t3 = mul_mi<DIR>(scale(a[1] - a[2], h));
+    a[0] = a[0] + t1;
+    a[1] = t2 + t3;
+    a[2] = t2 - t3;
+  }
+};
+
+template <int DIR, typename F>
+struct Dft<4, DIR, F> {
+  static __device__ __forceinline__ void run(cx<F>* a) {
+    const cx<F> t0 = a[0] + a[2], t1 = a[0] - a[2];
+    const cx<F> t2 = a[1] + a[3], t3 = mul_mi<DIR>(a[1] - a[3]);
+    a[0] = t0 + t2;
+    a[2] = t0 - t2;
+    a[1] = t1 + t3;
+    a[3] = t1 - t3;
+  }
+};
+
+template <int DIR, typename F>
+struct Dft<5, DIR, F> {
+  static __device__ __forceinline__ void run(cx<F>* a) {
+    constexpr F c1 = F(0.3090169943749474241022934);  // cos(2pi/5)
+    constexpr F c2 = F(0.8090169943749474241022934);  // -cos(4pi/5)
+    constexpr F s1 = F(0.9510565162951535721164393);  // sin(2pi/5)
+    constexpr F s2 = F(0.5877852522924731291687060);  // sin(4pi/5)
+    const cx<F> t1 = a[1] + a[4], t2 = a[2] + a[3];
+    const cx<F> t3 = a[1] - a[4], t4 = a[2] - a[3];
+    const cx<F> a0 = a[0];
+    const cx<F> u1 = {a0.x + c1 * t1.x - c2 * t2.x, a0.y + c1 * t1.y - c2 * t2.y};
+    const cx<F> u2 = {a0.x - c2 * t1.x + c1 * t2.x, a0.y - c2 * t1.y + c1 * t2.y};
+    const cx<F> v1 = mul_mi<DIR>(cx<F>{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y});
+    const cx<F> v2 = mul_mi<DIR>(cx<F>{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y});
+    a[0] = a0 + t1 + t2;
+    a[1] = u1 + v1;
+    a[4] = u1 - v1;
+    a[2] = u2 + v2;
+    a[3] = u2 - v2;
+  }
+};
+
+template <int R, int R1, int IDX, int DIR, typename F>
+__device__ __forceinline__ void apply_rot(cx<F>* c) {
+  if constexpr (IDX < R) {
+    c[IDX] = rot<R, (IDX / R1) * (IDX % R1), DIR>(c[IDX]);
+    apply_rot<R, R1, IDX + 1, DIR, F>(c);
+  }
+}
+
+// Cooley-Tukey split R = R1*R2 inside registers: input m = R2*p + q,
+// output k = k1 + R1*k2.
+template <int R1, int R2, int DIR, typename F>
+__device__ __forceinline__ void dft_ct(cx<F>* a) {
+  constexpr int R = R1 * R2;
+  cx<F> c[R];
+#pragma unroll
+  for (int q = 0; q < R2; ++q) {
+    cx<F> s[R1];
+#pragma unroll
+    for (int p = 0; p < R1; ++p) s[p] = a[R2 * p + q];
+    Dft<R1, DIR, F>::run(s);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) c[q * R1 + k1] = s[k1];
+  }
+  // twiddles exp(DIR*2*pi*i*q*k1/R) on c[q*R1 + k1]
+  apply_rot<R, R1, 0, DIR, F>(c);
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    cx<F> s[R2];
+#pragma unroll
+    for (int q = 0; q < R2; ++q) s[q] = c[q * R1 + k1];
+    Dft<R2, DIR, F>::run(s);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) a[k1 + R1 * k2] = s[k2];
+  }
+}
+
+template <int DIR, typename F>
+struct Dft<6, DIR, F> {
+  static __device__ __forceinline__ void run(cx<F>* a) { dft_ct<3, 2, DIR, F>(a); }
+};
+template <int DIR, typename F>
+struct Dft<8, DIR, F> {
+  static __device__ __forceinline__ void run(cx<F>* a) { dft_ct<4, 2, DIR, F>(a); }
+};
+template <int DIR, typename F>
+struct Dft<10, DIR, F> {
+  static __device__ __forceinline__ void run(cx<F>* a) { dft_ct<5, 2, DIR, F>(a); }
+};
+template <int DIR, typename F>
+struct Dft<12, DIR, F> {
+  static __device__ __forceinline__ void run(cx<F>* a) { dft_ct<4, 3, DIR, F>(a); }
+};
+template <int DIR, typename F>
+struct Dft<16, DIR, F> {
+  static __device__ __forceinline__ void run(cx<F>* a) { dft_ct<4, 4, DIR, F>(a); }
+};
+
+// ---------------------------------------------------------------------------
+// Plan of one line transform (host-built, passed by value in kernel args).
+
+struct LinePlan {
+  int n;                    // transform length
+  int L;                    // threads per line = n / E
+  int np;                   // number of passes
+  int radix[FF_MAXP];       // radix of each pass
+  int ns[FF_MAXP];          // product of the radices before the pass
+  uint32_t nsinv[FF_MAXP];  // ceil(2^32 / ns) for ns > 1 (exact j/ns for j, ns < 2^16)
+  int twstep[FF_MAXP];      // n / (ns * radix): twiddle exponent step
+};
+
+__device__ __forceinline__ int fast_mod(int j, int ns, uint32_t inv) {
+  const int qt = (int)__umulhi((uint32_t)j, inv);
+  return j - qt * ns;
+}
+
+// Radices supported by a given elements-per-thread E.
+template <int E, int R>
+constexpr bool radix_ok() {
+  return R <= E && E % R == 0 && (R == 2 || R == 3 || R == 4 || R == 5 || R == 6 || R == 8 ||
+                                  R == 10 || R == 12 || R == 16);
+}
+
+// Butterflies of one pass on registers (twiddles applied to the inputs).
+template <int E, int R, int DIR, typename F>
+__device__ __forceinline__ void pass_regs(cx<F> (&v)[E], int r0, int L, int ns, uint32_t nsinv, int twstep,
+                                          const cx<F>* __restrict__ tw) {
+  constexpr int Q = E / R;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    cx<F> a[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) a[m] = v[q + m * Q];
+    if (ns > 1) {
+      const int j = r0 + q * L;
+      const int e = fast_mod(j, ns, nsinv) * twstep;
+#pragma unroll
+      for (int m = 1; m < R; ++m) a[m] = twmul<DIR>(a[m], tw[m * e]);
+    }
+    Dft<R, DIR, F>::run(a);
+#pragma unroll
+    for (int m = 0; m < R; ++m) v[q + m * Q] = a[m];
+  }
+}
+
+// Outputs of a pass to their autosorted positions in LDS.
+template <int E, int R, typename F, typename Idx>
+__device__ __forceinline__ void scatter(const cx<F> (&v)[E], cx<F>* lds, const Idx& idx, int r0, int L, int ns,
+                                        uint32_t nsinv) {
+  constexpr int Q = E / R;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = r0 + q * L;
+    const int k = ns > 1 ? fast_mod(j, ns, nsinv) : 0;
+    const int base = (j - k) * R + k;
+#pragma unroll
+    for (int m = 0; m < R; ++m) lds[idx(base + m * ns)] = v[q + m * Q];
+  }
+}
+
+template <int E, typename F, typename Idx>
+__device__ __forceinline__ void gather(cx<F> (&v)[E], const cx<F>* lds, const Idx& idx, int r0, int L) {
+#pragma unroll
+  for (int i = 0; i < E; ++i) v[i] = lds[idx(r0 + i * L)];
+}
+
+template <int E, typename F, typename Idx>
+__device__ __forceinline__ void put_natural(const cx<F> (&v)[E], cx<F>* lds, const Idx& idx, int r0, int L) {
+#pragma unroll
+  for (int i = 0; i < E; ++i) lds[idx(r0 + i * L)] = v[i];
+}
+
+// One pass with compile-time radix R: butterflies, then (unless last) the LDS
+// exchange to natural order for the next pass.
+template <int E, int R, int DIR, typename F, typename Idx>
+__device__ __forceinline__ void run_pass(cx<F> (&v)[E], const LinePlan& lp, int p, const cx<F>* __restrict__ tw,
+                                         cx<F>* lds, const Idx& idx, int r0, bool last) {
+  if constexpr (radix_ok<E, R>()) {
+    pass_regs<E, R, DIR, F>(v, r0, lp.L, lp.ns[p], lp.nsinv[p], lp.twstep[p], tw);
+    if (!last) {
+      __syncthreads();
+      scatter<E, R, F>(v, lds, idx, r0, lp.L, lp.ns[p], lp.nsinv[p]);
+      __syncthreads();
+      gather<E, F>(v, lds, idx, r0, lp.L);
+    }
+  }
+}
+
+// Full line transform.  All threads of the workgroup must call it (it contains
+// workgroup barriers); threads of padding lines simply transform garbage.
+template <int E, int DIR, typename F, typename Idx>
+__device__ __forceinline__ void fft_line(cx<F> (&v)[E], const LinePlan& lp, const cx<F>* __restrict__ tw,
+                                         cx<F>* lds, const Idx& idx, int r0) {
+  for (int p = 0; p < lp.np; ++p) {
+    const bool last = (p + 1 == lp.np);
+    switch (lp.radix[p]) {
+      case 2: run_pass<E, 2, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 3: run_pass<E, 3, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 4: run_pass<E, 4, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 5: run_pass<E, 5, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 6: run_pass<E, 6, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 8: run_pass<E, 8, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 10: run_pass<E, 10, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 12: run_pass<E, 12, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 16: run_pass<E, 16, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
+      default: break;
+    }
+  }
+}
+
+// LDS layouts.  Column tiles: element (line c, position r) at r*T + c plus one
+// pad slot every 2^PS positions.  Row tiles: line-major with a padded pitch.
+struct ColIdx {
+  int T, c, ps;
+  __device__ __forceinline__ int operator()(int r) const { return r * T + c + (r >> ps); }
+};
+struct RowIdx {
+  int base, ps;
+  __device__ __forceinline__ int operator()(int r) const { return base + r + (r >> ps); }
+};
+
+}  // namespace ff

@@ -1,0 +1,72 @@
+// Host-side internals shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fishfry.h"
+#include "ff_kernels.hpp"
+
+namespace ff {
+
+// ---- errors ---------------------------------------------------------------
+ff_status set_error(ff_status st, const std::string& msg);
+ff_status hip_check(hipError_t e, const char* what);
+void clear_error();
+
+// Library-owned device memory for small reductions (one block per device,
+// grown on demand, kept until process exit).  Callers hold scratch_mutex()
+// while the memory is in use.
+ff_status device_scratch(size_t ndoubles, double** out);
+std::mutex& scratch_mutex();
+
+// ---- line transforms --------------------------------------------------------
+// Supported elements-per-thread values (one kernel instantiation unit each).
+constexpr int kEValues[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 15, 16, 20, 24, 30};
+
+struct LineCfg {
+  int n = 0;
+  int E = 0;
+  LinePlan lp{};
+};
+
+// Chooses E and a radix sequence for a length-n transform; false when n has a
+// prime factor above 5 or needs more than FF_MAX_THREADS threads per line.
+bool plan_line(int n, LineCfg& out, int forceE = 0);
+
+// exp(-2 pi i t / n) in long double with exact integer argument reduction.
+void twiddle_ld(long long t, long long n, long double& c, long double& s);
+
+struct ColCfg {
+  LineCfg line;
+  int T = 1, lgT = 0, ps = 30;
+  size_t lds = 0;
+  int threads = 0;
+};
+
+struct RowCfg {
+  LineCfg line;  // transform length nz/2 (even nz) or nz (odd nz)
+  int T = 1, ps = 30, pitch = 0;
+  size_t lds = 0;
+  int threads = 0;
+  uint32_t Linv = 0;
+};
+
+bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int forceE = 0, int forceT = 0);
+bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int forceE = 0, int forceT = 0);
+
+hipError_t launch_col(int E, int f32, int mode, const Launch& l, const ColArgs& a);
+hipError_t launch_r2c(int E, int f32, int in_f32, int odd, const Launch& l, const RowArgs& a);
+hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, const Launch& l, const RowArgs& a);
+
+// ---- driver kernels ---------------------------------------------------------
+ff_status field_check(const ff_field* f);
+ff_status field_init(const ff_field* f, double* out, hipStream_t s);
+ff_status field_diff(const ff_field* f, const double* phi, double out[3], hipStream_t s);
+
+}  // namespace ff

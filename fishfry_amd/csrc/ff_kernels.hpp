@@ -1,0 +1,319 @@
+// Pass kernels of the periodic Poisson solve.
+//
+// Data layout in HBM (all passes): the half spectrum is complex [i][j][kh],
+// kh fastest, row pitch nh = nz/2+1 -- the layout the reference has after its
+// D2Z (ParisPencil.cpp:224) -- so no pass ever transposes.  The z passes work
+// on contiguous rows; the y and x passes work on strided columns, a workgroup
+// owning a tile of T neighbouring kh columns so every row of the tile is one
+// contiguous T*sizeof(complex) segment in HBM.
+//
+//   k_r2c   z forward, real rows -> half-spectrum rows       (ParisPencil.cpp:224)
+//   k_col   y forward / inverse, or x forward*filter*inverse (:271, :318-347, :393)
+//   k_c2r   z inverse + the 1/(nx*ny*nz) scale                (:439, :466-481)
+#pragma once
+
+#include "ff_fft.hpp"
+
+namespace ff {
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
+// XCD walks a contiguous range of tiles (neighbouring tiles share L2 lines).
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int x = b & 7, s = b >> 3;
+  return x * q + (x < r ? x : r) + s;
+}
+
+__device__ __forceinline__ double sqr(double x) { return x * x; }
+
+struct ColArgs {
+  const void* src;
+  void* dst;
+  long long s_outer;  // complex elements between planes
+  long long s_r;      // complex elements between transform positions
+  int ncols;          // columns per plane (nh)
+  int nouter;         // planes
+  int T, lgT;         // columns per workgroup (power of two)
+  int tiles;          // tiles per plane
+  int nblocks;        // nouter * tiles
+  int ps;             // LDS pad shift
+  int nx, ny;         // extents for the spectral symbol (x pass)
+  double ddi, ddj, ddk;
+  double* dc_out;  // SYMBOL mode: receives the forward (0,0,0) term (sum of the input)
+  const void* tw;
+  LinePlan lp;
+};
+
+// MODE: 0 forward, 1 inverse, 2 forward * filter * inverse (the solve),
+//       3 forward * symbol * inverse (applies L_h for a residual).
+enum { COL_FWD = 0, COL_INV = 1, COL_FILTER = 2, COL_SYMBOL = 3 };
+
+template <typename F, int E, int MODE>
+__global__ __launch_bounds__(FF_MAX_THREADS) void k_col(const ColArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int c = tid & (a.T - 1);
+  const int r0 = tid >> a.lgT;
+  const int L = a.lp.L;
+  const int b = xcd_remap(blockIdx.x, a.nblocks);
+  const int outer = b / a.tiles;
+  const int tile = b - outer * a.tiles;
+  const int col = tile * a.T + c;
+  const bool valid = col < a.ncols;
+  const long long base = (long long)outer * a.s_outer + col;
+  // src may equal dst (in-place pass): every thread stores exactly the
+  // elements it loaded, after all of its loads.
+  const cx<F>* src = reinterpret_cast<const cx<F>*>(a.src) + base;
+  cx<F>* dst = reinterpret_cast<cx<F>*>(a.dst) + base;
+  const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
+
+  cx<F> v[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    if (valid) v[i] = src[(long long)(r0 + i * L) * a.s_r];
+    else v[i] = cx<F>{F(0), F(0)};
+  }
+  const ColIdx idx{a.T, c, a.ps};
+  if constexpr (MODE == COL_FWD) {
+    fft_line<E, -1, F>(v, a.lp, tw, lds, idx, r0);
+  } else if constexpr (MODE == COL_INV) {
+    fft_line<E, +1, F>(v, a.lp, tw, lds, idx, r0);
+  } else {
+    fft_line<E, -1, F>(v, a.lp, tw, lds, idx, r0);
+    // spectral filter of ParisPencil.cpp:328-343 at (i, j, kh) = (r, outer, col)
+    const int jj = outer, kk = col;
+    const double j2 = sqr(double(jj < a.ny - jj ? jj : a.ny - jj) * a.ddj);
+    const double k2 = sqr(double(kk) * a.ddk);
+    if constexpr (MODE == COL_SYMBOL) {
+      if (a.dc_out && valid && r0 == 0 && jj == 0 && kk == 0) *a.dc_out = double(v[0].x);
+    }
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int ii = r0 + i * L;
+      const double i2 = sqr(double(ii < a.nx - ii ? ii : a.nx - ii) * a.ddi);
+      double d;
+      if constexpr (MODE == COL_FILTER) d = (ii | jj | kk) ? -1.0 / (i2 + j2 + k2) : 0.0;
+      else d = -(i2 + j2 + k2);
+      const F df = F(d);
+      v[i] = cx<F>{df * v[i].x, df * v[i].y};
+    }
+    fft_line<E, +1, F>(v, a.lp, tw, lds, idx, r0);
+  }
+  if (valid) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) dst[(long long)(r0 + i * L) * a.s_r] = v[i];
+  }
+}
+
+struct RowArgs {
+  const void* src;
+  void* dst;
+  const void* aux;  // f for the residual epilogue
+  long long nrows;
+  int nz, nh;
+  int T;          // rows per workgroup
+  uint32_t Linv;  // ceil(2^32 / L)
+  int nblocks;
+  int ps, pitch;  // LDS pad shift and row pitch (elements)
+  double scale;   // 1/(nx*ny*nz)
+  double shift;   // subtracted by the accumulate epilogue (times *shiftp if set)
+  const double* shiftp;
+  const void* tw;      // W_n of the transform length
+  const void* twpost;  // exp(-2 pi i k / nz), k < nz/2 (even nz)
+  LinePlan lp;
+};
+
+template <typename T>
+__device__ __forceinline__ double to_d(T x) { return double(x); }
+
+// z forward: real rows (IN precision) -> half-spectrum rows (F precision).
+// Even nz: pack (x[2r], x[2r+1]) as one complex, an nz/2-point transform, then
+// split the even/odd spectra.  Odd nz: a full nz-point transform of (x, 0).
+template <typename F, typename IN, int E, bool ODD>
+__global__ __launch_bounds__(FF_MAX_THREADS) void k_r2c(const RowArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int L = a.lp.L;
+  const int line = L == 1 ? tid : (int)__umulhi((uint32_t)tid, a.Linv);
+  const int r0 = tid - line * L;
+  const long long row = (long long)xcd_remap(blockIdx.x, a.nblocks) * a.T + line;
+  const bool valid = row < a.nrows;
+  const int M = a.lp.n;
+  const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
+  const RowIdx idx{line * a.pitch, a.ps};
+  cx<F> v[E];
+  if constexpr (!ODD) {
+    const IN* __restrict__ src = reinterpret_cast<const IN*>(a.src) + row * a.nz;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      if (valid) {
+        const int k = r0 + i * L;
+        if constexpr (sizeof(IN) == 8) {
+          const cx<double> p = reinterpret_cast<const cx<double>*>(src)[k];
+          v[i] = cx<F>{F(p.x), F(p.y)};
+        } else {
+          const cx<float> p = reinterpret_cast<const cx<float>*>(src)[k];
+          v[i] = cx<F>{F(p.x), F(p.y)};
+        }
+      } else {
+        v[i] = cx<F>{F(0), F(0)};
+      }
+    }
+    fft_line<E, -1, F>(v, a.lp, tw, lds, idx, r0);
+    __syncthreads();
+    put_natural<E, F>(v, lds, idx, r0, L);
+    __syncthreads();
+    const cx<F>* __restrict__ twp = reinterpret_cast<const cx<F>*>(a.twpost);
+    cx<F>* __restrict__ dst = reinterpret_cast<cx<F>*>(a.dst) + row * a.nh;
+    if (valid) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int k = r0 + i * L;
+        const cx<F> zk = lds[idx(k)];
+        const cx<F> zm = conj(lds[idx(k ? M - k : 0)]);
+        // even spectrum (zk + zm)/2, odd spectrum (zk - zm)/(2i)
+        const cx<F> ev = {F(0.5) * (zk.x + zm.x), F(0.5) * (zk.y + zm.y)};
+        const cx<F> od = {F(0.5) * (zk.y - zm.y), F(-0.5) * (zk.x - zm.x)};
+        dst[k] = ev + twmul<-1>(od, twp[k]);
+      }
+      if (r0 == 0) {
+        const cx<F> z0 = lds[idx(0)];
+        dst[M] = cx<F>{z0.x - z0.y, F(0)};
+      }
+    }
+  } else {
+    const IN* __restrict__ src = reinterpret_cast<const IN*>(a.src) + row * a.nz;
+#pragma unroll
+    for (int i = 0; i < E; ++i) v[i] = cx<F>{valid ? F(src[r0 + i * L]) : F(0), F(0)};
+    fft_line<E, -1, F>(v, a.lp, tw, lds, idx, r0);
+    cx<F>* __restrict__ dst = reinterpret_cast<cx<F>*>(a.dst) + row * a.nh;
+    if (valid) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int k = r0 + i * L;
+        if (k < a.nh) dst[k] = v[i];
+      }
+    }
+  }
+}
+
+// Epilogues of the z inverse: store scale*x, store the residual f - scale*x,
+// or accumulate out += scale*x - shift.
+enum { EPI_STORE = 0, EPI_RESIDUAL = 1, EPI_ACCUM = 2 };
+
+template <int EPI, typename OUT>
+__device__ __forceinline__ void c2r_out1(OUT* out, const double* f, long long off, double val, double scale,
+                                         double shift) {
+  if constexpr (EPI == EPI_STORE) {
+    out[off] = OUT(scale * val);
+  } else if constexpr (EPI == EPI_RESIDUAL) {
+    out[off] = OUT(f[off] - scale * val);
+  } else {
+    out[off] = OUT(double(out[off]) + (scale * val - shift));
+  }
+}
+
+// Two neighbouring outputs (2r, 2r+1) with one vector access.
+template <int EPI, typename OUT>
+__device__ __forceinline__ void c2r_out2(OUT* out, const double* f, long long r, double v0, double v1,
+                                         double scale, double shift) {
+  cx<OUT>* o = reinterpret_cast<cx<OUT>*>(out) + r;
+  if constexpr (EPI == EPI_STORE) {
+    *o = cx<OUT>{OUT(scale * v0), OUT(scale * v1)};
+  } else if constexpr (EPI == EPI_RESIDUAL) {
+    const cx<double> fp = reinterpret_cast<const cx<double>*>(f)[r];
+    *o = cx<OUT>{OUT(fp.x - scale * v0), OUT(fp.y - scale * v1)};
+  } else {
+    const cx<OUT> p = *o;
+    *o = cx<OUT>{OUT(double(p.x) + (scale * v0 - shift)), OUT(double(p.y) + (scale * v1 - shift))};
+  }
+}
+
+// z inverse: half-spectrum rows (F) -> real rows (OUT) with an epilogue.
+// The imaginary parts of the DC and (even nz) Nyquist terms are ignored,
+// i.e. the rows are read as Hermitian half spectra.
+template <typename F, typename OUT, int E, bool ODD, int EPI>
+__global__ __launch_bounds__(FF_MAX_THREADS) void k_c2r(const RowArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int L = a.lp.L;
+  const int line = L == 1 ? tid : (int)__umulhi((uint32_t)tid, a.Linv);
+  const int r0 = tid - line * L;
+  const long long row = (long long)xcd_remap(blockIdx.x, a.nblocks) * a.T + line;
+  const bool valid = row < a.nrows;
+  const int M = a.lp.n;
+  const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
+  const RowIdx idx{line * a.pitch, a.ps};
+  const cx<F>* __restrict__ src = reinterpret_cast<const cx<F>*>(a.src) + row * a.nh;
+  const int nload = ODD ? a.nh : M + 1;
+  // stage the half spectrum in LDS
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int k = r0 + i * L;
+    if (k < nload) lds[idx(k)] = valid ? src[k] : cx<F>{F(0), F(0)};
+  }
+  if (!ODD && r0 == 0) lds[idx(M)] = valid ? src[M] : cx<F>{F(0), F(0)};
+  __syncthreads();
+  cx<F> v[E];
+  if constexpr (!ODD) {
+    const cx<F>* __restrict__ twp = reinterpret_cast<const cx<F>*>(a.twpost);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int k = r0 + i * L;
+      cx<F> xk = lds[idx(k)];
+      cx<F> xm = lds[idx(M - k)];
+      if (k == 0) {
+        xk.y = F(0);
+        xm.y = F(0);
+      }
+      const cx<F> cm = conj(xm);
+      const cx<F> A = xk + cm;
+      const cx<F> B = twmul<+1>(xk - cm, twp[k]);
+      v[i] = cx<F>{A.x - B.y, A.y + B.x};
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int k = r0 + i * L;
+      cx<F> y = k < a.nh ? lds[idx(k)] : conj(lds[idx(a.nz - k)]);
+      if (k == 0) y.y = F(0);
+      v[i] = y;
+    }
+  }
+  fft_line<E, +1, F>(v, a.lp, tw, lds, idx, r0);
+  if (valid) {
+    const double shift = a.shiftp ? a.shift * *a.shiftp : a.shift;
+    OUT* out = reinterpret_cast<OUT*>(a.dst) + row * a.nz;
+    const double* f = reinterpret_cast<const double*>(a.aux) + (EPI == EPI_RESIDUAL ? row * a.nz : 0);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int r = r0 + i * L;
+      if constexpr (!ODD) {
+        c2r_out2<EPI, OUT>(out, f, r, to_d(v[i].x), to_d(v[i].y), a.scale, shift);
+      } else {
+        c2r_out1<EPI, OUT>(out, f, r, to_d(v[i].x), a.scale, shift);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers: one explicit instantiation unit per E (ff_inst_E*.hip).
+
+struct Launch {
+  dim3 grid, block;
+  size_t lds;
+  hipStream_t stream;
+};
+
+template <int E>
+struct PassLaunchers {
+  static hipError_t col(int f32, int mode, const Launch& l, const ColArgs& a);
+  static hipError_t r2c(int f32, int in_f32, int odd, const Launch& l, const RowArgs& a);
+  static hipError_t c2r(int f32, int out_kind, int odd, int epi, const Launch& l, const RowArgs& a);
+};
+
+}  // namespace ff

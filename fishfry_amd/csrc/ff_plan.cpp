@@ -1,0 +1,276 @@
+// Host planner: radix plans, tile shapes, twiddles, kernel dispatch and the
+// library's error / scratch bookkeeping.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+
+#include "ff_internal.hpp"
+
+namespace ff {
+
+// ---- errors ---------------------------------------------------------------
+
+static thread_local std::string g_last_error;
+
+ff_status set_error(ff_status st, const std::string& msg) {
+  g_last_error = msg;
+  return st;
+}
+
+void clear_error() { g_last_error.clear(); }
+
+ff_status hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return FF_OK;
+  char buf[256];
+  snprintf(buf, sizeof buf, "%s: %s (%s)", what, hipGetErrorName(e), hipGetErrorString(e));
+  return set_error(e == hipErrorOutOfMemory ? FF_ERR_NOMEM : FF_ERR_HIP, buf);
+}
+
+const char* last_error_text() { return g_last_error.c_str(); }
+
+// ---- scratch ----------------------------------------------------------------
+
+std::mutex& scratch_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+ff_status device_scratch(size_t ndoubles, double** out) {
+  // per device, grown on demand; callers serialize through scratch_mutex()
+  static std::map<int, std::pair<double*, size_t>> blocks;
+  int dev = 0;
+  ff_status st = hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (st != FF_OK) return st;
+  auto& b = blocks[dev];
+  if (b.second < ndoubles) {
+    if (b.first) {
+      st = hip_check(hipDeviceSynchronize(), "scratch sync");
+      if (st != FF_OK) return st;
+      (void)hipFree(b.first);
+      b.first = nullptr;
+      b.second = 0;
+    }
+    st = hip_check(hipMalloc(&b.first, ndoubles * sizeof(double)), "scratch hipMalloc");
+    if (st != FF_OK) return st;
+    b.second = ndoubles;
+  }
+  *out = b.first;
+  return FF_OK;
+}
+
+// ---- twiddles ---------------------------------------------------------------
+
+void twiddle_ld(long long t, long long n, long double& c, long double& s) {
+  // exp(-2 pi i t/n).  4t = q n + r (0 <= r < n): the angle is q quarter turns
+  // plus (pi/2)(r/n); for r/n > 1/2 the complementary angle is used so the
+  // long-double argument stays below pi/4 and mirrored points agree exactly.
+  t %= n;
+  if (t < 0) t += n;
+  const long long v = 4 * t;
+  const long long q = v / n;
+  const long long r = v - q * n;
+  const long double hp = 1.5707963267948966192313216916397514L;  // pi/2
+  long double cb, sb;
+  if (2 * r <= n) {
+    const long double a = hp * ((long double)r / (long double)n);
+    cb = cosl(a);
+    sb = sinl(a);
+  } else {
+    const long double a = hp * ((long double)(n - r) / (long double)n);
+    cb = sinl(a);
+    sb = cosl(a);
+  }
+  long double cc, ss;
+  switch (q & 3) {
+    case 0: cc = cb; ss = sb; break;
+    case 1: cc = -sb; ss = cb; break;
+    case 2: cc = -cb; ss = -sb; break;
+    default: cc = sb; ss = -cb; break;
+  }
+  c = cc;
+  s = -ss;
+}
+
+// ---- radix plans ------------------------------------------------------------
+
+static const int kRadices[] = {16, 12, 10, 8, 6, 5, 4, 3, 2};
+
+// Minimum number of passes to factor m with radices dividing E; fills seq.
+static int min_passes(int m, int E, std::map<int, std::pair<int, int>>& memo) {
+  if (m == 1) return 0;
+  auto it = memo.find(m);
+  if (it != memo.end()) return it->second.first;
+  int best = 1 << 20, bestR = 0;
+  for (int R : kRadices) {
+    if (R > E || E % R || m % R) continue;
+    const int p = min_passes(m / R, E, memo);
+    if (p + 1 < best) {
+      best = p + 1;
+      bestR = R;
+    }
+  }
+  memo[m] = {best, bestR};
+  return best;
+}
+
+static bool radix_seq(int n, int E, std::vector<int>& seq) {
+  std::map<int, std::pair<int, int>> memo;
+  const int p = min_passes(n, E, memo);
+  if (p >= (1 << 20) || p > FF_MAXP) return false;
+  seq.clear();
+  int m = n;
+  while (m > 1) {
+    const int R = memo[m].second;
+    seq.push_back(R);
+    m /= R;
+  }
+  // largest radix first: the first pass needs no twiddles
+  std::sort(seq.begin(), seq.end(), [](int a, int b) { return a > b; });
+  return true;
+}
+
+static void fill_plan(int n, int E, const std::vector<int>& seq, LineCfg& out) {
+  out.n = n;
+  out.E = E;
+  LinePlan& lp = out.lp;
+  memset(&lp, 0, sizeof lp);
+  lp.n = n;
+  lp.L = n / E;
+  lp.np = (int)seq.size();
+  int ns = 1;
+  for (int p = 0; p < lp.np; ++p) {
+    lp.radix[p] = seq[p];
+    lp.ns[p] = ns;
+    lp.nsinv[p] = ns > 1 ? (uint32_t)(((1ull << 32) + (uint64_t)ns - 1) / (uint64_t)ns) : 0u;
+    lp.twstep[p] = n / (ns * seq[p]);
+    ns *= seq[p];
+  }
+}
+
+bool plan_line(int n, LineCfg& out, int forceE) {
+  if (n < 1) return false;
+  if (n == 1) {
+    fill_plan(1, 1, {}, out);
+    return true;
+  }
+  double bestCost = 1e30;
+  int bestE = 0;
+  std::vector<int> bestSeq;
+  for (int E : kEValues) {
+    if (E == 1) continue;
+    if (forceE && E != forceE) continue;
+    if (n % E || n / E > FF_MAX_THREADS) continue;
+    std::vector<int> seq;
+    if (!radix_seq(n, E, seq)) continue;
+    const double cost = (double)seq.size() + 0.04 * E;
+    if (cost < bestCost) {
+      bestCost = cost;
+      bestE = E;
+      bestSeq = seq;
+    }
+  }
+  if (!bestE) return false;
+  fill_plan(n, bestE, bestSeq, out);
+  return true;
+}
+
+static int floor_pow2(int v) {
+  int p = 1;
+  while (p * 2 <= v) p *= 2;
+  return p;
+}
+
+bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int forceE, int forceT) {
+  if (!plan_line(n, out.line, forceE)) return false;
+  const int L = out.line.lp.L;
+  int T = forceT ? forceT : floor_pow2(FF_MAX_THREADS / L);
+  if (!forceT) {
+    // no wider than the (power-of-two rounded) number of columns
+    int cap = 1;
+    while (cap < ncols) cap *= 2;
+    if (T > cap) T = cap;
+  }
+  if (T < 1 || (T & (T - 1)) || T * L > FF_MAX_THREADS) return false;
+  out.T = T;
+  out.lgT = 0;
+  while ((1 << out.lgT) < T) out.lgT++;
+  out.ps = 4;
+  out.threads = T * L;
+  const size_t elems = (size_t)n * T + ((size_t)(n - 1) >> out.ps) + 1;
+  out.lds = elems * (size_t)elem_bytes;
+  return out.lds <= 160 * 1024;
+}
+
+bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int forceE, int forceT) {
+  const int M = (nz % 2 == 0) ? nz / 2 : nz;
+  if (!plan_line(M, out.line, forceE)) return false;
+  const int L = out.line.lp.L;
+  int T = forceT ? forceT : FF_MAX_THREADS / L;
+  if (T < 1 || T * L > FF_MAX_THREADS) return false;
+  out.T = T;
+  out.ps = 4;
+  out.pitch = (M + 1) + (M >> out.ps) + 1;
+  out.threads = T * L;
+  out.Linv = L > 1 ? (uint32_t)(((1ull << 32) + (uint64_t)L - 1) / (uint64_t)L) : 0u;
+  out.lds = (size_t)T * out.pitch * elem_bytes;
+  return out.lds <= 160 * 1024;
+}
+
+// ---- dispatch ---------------------------------------------------------------
+
+#define FF_DECL(E)                                                                                         \
+  template <>                                                                                              \
+  hipError_t PassLaunchers<E>::col(int, int, const Launch&, const ColArgs&);                               \
+  template <>                                                                                              \
+  hipError_t PassLaunchers<E>::r2c(int, int, int, const Launch&, const RowArgs&);                          \
+  template <>                                                                                              \
+  hipError_t PassLaunchers<E>::c2r(int, int, int, int, const Launch&, const RowArgs&);
+FF_DECL(1)
+FF_DECL(2)
+FF_DECL(3)
+FF_DECL(4)
+FF_DECL(5)
+FF_DECL(6)
+FF_DECL(8)
+FF_DECL(10)
+FF_DECL(12)
+FF_DECL(15)
+FF_DECL(16)
+FF_DECL(20)
+FF_DECL(24)
+FF_DECL(30)
+#undef FF_DECL
+
+#define FF_SWITCH_E(E, CALL)        \
+  switch (E) {                      \
+    case 1: return PassLaunchers<1>::CALL;   \
+    case 2: return PassLaunchers<2>::CALL;   \
+    case 3: return PassLaunchers<3>::CALL;   \
+    case 4: return PassLaunchers<4>::CALL;   \
+    case 5: return PassLaunchers<5>::CALL;   \
+    case 6: return PassLaunchers<6>::CALL;   \
+    case 8: return PassLaunchers<8>::CALL;   \
+    case 10: return PassLaunchers<10>::CALL; \
+    case 12: return PassLaunchers<12>::CALL; \
+    case 15: return PassLaunchers<15>::CALL; \
+    case 16: return PassLaunchers<16>::CALL; \
+    case 20: return PassLaunchers<20>::CALL; \
+    case 24: return PassLaunchers<24>::CALL; \
+    case 30: return PassLaunchers<30>::CALL; \
+    default: return hipErrorInvalidValue;    \
+  }
+
+hipError_t launch_col(int E, int f32, int mode, const Launch& l, const ColArgs& a) {
+  FF_SWITCH_E(E, col(f32, mode, l, a))
+}
+hipError_t launch_r2c(int E, int f32, int in_f32, int odd, const Launch& l, const RowArgs& a) {
+  FF_SWITCH_E(E, r2c(f32, in_f32, odd, l, a))
+}
+hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, const Launch& l, const RowArgs& a) {
+  FF_SWITCH_E(E, c2r(f32, out_kind, odd, epi, l, a))
+}
+#undef FF_SWITCH_E
+
+}  // namespace ff

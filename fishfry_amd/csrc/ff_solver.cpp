@@ -1,0 +1,304 @@
+// Plan construction and the pass sequence of the periodic Poisson solve.
+//
+// At 1x1x1 tasks the reference's 25 steps (ParisPencil.cpp:154-484) are 6 FFT
+// executions, one filter, one scale and 17 reorders / self all-to-alls.  Here
+// they are five kernels, each reading and writing the whole array once:
+//   z forward (D2Z, :224), y forward (:271), x forward*filter*inverse
+//   (:318,:322-344,:347), y inverse (:393), z inverse (Z2D, :439) with the
+//   1/(nx*ny*nz) scale of :466-481 in its epilogue.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "ff_solver.hpp"
+
+namespace ff {
+
+size_t reference_bytes(const int n[3], const int m[3]) {
+  // ParisPencil.cpp:77 (nh), :83-85 (mp x mq split of mk), :110-134
+  const long nh = n[2] / 2 + 1;
+  const long mi = m[0], mj = m[1], mk = m[2];
+  long mq = std::lround(std::sqrt((double)mk));
+  while (mk % mq) mq--;
+  const long mp = mk / mq;
+  const long dh = (nh + mk - 1) / mk;
+  const long di = (n[0] + mi - 1) / mi;
+  const long dj = (n[1] + mj - 1) / mj;
+  const long dk = (n[2] + mk - 1) / mk;
+  const long dip = (di + mp - 1) / mp;
+  const long djq = (dj + mq - 1) / mq;
+  const long mjq = mj * mq;
+  const long dhq = (nh + mjq - 1) / mjq;
+  const long mip = mi * mp;
+  const long djp = (n[1] + mip - 1) / mip;
+  const long nmax = std::max({di * dj * dk, mp * mq * dip * djq * dk, 2 * dip * djq * mk * dh,
+                              2 * dip * mp * djq * mq * dh, 2 * dip * djq * mjq * dhq, 2 * dip * dhq * mip * djp,
+                              2 * djp * dhq * mip * dip});
+  return (size_t)nmax * sizeof(double);
+}
+
+static double dd_const(int n, double lo, double hi) {
+  // ParisPencil.cpp:68-70, same operand order
+  return 2.0 * M_PI * double(n - 1) / (double(n) * (hi - lo));
+}
+
+// Append the table W_n[t] = exp(-2 pi i t/n), t < count, in precision prec.
+static void push_table(std::vector<unsigned char>& buf, long long n, long long count, int prec, size_t& off) {
+  const size_t elem = prec ? 8 : 16;
+  off = (buf.size() + 255) / 256 * 256;
+  buf.resize(off + (size_t)count * elem);
+  for (long long t = 0; t < count; ++t) {
+    long double c, s;
+    twiddle_ld(t, n, c, s);
+    if (prec) {
+      const float v[2] = {(float)c, (float)s};
+      memcpy(&buf[off + t * elem], v, sizeof v);
+    } else {
+      const double v[2] = {(double)c, (double)s};
+      memcpy(&buf[off + t * elem], v, sizeof v);
+    }
+  }
+}
+
+ff_status plan_create(const int n[3], const double lo[3], const double hi[3], const ff_options* opts,
+                      ff_plan** out) {
+  *out = nullptr;
+  for (int a = 0; a < 3; ++a) {
+    if (n[a] < 1) return set_error(FF_ERR_INVALID, "grid extents must be >= 1");
+    if (!(hi[a] > lo[a])) return set_error(FF_ERR_INVALID, "hi must exceed lo on every axis");
+  }
+  if (n[0] > 65535 || n[1] > 65535) return set_error(FF_ERR_INVALID, "nx, ny must be <= 65535");
+  ff_options o{FF_FP64, 0, 0};
+  if (opts) o = *opts;
+  if (o.precision != FF_FP64 && o.precision != FF_FP32_REFINE)
+    return set_error(FF_ERR_INVALID, "options.precision must be FF_FP64 or FF_FP32_REFINE");
+  if (o.refine_steps < 0 || o.refine_steps > 8) return set_error(FF_ERR_INVALID, "refine_steps must be 0..8");
+  if (o.flags & ~FF_PRESERVE_INPUT) return set_error(FF_ERR_INVALID, "unknown bits in options.flags");
+
+  ff_plan* p = new (std::nothrow) ff_plan();
+  if (!p) return set_error(FF_ERR_NOMEM, "host allocation failed");
+  memcpy(p->n, n, sizeof p->n);
+  memcpy(p->lo, lo, sizeof p->lo);
+  memcpy(p->hi, hi, sizeof p->hi);
+  p->opt = o;
+  p->nh = n[2] / 2 + 1;
+  p->N = (long long)n[0] * n[1] * n[2];
+  p->Nh = (long long)n[0] * n[1] * p->nh;
+  p->ddi = dd_const(n[0], lo[0], hi[0]);
+  p->ddj = dd_const(n[1], lo[1], hi[1]);
+  p->ddk = dd_const(n[2], lo[2], hi[2]);
+  const int ones[3] = {1, 1, 1};
+  p->bytes = reference_bytes(n, ones);
+
+  ff_status st = hip_check(hipGetDevice(&p->device), "hipGetDevice");
+  if (st != FF_OK) {
+    delete p;
+    return st;
+  }
+  const int nprec = (o.precision == FF_FP64) ? 1 : 2;
+  for (int prec = 0; prec < nprec; ++prec) {
+    const int eb = prec ? 8 : 16;
+    if (!make_row_cfg(n[2], eb, p->zr[prec]) || !make_col_cfg(n[1], p->nh, eb, p->cy[prec]) ||
+        !make_col_cfg(n[0], p->nh, eb, p->cx[prec])) {
+      delete p;
+      char buf[160];
+      snprintf(buf, sizeof buf,
+               "no transform plan for %dx%dx%d: every extent (nz/2 for even nz) must be 2^a 3^b 5^c and "
+               "<= 4096",
+               n[0], n[1], n[2]);
+      return set_error(FF_ERR_UNSUPPORTED, buf);
+    }
+  }
+
+  // twiddle tables
+  std::vector<unsigned char> buf;
+  size_t oz[2], ozp[2], oy[2], ox[2];
+  const int nz = n[2];
+  const bool odd = nz % 2;
+  for (int prec = 0; prec < nprec; ++prec) {
+    const long long M = odd ? nz : nz / 2;
+    push_table(buf, M, M, prec, oz[prec]);
+    push_table(buf, nz, odd ? 1 : M, prec, ozp[prec]);
+    push_table(buf, n[1], n[1], prec, oy[prec]);
+    push_table(buf, n[0], n[0], prec, ox[prec]);
+  }
+  st = hip_check(hipMalloc(&p->tw_mem, buf.size()), "twiddle hipMalloc");
+  if (st == FF_OK) st = hip_check(hipMemcpy(p->tw_mem, buf.data(), buf.size(), hipMemcpyHostToDevice), "twiddle copy");
+  if (st != FF_OK) {
+    plan_destroy(p);
+    return st;
+  }
+  for (int prec = 0; prec < nprec; ++prec) {
+    unsigned char* b = static_cast<unsigned char*>(p->tw_mem);
+    p->tw_z[prec] = b + oz[prec];
+    p->tw_zpost[prec] = b + ozp[prec];
+    p->tw_y[prec] = b + oy[prec];
+    p->tw_x[prec] = b + ox[prec];
+  }
+  if (o.precision == FF_FP32_REFINE) {
+    const size_t spec = (size_t)p->Nh * 16;
+    const size_t res = ((size_t)p->N * 4 + 255) / 256 * 256;
+    p->scratch_bytes = spec + res + 256;
+    st = hip_check(hipMalloc(&p->scratch, p->scratch_bytes), "refinement work area hipMalloc");
+    if (st != FF_OK) {
+      plan_destroy(p);
+      return st;
+    }
+    unsigned char* b = static_cast<unsigned char*>(p->scratch);
+    p->spec = b;
+    p->res = reinterpret_cast<float*>(b + spec);
+    p->dc = reinterpret_cast<double*>(b + spec + res);
+  }
+  if (o.precision == FF_FP64 && (o.flags & FF_PRESERVE_INPUT)) {
+    st = hip_check(hipMalloc(&p->work, (size_t)p->Nh * 16), "work array hipMalloc");
+    if (st != FF_OK) {
+      plan_destroy(p);
+      return st;
+    }
+  }
+  *out = p;
+  return FF_OK;
+}
+
+void plan_destroy(ff_plan* p) {
+  if (!p) return;
+  if (p->tw_mem) (void)hipFree(p->tw_mem);
+  if (p->scratch) (void)hipFree(p->scratch);
+  if (p->work) (void)hipFree(p->work);
+  delete p;
+}
+
+// ---- passes -------------------------------------------------------------------
+
+static RowArgs row_args(const ff_plan* p, int prec) {
+  const RowCfg& c = p->zr[prec];
+  RowArgs a;
+  memset(&a, 0, sizeof a);
+  a.nrows = (long long)p->n[0] * p->n[1];
+  a.nz = p->n[2];
+  a.nh = p->nh;
+  a.T = c.T;
+  a.Linv = c.Linv;
+  a.nblocks = (int)((a.nrows + c.T - 1) / c.T);
+  a.ps = c.ps;
+  a.pitch = c.pitch;
+  a.scale = 1.0 / (double(p->n[0]) * double(p->n[1]) * double(p->n[2]));  // ParisPencil.cpp:467
+  a.shift = 0.0;
+  a.shiftp = nullptr;
+  a.tw = p->tw_z[prec];
+  a.twpost = p->tw_zpost[prec];
+  a.lp = c.line.lp;
+  return a;
+}
+
+ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void* dst, hipStream_t s) {
+  RowArgs a = row_args(p, prec);
+  a.src = src;
+  a.dst = dst;
+  const RowCfg& c = p->zr[prec];
+  const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
+  return hip_check(launch_r2c(c.line.E, prec, in_f32, p->n[2] % 2, l, a), "z forward launch");
+}
+
+ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void* src, void* dst, const double* f,
+                   const double* shiftp, hipStream_t s) {
+  RowArgs a = row_args(p, prec);
+  a.src = src;
+  a.dst = dst;
+  a.aux = f;
+  if (shiftp) {
+    a.shiftp = shiftp;
+    a.shift = a.scale;  // shift = (sum of phi) / N
+  }
+  const RowCfg& c = p->zr[prec];
+  const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
+  return hip_check(launch_c2r(c.line.E, prec, out_kind, p->n[2] % 2, epi, l, a), "z inverse launch");
+}
+
+ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* src, void* dst, hipStream_t s,
+                  double* dc_out) {
+  const ColCfg& c = axis == 1 ? p->cy[prec] : p->cx[prec];
+  ColArgs a;
+  memset(&a, 0, sizeof a);
+  a.src = src;
+  a.dst = dst;
+  a.ncols = p->nh;
+  if (axis == 1) {  // y: planes i, stride nh   (ParisPencil.cpp:138 plan, batch nx*nh)
+    a.nouter = p->n[0];
+    a.s_outer = (long long)p->n[1] * p->nh;
+    a.s_r = p->nh;
+  } else {  // x: planes j, stride ny*nh       (ParisPencil.cpp:137 plan, batch ny*nh)
+    a.nouter = p->n[1];
+    a.s_outer = p->nh;
+    a.s_r = (long long)p->n[1] * p->nh;
+  }
+  a.T = c.T;
+  a.lgT = c.lgT;
+  a.tiles = (a.ncols + c.T - 1) / c.T;
+  const long long nb = (long long)a.nouter * a.tiles;
+  if (nb > INT_MAX) return set_error(FF_ERR_INVALID, "grid too large");
+  a.nblocks = (int)nb;
+  a.ps = c.ps;
+  a.nx = p->n[0];
+  a.ny = p->n[1];
+  a.ddi = p->ddi;
+  a.ddj = p->ddj;
+  a.ddk = p->ddk;
+  a.dc_out = dc_out;
+  a.tw = axis == 1 ? p->tw_y[prec] : p->tw_x[prec];
+  a.lp = c.line.lp;
+  const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
+  return hip_check(launch_col(c.line.E, prec, mode, l, a), "column pass launch");
+}
+
+static ff_status solve_fp32_refine(const ff_plan* p, const double* f, double* phi, hipStream_t s) {
+  void* spec = p->spec;
+  ff_status st;
+  // phi0: FP32 passes, FP64 in and out
+  if ((st = run_zfwd(p, 1, 0, f, spec, s))) return st;
+  if ((st = run_col(p, 1, 1, COL_FWD, spec, spec, s))) return st;
+  if ((st = run_col(p, 1, 0, COL_FILTER, spec, spec, s))) return st;
+  if ((st = run_col(p, 1, 1, COL_INV, spec, spec, s))) return st;
+  if ((st = run_zinv(p, 1, 0, EPI_STORE, spec, phi, nullptr, nullptr, s))) return st;
+  for (int it = 0; it < p->opt.refine_steps; ++it) {
+    // r = f - L_h phi in FP64, with the same spectral symbol the filter inverts;
+    // the x pass also captures sum(phi) for the FP64 mean removal
+    if ((st = run_zfwd(p, 0, 0, phi, spec, s))) return st;
+    if ((st = run_col(p, 0, 1, COL_FWD, spec, spec, s))) return st;
+    if ((st = run_col(p, 0, 0, COL_SYMBOL, spec, spec, s, p->dc))) return st;
+    if ((st = run_col(p, 0, 1, COL_INV, spec, spec, s))) return st;
+    if ((st = run_zinv(p, 0, 1, EPI_RESIDUAL, spec, p->res, f, nullptr, s))) return st;
+    // correction in FP32: phi += L_h^-1 r - mean(phi)
+    if ((st = run_zfwd(p, 1, 1, p->res, spec, s))) return st;
+    if ((st = run_col(p, 1, 1, COL_FWD, spec, spec, s))) return st;
+    if ((st = run_col(p, 1, 0, COL_FILTER, spec, spec, s))) return st;
+    if ((st = run_col(p, 1, 1, COL_INV, spec, spec, s))) return st;
+    if ((st = run_zinv(p, 1, 0, EPI_ACCUM, spec, phi, nullptr, p->dc, s))) return st;
+  }
+  return FF_OK;
+}
+
+ff_status solve_enqueue(const ff_plan* p, double* density, double* potential, hipStream_t s, hipEvent_t* ev) {
+  if (p->opt.precision == FF_FP32_REFINE) return solve_fp32_refine(p, density, potential, s);
+  ff_status st;
+  // ping-pong density <-> potential (or potential <-> the plan's work array
+  // with FF_PRESERVE_INPUT); the result lands in potential as in
+  // ParisPencil.cpp:162-163,478
+  void* w = p->work ? p->work : (void*)density;
+  if ((st = run_zfwd(p, 0, 0, density, potential, s))) return st;
+  if (ev && (st = hip_check(hipEventRecord(ev[0], s), "event"))) return st;
+  if ((st = run_col(p, 0, 1, COL_FWD, potential, w, s))) return st;
+  if (ev && (st = hip_check(hipEventRecord(ev[1], s), "event"))) return st;
+  if ((st = run_col(p, 0, 0, COL_FILTER, w, potential, s))) return st;
+  if (ev && (st = hip_check(hipEventRecord(ev[2], s), "event"))) return st;
+  if ((st = run_col(p, 0, 1, COL_INV, potential, w, s))) return st;
+  if (ev && (st = hip_check(hipEventRecord(ev[3], s), "event"))) return st;
+  if ((st = run_zinv(p, 0, 0, EPI_STORE, w, potential, nullptr, nullptr, s))) return st;
+  if (ev && (st = hip_check(hipEventRecord(ev[4], s), "event"))) return st;
+  return FF_OK;
+}
+
+}  // namespace ff

@@ -1,0 +1,53 @@
+// The plan object behind the C-ABI handle and the pass sequencer.
+#pragma once
+
+#include "ff_internal.hpp"
+
+struct ff_plan {
+  int n[3];
+  double lo[3], hi[3];
+  int nh;
+  long long N;   // nx*ny*nz
+  long long Nh;  // nx*ny*nh
+  double ddi, ddj, ddk;
+  size_t bytes;
+  ff_options opt;
+  int device;
+  // pass configurations per precision (0 = FP64, 1 = FP32)
+  ff::RowCfg zr[2];
+  ff::ColCfg cy[2], cx[2];
+  const void* tw_z[2];
+  const void* tw_zpost[2];
+  const void* tw_y[2];
+  const void* tw_x[2];
+  void* tw_mem;
+  // FP32 + refinement work area: FP64 spectrum (also hosts the FP32 one),
+  // the FP32 residual and one double for the mean of phi.
+  void* scratch;
+  size_t scratch_bytes;
+  void* spec;   // 16*Nh bytes
+  float* res;   // N floats
+  double* dc;   // 1 double
+  // FP64 with FF_PRESERVE_INPUT: third work array (16*Nh bytes)
+  void* work;
+};
+
+namespace ff {
+
+// Pass kernels over caller-provided buffers.  prec: 0 FP64, 1 FP32.
+ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void* dst, hipStream_t s);
+ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* src, void* dst, hipStream_t s,
+                  double* dc_out = nullptr);
+ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void* src, void* dst, const double* f,
+                   const double* shiftp, hipStream_t s);
+
+// The whole solve; ev (optional) receives one record after each of the five
+// FP64 passes.
+ff_status solve_enqueue(const ff_plan* p, double* density, double* potential, hipStream_t s, hipEvent_t* ev);
+
+ff_status plan_create(const int n[3], const double lo[3], const double hi[3], const ff_options* opts,
+                      ff_plan** out);
+void plan_destroy(ff_plan* p);
+size_t reference_bytes(const int n[3], const int m[3]);
+
+}  // namespace ff

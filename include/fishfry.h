@@ -1,0 +1,149 @@
+/*
+ * fishfry.h -- C-ABI of the MI355X (gfx950) periodic Poisson solver.
+ *
+ * The drop-in boundary for fishfry's solver plugin.  The reference binds its
+ * solver as the template parameter P of FishFry<P> (FishFry.hpp:103-104,
+ * main.cpp:48 instantiates FishFry<ParisPencil>); P must provide a
+ * constructor, bytes(), solve() and a destructor (ParisPencil.hpp:23-44).
+ * Each of those maps onto one entry point here; the C++ facade
+ * fishfry_amd/host/ParisPencil.hpp wraps them back into that class.
+ *
+ * Conventions: plain pointers and sizes only.  Device buffers are HIP device
+ * pointers; a stream is a hipStream_t passed as void* (NULL = the null
+ * stream).  Every entry point returns an ff_status and never exits the
+ * process; ff_last_error() returns the calling thread's last message.
+ */
+#ifndef FISHFRY_H
+#define FISHFRY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FF_ABI_VERSION 1
+
+typedef enum ff_status {
+  FF_OK = 0,
+  FF_ERR_INVALID = 1,     /* bad argument (sizes, null pointers, options)          */
+  FF_ERR_UNSUPPORTED = 2, /* valid for the reference, not supported here (m != 1)  */
+  FF_ERR_BYTES = 3,       /* work arrays smaller than ff_plan_bytes()              */
+  FF_ERR_HIP = 4,         /* a HIP runtime call failed                             */
+  FF_ERR_NOMEM = 5,       /* device or host allocation failed                      */
+  FF_ERR_REJECTED = 6,    /* serving: request does not fit the memory budget      */
+  FF_ERR_TIMEOUT = 7,     /* serving: wait timed out                                */
+  FF_ERR_NOT_FOUND = 8    /* serving: unknown request id                            */
+} ff_status;
+
+typedef enum ff_precision {
+  FF_FP64 = 0,        /* FP64 storage and arithmetic in every pass                */
+  FF_FP32_REFINE = 1  /* FP32 passes, FP64 residual + mean removal, refine_steps
+                         corrections (0 = plain FP32 solve)                         */
+} ff_precision;
+
+/* ff_options.flags */
+#define FF_PRESERVE_INPUT 1 /* FP64: the plan owns a third work array (ff_plan_bytes()
+                               more device memory) and solve leaves density intact;
+                               FP32 modes always preserve density                   */
+
+typedef struct ff_options {
+  int precision;    /* ff_precision                                             */
+  int refine_steps; /* FF_FP32_REFINE only: 0..8                                */
+  int flags;        /* 0 or FF_PRESERVE_INPUT                                   */
+} ff_options;
+
+/* One phase time stamp: the reference's TimeStamp (TimeStamp.hpp:7), a
+ * (name, seconds) pair; seconds are on a steady host clock. */
+typedef struct ff_stamp {
+  char name[64];
+  double seconds;
+} ff_stamp;
+
+#define FF_SOLVE_STAMPS 14 /* stamps written by ff_solve, ParisPencil.cpp:157-483 */
+
+typedef struct ff_plan ff_plan;
+
+/* ParisPencil::ParisPencil(n, lo, hi, m, id) -- ParisPencil.hpp:23,
+ * ParisPencil.cpp:66-141.  n: global cells per axis; lo/hi: physical bounds
+ * (hi is one cell below the periodic end); m: tasks per axis, must be {1,1,1}
+ * (FF_ERR_UNSUPPORTED otherwise); id must be {0,0,0}.  opts may be NULL (FP64).
+ * Builds radix plans and twiddle tables on the current HIP device. */
+ff_status ff_plan_create(const int n[3], const double lo[3], const double hi[3], const int m[3],
+                         const int id[3], const ff_options* opts, ff_plan** out);
+
+/* ParisPencil::bytes() -- ParisPencil.hpp:30, formula ParisPencil.cpp:108-134.
+ * Bytes each of density and potential must hold.  0 for a NULL plan. */
+size_t ff_plan_bytes(const ff_plan* plan);
+
+/* ParisPencil::solve(bytes, density, potential, stamps) -- ParisPencil.hpp:44,
+ * ParisPencil.cpp:154-484.  density: [nx][ny][nz] FP64 input, clobbered
+ * (FP64 mode) or preserved (FP32 modes); potential: receives phi in the same
+ * layout.  bytes < ff_plan_bytes() -> FF_ERR_BYTES.  Work is enqueued on
+ * stream and the call returns after it completes (the reference ends with a
+ * device synchronize, ParisPencil.cpp:482).  stamps may be NULL; otherwise
+ * it receives up to *n_stamps entries (FF_SOLVE_STAMPS with the reference's
+ * phase names) and *n_stamps is set to the count written. */
+ff_status ff_solve(const ff_plan* plan, size_t bytes, double* density, double* potential, void* stream,
+                   ff_stamp* stamps, int* n_stamps);
+
+/* Same as ff_solve but only enqueues the work on stream (no host wait, no
+ * stamps): for callers that overlap solves or capture them in a HIP graph. */
+ff_status ff_solve_async(const ff_plan* plan, size_t bytes, double* density, double* potential, void* stream);
+
+/* ParisPencil::~ParisPencil -- ParisPencil.cpp:143-152.  NULL is a no-op. */
+ff_status ff_plan_destroy(ff_plan* plan);
+
+/* Thread-local text of the last error ("" if none). */
+const char* ff_last_error(void);
+
+/* The reference's per-array byte formula for any task grid m
+ * (ParisPencil.cpp:83-85,108-134), without creating a plan. */
+size_t ff_reference_bytes(const int n[3], const int m[3]);
+
+/* Human-readable JSON description of a plan (radix plans, tiles, bytes). */
+ff_status ff_plan_describe(const ff_plan* plan, char* buf, size_t len);
+
+/* One pass of the solve, for per-pass parity checks and timing:
+ * 0 z forward (real src -> complex dst), 1 y forward, 2 x forward*filter*
+ * inverse, 3 y inverse, 4 z inverse + scale (complex src -> real dst).
+ * src/dst follow the FP64 layouts of ParisPencil.cpp (D2Z output [i][j][kh]). */
+ff_status ff_solve_pass(const ff_plan* plan, int pass, const double* src, double* dst, void* stream);
+
+/* ----------------------------------------------------------------------- */
+/* Driver kernels of FishFry (FishFry.hpp:14-101).                          */
+
+#define FF_MAX_MODES 32
+
+typedef struct ff_field {
+  int n[3];       /* local extents ni, nj, nk (FishFry.hpp:112-114)                  */
+  double lo[3];   /* xLo, yLo, zLo (FishFry.hpp:137-139)                             */
+  double d[3];    /* dx, dy, dz (FishFry.hpp:109-111)                                */
+  double shift[3];/* phase shift added to x, y, z (0 for the reference benchmark)    */
+  double scale;   /* amplitude (1 for the reference benchmark)                       */
+  double dd[3];   /* filter constants for the exact potential of extra modes         */
+  int nmodes;     /* extra Fourier modes a*cos(k.x) + b*sin(k.x), k integer          */
+  int mode_k[FF_MAX_MODES][3];
+  double mode_a[FF_MAX_MODES];
+  double mode_b[FF_MAX_MODES];
+} ff_field;
+
+/* Fill ff_field with the reference benchmark's grid for n at 1x1x1 tasks:
+ * lo = -pi, d = 2pi/n, shift 0, scale 1, no extra modes (FishFry.hpp:108-139). */
+void ff_field_default(const int n[3], ff_field* out);
+
+/* The init kernel (FishFry.hpp:14-45): f = scale * lap(phi)(x + shift) plus
+ * the extra modes, written to f ([ni][nj][nk], FP64).  Enqueued on stream. */
+ff_status ff_field_init(const ff_field* field, double* f, void* stream);
+
+/* The diff kernel (FishFry.hpp:57-101) with a deterministic two-stage
+ * reduction: out[0] = sum|e|, out[1] = sum e^2, out[2] = max|e|, where e is
+ * the exact potential minus phi.  phi == NULL evaluates the exact potential
+ * against zero (the norm scales of FishFry.hpp:146-159).  Waits on stream. */
+ff_status ff_field_diff(const ff_field* field, const double* phi, double out[3], void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

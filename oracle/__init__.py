@@ -48,6 +48,8 @@ def lib():
         L.ora_bytes.argtypes = [_ip, _ip]
         L.ora_bytes.restype = ctypes.c_size_t
         L.ora_solve.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, ctypes.c_int]
+        L.ora_filter.argtypes = [_ip, _dp, _dp, _dp]
+        L.ora_filter.restype = None
         L.ora_init_field.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 6 + [_dp]
         L.ora_diff.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 6 + [_dp, _dp]
         L.ora_twiddle.argtypes = [ctypes.c_long, ctypes.c_long, _dp, _dp]
@@ -128,6 +130,18 @@ def solve(density: np.ndarray, lo=None, hi=None, nthreads: int = 0) -> np.ndarra
     if lib().ora_solve(_ints(n), _dbls(lo), _dbls(hi), _ptr(d), _ptr(out), _ptr(work), int(nthreads)):
         raise RuntimeError("ora_solve failed")
     return out
+
+
+def filter_half(spec: np.ndarray, nz: int, lo=None, hi=None) -> np.ndarray:
+    """Spectral filter of ParisPencil.cpp:322-344 applied to a copy of the
+    half spectrum spec[nx][ny][nz/2+1]."""
+    a = np.ascontiguousarray(spec, dtype=np.complex128).copy()
+    n = (a.shape[0], a.shape[1], nz)
+    assert a.shape[2] == nz // 2 + 1
+    if lo is None or hi is None:
+        lo, hi, _ = grid(n)
+    lib().ora_filter(_ints(n), _dbls(lo), _dbls(hi), _ptr(a.view(np.float64)))
+    return a
 
 
 def init_field(n) -> np.ndarray:

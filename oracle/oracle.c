@@ -307,35 +307,14 @@ size_t ora_bytes(const int n[3], const int m[3])
 
 static inline double sqr(double x) { return x * x; }
 
-int ora_solve(const int n[3], const double lo[3], const double hi[3],
-              const double *density, double *potential, double *work, int nthreads)
+void ora_filter(const int n[3], const double lo[3], const double hi[3], double *spec)
 {
   const int ni = n[0], nj = n[1], nk = n[2];
-  if (ni < 1 || nj < 1 || nk < 1) return -1;
   const int nh = nk / 2 + 1;
-#ifdef _OPENMP
-  if (nthreads > 0) omp_set_num_threads(nthreads);
-#else
-  (void)nthreads;
-#endif
   const double ddi = ora_dd(ni, lo[0], hi[0]);
   const double ddj = ora_dd(nj, lo[1], hi[1]);
   const double ddk = ora_dd(nk, lo[2], hi[2]);
-
-  plan1d pi_, pj_;
-  if (plan_init(&pi_, ni)) return -1;
-  if (plan_init(&pj_, nj)) { plan_free(&pi_); return -1; }
-  cplx *w = (cplx *)work;
-  int err = 0;
-
-  /* D2Z along z: [i][j][k] -> [i][j][kh]  (ParisPencil.cpp:224) */
-  err |= ora_rfft(nk, density, work, (long)ni * nj);
-  /* Z2Z forward along y: column stride nh  (ParisPencil.cpp:271) */
-  for (int i = 0; i < ni && !err; i++) err |= fft_lines(&pj_, w + (long)i * nj * nh, nh, nh, 1, -1);
-  /* Z2Z forward along x: column stride nj*nh  (ParisPencil.cpp:318) */
-  if (!err) err |= fft_lines(&pi_, w, (long)nj * nh, (long)nj * nh, 1, -1);
-  if (err) { plan_free(&pi_); plan_free(&pj_); return -1; }
-
+  cplx *w = (cplx *)spec;
   /* spectral filter  (ParisPencil.cpp:322-344), same expression order */
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < ni; i++) {
@@ -354,6 +333,36 @@ int ora_solve(const int n[3], const double lo[3], const double hi[3],
       }
     }
   }
+
+}
+
+int ora_solve(const int n[3], const double lo[3], const double hi[3],
+              const double *density, double *potential, double *work, int nthreads)
+{
+  const int ni = n[0], nj = n[1], nk = n[2];
+  if (ni < 1 || nj < 1 || nk < 1) return -1;
+  const int nh = nk / 2 + 1;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+
+  plan1d pi_, pj_;
+  if (plan_init(&pi_, ni)) return -1;
+  if (plan_init(&pj_, nj)) { plan_free(&pi_); return -1; }
+  cplx *w = (cplx *)work;
+  int err = 0;
+
+  /* D2Z along z: [i][j][k] -> [i][j][kh]  (ParisPencil.cpp:224) */
+  err |= ora_rfft(nk, density, work, (long)ni * nj);
+  /* Z2Z forward along y: column stride nh  (ParisPencil.cpp:271) */
+  for (int i = 0; i < ni && !err; i++) err |= fft_lines(&pj_, w + (long)i * nj * nh, nh, nh, 1, -1);
+  /* Z2Z forward along x: column stride nj*nh  (ParisPencil.cpp:318) */
+  if (!err) err |= fft_lines(&pi_, w, (long)nj * nh, (long)nj * nh, 1, -1);
+  if (err) { plan_free(&pi_); plan_free(&pj_); return -1; }
+
+  ora_filter(n, lo, hi, work);
 
   /* Z2Z inverse along x, then y  (ParisPencil.cpp:347,393) */
   err |= fft_lines(&pi_, w, (long)nj * nh, (long)nj * nh, 1, +1);

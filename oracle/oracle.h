@@ -51,6 +51,10 @@ double ora_dd(int n, double lo, double hi);
  * ParisPencil.cpp:108-134 (general m[3]). */
 size_t ora_bytes(const int n[3], const int m[3]);
 
+/* The spectral filter alone, in place on a half spectrum [ni][nj][nk/2+1]
+ * (complex interleaved): ParisPencil.cpp:322-344 at 1x1x1 tasks. */
+void ora_filter(const int n[3], const double lo[3], const double hi[3], double *spec);
+
 /* Full periodic Poisson solve at 1x1x1 tasks, ParisPencil.cpp:154-484:
  * D2Z(z), Z2Z fwd (y), Z2Z fwd (x), spectral filter (:322-344), Z2Z inv (x),
  * Z2Z inv (y), Z2D (z), scale by 1/(nx*ny*nz) (:466-481).

@@ -1,0 +1,100 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol the headers
+declare, and its host-only paths (byte formula, argument validation, field
+defaults) behave like the reference without touching a GPU."""
+import ctypes
+import math
+import os
+import re
+
+import pytest
+
+import fishfry_amd as ff
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(ff_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = ff.lib()
+    names = declared_symbols()
+    assert len(names) >= 12
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("n,m", [
+    ((64, 64, 64), (1, 1, 1)), ((1024, 1024, 1024), (1, 1, 1)), ((12, 10, 9), (1, 1, 1)),
+    ((128, 256, 384), (1, 2, 3)), ((100, 90, 80), (2, 2, 2)), ((33, 17, 65), (3, 1, 4)),
+    ((64, 64, 64), (1, 1, 8)), ((7, 5, 11), (1, 1, 1)),
+])
+def test_reference_bytes_matches_oracle(n, m):
+    assert ff.reference_bytes(n, m) == oracle.ref_bytes(n, m)
+
+
+def test_reference_bytes_known_values():
+    assert ff.reference_bytes((64, 64, 64)) == 2162688
+    assert ff.reference_bytes((1024, 1024, 1024)) == 8606711808
+    assert ff.reference_bytes((0, 4, 4)) == 0
+
+
+def test_plan_create_rejects_without_gpu_work():
+    lo, hi, _ = ff.benchmark_grid((16, 16, 16))
+    # more than one task is the reference's multi-rank case: unsupported on one GPU
+    with pytest.raises(ff.FishFryError) as e:
+        ff.ParisPencil((16, 16, 16), lo, hi, (1, 1, 2), (0, 0, 0))
+    assert e.value.status == ff.FF_ERR_UNSUPPORTED
+    with pytest.raises(ff.FishFryError) as e:
+        ff.ParisPencil((16, 16, 16), lo, hi, (1, 1, 1), (0, 1, 0))
+    assert e.value.status == ff.FF_ERR_INVALID
+    # a single cell on an axis makes hi == lo: the reference's dd is 0/0 there
+    lo1, hi1, _ = ff.benchmark_grid((1, 8, 8))
+    with pytest.raises(ff.FishFryError) as e:
+        ff.ParisPencil((1, 8, 8), lo1, hi1)
+    assert e.value.status == ff.FF_ERR_INVALID
+    L = ff.lib()
+    h = ctypes.c_void_p()
+    rc = L.ff_plan_create(None, None, None, None, None, None, ctypes.byref(h))
+    assert rc == ff.FF_ERR_INVALID and "null" in ff.last_error()
+    assert L.ff_plan_bytes(None) == 0
+    assert L.ff_plan_destroy(None) == ff.FF_OK
+
+
+def test_solve_rejects_bad_arguments_without_gpu_work():
+    L = ff.lib()
+    assert L.ff_solve(None, 0, None, None, None, None, None) == ff.FF_ERR_INVALID
+    assert L.ff_solve_async(None, 0, None, None, None) == ff.FF_ERR_INVALID
+    assert L.ff_solve_pass(None, 0, None, None, None) == ff.FF_ERR_INVALID
+
+
+def test_field_default_matches_reference_grid():
+    n = (64, 48, 1024)
+    f = ff.field_default(n)
+    for a in range(3):
+        assert f.n[a] == n[a]
+        assert f.d[a] == 2.0 * math.pi / n[a]
+        assert f.lo[a] == -math.pi
+        lo, hi, _ = oracle.grid(n)
+        assert f.dd[a] == oracle.dd(n[a], lo[a], hi[a])
+    assert f.scale == 1.0 and f.nmodes == 0
+    assert tuple(f.shift) == (0.0, 0.0, 0.0)
+
+
+def test_field_validation_without_gpu_work():
+    f = ff.field_default((8, 8, 8))
+    f.nmodes = 99
+    with pytest.raises(ff.FishFryError) as e:
+        ff.field_init(f, 0)
+    assert e.value.status == ff.FF_ERR_INVALID

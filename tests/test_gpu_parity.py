@@ -1,0 +1,274 @@
+"""GPU parity of the HIP solve against the CPU oracle (oracle/), the committed
+golden vectors (tests/golden/) and the benchmark's analytic potential.
+
+Bars: FP64 passes and solves agree with the oracle to a relative L2 of 1e-13
+(the acceptance bar for parity with the reference's FP64 solve is 1e-12); the
+analytic-phi norms stay at FP64 round-off.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import fishfry_amd as ff
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+TOL = 1e-13
+
+
+def rel(a, b):
+    d = np.linalg.norm(np.ravel(b))
+    return np.linalg.norm(np.ravel(a) - np.ravel(b)) / (d if d else 1.0)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make(n, **kw):
+    lo, hi, _ = ff.benchmark_grid(n)
+    plan = ff.ParisPencil(n, lo, hi, **kw)
+    nb = plan.bytes()
+    a = torch.zeros(nb // 8, dtype=torch.float64, device="cuda")
+    b = torch.zeros(nb // 8, dtype=torch.float64, device="cuda")
+    return plan, nb, a, b
+
+
+def put(buf, arr):
+    flat = np.ascontiguousarray(arr).view(np.float64).ravel()
+    buf[: flat.size].copy_(torch.from_numpy(flat))
+
+
+def get_real(buf, n):
+    N = n[0] * n[1] * n[2]
+    return buf[:N].cpu().numpy().reshape(n)
+
+
+def get_half(buf, n):
+    nh = n[2] // 2 + 1
+    M = n[0] * n[1] * nh
+    return buf[: 2 * M].cpu().numpy().view(np.complex128).reshape(n[0], n[1], nh)
+
+
+def gpu_solve(f, **kw):
+    n = f.shape
+    plan, nb, a, b = make(n, **kw)
+    put(a, f)
+    plan.solve(nb, a, b)
+    torch.cuda.synchronize()
+    out = get_real(b, n)
+    plan.close()
+    return out
+
+
+PASS_SHAPES = [(16, 24, 20), (32, 32, 32), (12, 10, 9), (64, 48, 40), (5, 6, 4), (9, 8, 12), (2, 8, 16),
+               (30, 25, 27)]
+
+
+@pytest.mark.parametrize("n", PASS_SHAPES)
+def test_each_pass_matches_oracle(n):
+    say("pass parity", n)
+    rng = np.random.default_rng(sum(n))
+    plan, nb, a, b = make(n)
+    nh = n[2] // 2 + 1
+    lo, hi, _ = oracle.grid(n)
+    # pass 0: z forward (D2Z)
+    f = rng.standard_normal(n)
+    put(a, f)
+    plan.solve_pass(0, a, b)
+    torch.cuda.synchronize()
+    assert rel(get_half(b, n), oracle.rfft(f)) < TOL
+    spec = rng.standard_normal((n[0], n[1], nh)) + 1j * rng.standard_normal((n[0], n[1], nh))
+    # pass 1: y forward
+    put(a, spec)
+    plan.solve_pass(1, a, b)
+    torch.cuda.synchronize()
+    assert rel(get_half(b, n), oracle.fft(spec, -1, axis=1)) < TOL
+    # pass 2: x forward * filter * inverse
+    put(a, spec)
+    plan.solve_pass(2, a, b)
+    torch.cuda.synchronize()
+    want = oracle.fft(oracle.filter_half(oracle.fft(spec, -1, axis=0), n[2], lo, hi), +1, axis=0)
+    assert rel(get_half(b, n), want) < TOL
+    # pass 3: y inverse
+    put(a, spec)
+    plan.solve_pass(3, a, b)
+    torch.cuda.synchronize()
+    assert rel(get_half(b, n), oracle.fft(spec, +1, axis=1)) < TOL
+    # pass 4: z inverse (Z2D) and the 1/N scale; Im of DC / Nyquist ignored
+    put(a, spec)
+    plan.solve_pass(4, a, b)
+    torch.cuda.synchronize()
+    want = oracle.irfft(spec, n[2]) * (1.0 / (float(n[0]) * float(n[1]) * float(n[2])))
+    assert rel(get_real(b, n), want) < TOL
+    plan.close()
+
+
+SOLVE_SHAPES = [(8, 8, 8), (12, 10, 9), (16, 24, 20), (32, 32, 32), (64, 64, 64), (6, 9, 15), (5, 6, 4),
+                (9, 8, 12), (48, 40, 36), (128, 64, 96), (2, 2, 2), (2, 3, 4), (2, 8, 16), (16, 2, 9),
+                (30, 25, 27), (100, 96, 80), (64, 64, 2), (4, 4, 2), (3, 3, 3), (250, 2, 6), (96, 160, 48)]
+
+
+@pytest.mark.parametrize("n", SOLVE_SHAPES)
+def test_solve_matches_oracle(n):
+    say("solve parity", n)
+    rng = np.random.default_rng(n[0] * 10007 + n[1] * 101 + n[2])
+    f = rng.standard_normal(n)
+    got = gpu_solve(f)
+    want = oracle.solve(f)
+    assert rel(got, want) < TOL, rel(got, want)
+
+
+def test_solve_matches_golden(golden):
+    tags = sorted({k[len("solve_rand_in_"):] for k in golden if k.startswith("solve_rand_in_")})
+    checked = 0
+    for kind in ("rand", "field"):
+        for tag in tags:
+            n = tuple(int(v) for v in tag.split("x"))
+            try:
+                ff.ParisPencil(n, *ff.benchmark_grid(n)[:2]).close()
+            except ff.FishFryError as e:
+                assert e.status == ff.FF_ERR_UNSUPPORTED  # prime factor > 5
+                continue
+            say("golden", kind, tag)
+            got = gpu_solve(golden[f"solve_{kind}_in_{tag}"])
+            assert rel(got, golden[f"solve_{kind}_out_{tag}"]) < TOL, (kind, tag)
+            checked += 1
+    assert checked >= 12
+
+
+@pytest.mark.parametrize("n", [(32, 32, 32), (64, 64, 64), (48, 40, 36), (128, 96, 64)])
+def test_benchmark_field_and_norms(n):
+    say("analytic", n)
+    plan, nb, a, b = make(n)
+    field = ff.field_default(n)
+    ff.field_init(field, a)
+    torch.cuda.synchronize()
+    f = get_real(a, n)
+    assert rel(f, oracle.init_field(n)) < 1e-14
+    plan.solve(nb, a, b)
+    torch.cuda.synchronize()
+    phi = get_real(b, n)
+    assert rel(phi, oracle.solve(f)) < TOL
+    l1, l2, linf = ff.field_norms(field, b)
+    assert linf < 2e-15 and l2 < 2e-15 and l1 < 2e-15
+    # the device error sums agree with the oracle's on a phi whose error is far
+    # above the 1-ulp differences between the two libms' exact potentials
+    rng = np.random.default_rng(3)
+    pert = phi + 1e-6 * rng.standard_normal(n)
+    put(b, pert)
+    got = np.array(ff.field_diff(field, b))
+    want = oracle.diff(pert)
+    assert np.all(np.abs(got - want) <= 1e-9 * np.abs(want)), (got, want)
+    # phi = None evaluates the exact potential against zero (FishFry.hpp:146-159)
+    s0 = np.array(ff.field_diff(field, None))
+    w0 = oracle.diff(np.zeros(n))
+    assert np.all(np.abs(s0 - w0) <= 1e-13 * np.abs(w0))
+    plan.close()
+
+
+def test_preserve_input_flag():
+    n = (48, 40, 36)
+    rng = np.random.default_rng(9)
+    f = rng.standard_normal(n)
+    plan, nb, a, b = make(n, flags=ff.FF_PRESERVE_INPUT)
+    put(a, f)
+    plan.solve(nb, a, b)
+    torch.cuda.synchronize()
+    assert np.array_equal(get_real(a, n), f)
+    first = get_real(b, n)
+    assert rel(first, oracle.solve(f)) < TOL
+    plan.solve(nb, a, b)
+    torch.cuda.synchronize()
+    assert np.array_equal(get_real(b, n), first)  # deterministic
+    plan.close()
+
+
+def test_stamps_and_contract():
+    n = (32, 24, 20)
+    plan, nb, a, b = make(n)
+    ff.field_init(ff.field_default(n), a)
+    stamps = []
+    plan.solve(nb, a, b, stamps)
+    names = [s[0] for s in stamps]
+    assert names == ["Total", "Reorder 3D block into sub-pencils", "Redistribute into Z pencils",
+                     "Real-to-complex FFT in Z", "Redistribute for Y pencils", "Forward FFT in Y",
+                     "Redistribute for X pencils", "X FFTs and filter", "Redistribute for Y pencils",
+                     "Backward FFT in Y", "Redistribute in Z pencils", "Complex-to-real FFT in Z",
+                     "Redistribute for 3D blocks", "Rearrange into 3D blocks and normalize"]
+    t = [s[1] for s in stamps]
+    assert all(t[i] <= t[i + 1] for i in range(len(t) - 1))
+    assert t[-1] > t[0]
+    # bytes check (ParisPencil.cpp:160)
+    with pytest.raises(ff.FishFryError) as e:
+        plan.solve(nb - 8, a, b)
+    assert e.value.status == ff.FF_ERR_BYTES
+    with pytest.raises(ff.FishFryError) as e:
+        plan.solve(nb, a, a)
+    assert e.value.status == ff.FF_ERR_INVALID
+    plan.close()
+    # extents with a prime factor above 5 have no radix plan yet
+    with pytest.raises(ff.FishFryError) as e:
+        ff.ParisPencil((14, 8, 8), *ff.benchmark_grid((14, 8, 8))[:2])
+    assert e.value.status == ff.FF_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("n", [(512, 512, 512), (768, 768, 768), (1000, 1000, 1000), (1024, 1024, 1024)])
+def test_large_benchmark_norms(n):
+    """Full-size FP64 solves: the analytic norms stay at round-off, and the
+    linearity / mean-removal properties hold (size-independent checks)."""
+    say("large", n)
+    plan, nb, a, b = make(n)
+    field = ff.field_default(n)
+    ff.field_init(field, a)
+    plan.solve(nb, a, b)
+    l1, l2, linf = ff.field_norms(field, b)
+    say("  norms", l1, l2, linf)
+    assert linf < 4e-15 and l2 < 4e-15 and l1 < 4e-15
+    # scale + constant shift: solve(2 f + 5) = 2 phi
+    N = n[0] * n[1] * n[2]
+    phi1 = b[:N].clone()
+    ff.field_init(field, a)
+    a[:N].mul_(2.0).add_(5.0)
+    plan.solve(nb, a, b)
+    d = (b[:N] - 2.0 * phi1).abs().max().item()
+    assert d < 1e-13 * phi1.abs().max().item() * 2
+    del phi1
+    plan.close()
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("n", [(64, 64, 64), (48, 40, 36)])
+def test_fp32_refinement_converges(n):
+    say("refine", n)
+    rng = np.random.default_rng(7)
+    f = oracle.init_field(n)
+    lo, hi, dx = oracle.grid(n)
+    x = [lo[a] + np.arange(n[a]) * dx[a] for a in range(3)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    for _ in range(8):
+        k = rng.integers(-6, 7, size=3)
+        f = f + rng.standard_normal() * np.cos(k[0] * X + k[1] * Y + k[2] * Z)
+    want = oracle.solve(f)
+    errs = []
+    for steps in (0, 1, 2, 3):
+        got = gpu_solve(f, precision=ff.FF_FP32_REFINE, refine_steps=steps)
+        errs.append(rel(got, want))
+        say("  steps", steps, "rel err vs FP64 oracle", errs[-1])
+    assert errs[0] < 1e-6
+    assert errs[1] < 1e-11
+    assert errs[2] < 1e-14
+    assert errs[3] < 1e-14
+    # the input is preserved in FP32 mode (refinement re-reads f)
+    plan, nb, a, b = make(n, precision=ff.FF_FP32_REFINE, refine_steps=1)
+    put(a, f)
+    plan.solve(nb, a, b)
+    torch.cuda.synchronize()
+    assert np.array_equal(get_real(a, n), f)
+    plan.close()
