@@ -38,6 +38,7 @@ FF_ERR_NOT_FOUND = 8
 FF_FP64 = 0
 FF_FP32_REFINE = 1
 FF_PRESERVE_INPUT = 1
+FF_REFERENCE_LAYOUT = 2
 FF_SOLVE_STAMPS = 14
 FF_MAX_MODES = 32
 
@@ -282,5 +283,5 @@ from .driver import FishFry  # noqa: E402  (needs the names above)
 __all__ = [
     "ParisPencil", "FishFry", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
     "field_norms", "reference_bytes", "benchmark_grid", "FishFryError", "lib", "LIB_PATH",
-    "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT",
+    "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_REFERENCE_LAYOUT",
 ]

@@ -26,11 +26,26 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 
 __device__ __forceinline__ double sqr(double x) { return x * x; }
 
+// Spectrum layout: element (i, j, kh) of the half spectrum lives at complex
+// offset ((i >> lgB)*ny + j)*(B*nh) + (i & (B-1))*nh + kh, i.e. x is split
+// into blocks of B planes with y between the blocks.  B = 1 is the
+// reference's [i][j][kh]; B > 1 keeps both column passes within a few dozen
+// pages per tile instead of one page per row for x.
+// Two-level strides address it: position p along an axis maps to
+// (p >> lg)*hi + (p & ((1 << lg) - 1))*lo.
+struct Stride2 {
+  long long hi, lo;
+  int lg;
+  __device__ __forceinline__ long long operator()(int p) const {
+    return (long long)(p >> lg) * hi + (long long)(p & ((1 << lg) - 1)) * lo;
+  }
+};
+
 struct ColArgs {
   const void* src;
   void* dst;
-  long long s_outer;  // complex elements between planes
-  long long s_r;      // complex elements between transform positions
+  Stride2 so;         // plane index -> offset
+  Stride2 sr;         // transform position -> offset
   int ncols;          // columns per plane (nh)
   int nouter;         // planes
   int T, lgT;         // columns per workgroup (power of two)
@@ -61,7 +76,7 @@ __global__ __launch_bounds__(FF_MAX_THREADS) void k_col(const ColArgs a) {
   const int tile = b - outer * a.tiles;
   const int col = tile * a.T + c;
   const bool valid = col < a.ncols;
-  const long long base = (long long)outer * a.s_outer + col;
+  const long long base = a.so(outer) + col;
   // src may equal dst (in-place pass): every thread stores exactly the
   // elements it loaded, after all of its loads.
   const cx<F>* src = reinterpret_cast<const cx<F>*>(a.src) + base;
@@ -71,7 +86,7 @@ __global__ __launch_bounds__(FF_MAX_THREADS) void k_col(const ColArgs a) {
   cx<F> v[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    if (valid) v[i] = src[(long long)(r0 + i * L) * a.s_r];
+    if (valid) v[i] = src[a.sr(r0 + i * L)];
     else v[i] = cx<F>{F(0), F(0)};
   }
   const ColIdx idx{a.T, c, a.ps};
@@ -102,7 +117,7 @@ __global__ __launch_bounds__(FF_MAX_THREADS) void k_col(const ColArgs a) {
   }
   if (valid) {
 #pragma unroll
-    for (int i = 0; i < E; ++i) dst[(long long)(r0 + i * L) * a.s_r] = v[i];
+    for (int i = 0; i < E; ++i) dst[a.sr(r0 + i * L)] = v[i];
   }
 }
 
@@ -112,6 +127,7 @@ struct RowArgs {
   const void* aux;  // f for the residual epilogue
   long long nrows;
   int nz, nh;
+  int ny, lgB;    // spectrum layout (see Stride2)
   int T;          // rows per workgroup
   uint32_t Linv;  // ceil(2^32 / L)
   int nblocks;
@@ -126,6 +142,14 @@ struct RowArgs {
 
 template <typename T>
 __device__ __forceinline__ double to_d(T x) { return double(x); }
+
+// Complex offset of spectrum row (i, j), row = i*ny + j.
+__device__ __forceinline__ long long spec_row(const RowArgs& a, long long row) {
+  const int i = (int)(row / a.ny);
+  const int j = (int)(row - (long long)i * a.ny);
+  const int B = 1 << a.lgB;
+  return ((long long)(i >> a.lgB) * a.ny + j) * ((long long)B * a.nh) + (long long)(i & (B - 1)) * a.nh;
+}
 
 // z forward: real rows (IN precision) -> half-spectrum rows (F precision).
 // Even nz: pack (x[2r], x[2r+1]) as one complex, an nz/2-point transform, then
@@ -166,7 +190,7 @@ __global__ __launch_bounds__(FF_MAX_THREADS) void k_r2c(const RowArgs a) {
     put_natural<E, F>(v, lds, idx, r0, L);
     __syncthreads();
     const cx<F>* __restrict__ twp = reinterpret_cast<const cx<F>*>(a.twpost);
-    cx<F>* __restrict__ dst = reinterpret_cast<cx<F>*>(a.dst) + row * a.nh;
+    cx<F>* __restrict__ dst = reinterpret_cast<cx<F>*>(a.dst) + (valid ? spec_row(a, row) : 0);
     if (valid) {
 #pragma unroll
       for (int i = 0; i < E; ++i) {
@@ -188,7 +212,7 @@ __global__ __launch_bounds__(FF_MAX_THREADS) void k_r2c(const RowArgs a) {
 #pragma unroll
     for (int i = 0; i < E; ++i) v[i] = cx<F>{valid ? F(src[r0 + i * L]) : F(0), F(0)};
     fft_line<E, -1, F>(v, a.lp, tw, lds, idx, r0);
-    cx<F>* __restrict__ dst = reinterpret_cast<cx<F>*>(a.dst) + row * a.nh;
+    cx<F>* __restrict__ dst = reinterpret_cast<cx<F>*>(a.dst) + (valid ? spec_row(a, row) : 0);
     if (valid) {
 #pragma unroll
       for (int i = 0; i < E; ++i) {
@@ -247,7 +271,7 @@ __global__ __launch_bounds__(FF_MAX_THREADS) void k_c2r(const RowArgs a) {
   const int M = a.lp.n;
   const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
   const RowIdx idx{line * a.pitch, a.ps};
-  const cx<F>* __restrict__ src = reinterpret_cast<const cx<F>*>(a.src) + row * a.nh;
+  const cx<F>* __restrict__ src = reinterpret_cast<const cx<F>*>(a.src) + (valid ? spec_row(a, row) : 0);
   const int nload = ODD ? a.nh : M + 1;
   // stage the half spectrum in LDS
 #pragma unroll

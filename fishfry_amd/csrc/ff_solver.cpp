@@ -76,7 +76,8 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   if (o.precision != FF_FP64 && o.precision != FF_FP32_REFINE)
     return set_error(FF_ERR_INVALID, "options.precision must be FF_FP64 or FF_FP32_REFINE");
   if (o.refine_steps < 0 || o.refine_steps > 8) return set_error(FF_ERR_INVALID, "refine_steps must be 0..8");
-  if (o.flags & ~FF_PRESERVE_INPUT) return set_error(FF_ERR_INVALID, "unknown bits in options.flags");
+  if (o.flags & ~(FF_PRESERVE_INPUT | FF_REFERENCE_LAYOUT))
+    return set_error(FF_ERR_INVALID, "unknown bits in options.flags");
 
   ff_plan* p = new (std::nothrow) ff_plan();
   if (!p) return set_error(FF_ERR_NOMEM, "host allocation failed");
@@ -92,6 +93,13 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   p->ddk = dd_const(n[2], lo[2], hi[2]);
   const int ones[3] = {1, 1, 1};
   p->bytes = reference_bytes(n, ones);
+  // Block x when an x-plane of the half spectrum spans more than a 2 MiB
+  // page: B = the largest power of two <= 16 dividing nx.
+  p->lgB = 0;
+  if ((long long)n[1] * p->nh * 16 > (2LL << 20)) {
+    while (p->lgB < 4 && n[0] % (2 << p->lgB) == 0) p->lgB++;
+  }
+  if (o.flags & FF_REFERENCE_LAYOUT) p->lgB = 0;
 
   ff_status st = hip_check(hipGetDevice(&p->device), "hipGetDevice");
   if (st != FF_OK) {
@@ -180,6 +188,8 @@ static RowArgs row_args(const ff_plan* p, int prec) {
   a.nrows = (long long)p->n[0] * p->n[1];
   a.nz = p->n[2];
   a.nh = p->nh;
+  a.ny = p->n[1];
+  a.lgB = p->lgB;
   a.T = c.T;
   a.Linv = c.Linv;
   a.nblocks = (int)((a.nrows + c.T - 1) / c.T);
@@ -226,14 +236,16 @@ ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* sr
   a.src = src;
   a.dst = dst;
   a.ncols = p->nh;
-  if (axis == 1) {  // y: planes i, stride nh   (ParisPencil.cpp:138 plan, batch nx*nh)
+  const long long Bnh = (long long)p->nh << p->lgB;
+  const long long blk = (long long)p->n[1] * Bnh;  // one block of 2^lgB x-planes
+  if (axis == 1) {  // y: planes i, positions j   (ParisPencil.cpp:138 plan, batch nx*nh)
     a.nouter = p->n[0];
-    a.s_outer = (long long)p->n[1] * p->nh;
-    a.s_r = p->nh;
-  } else {  // x: planes j, stride ny*nh       (ParisPencil.cpp:137 plan, batch ny*nh)
+    a.so = Stride2{blk, p->nh, p->lgB};
+    a.sr = Stride2{0, Bnh, 30};
+  } else {  // x: planes j, positions i          (ParisPencil.cpp:137 plan, batch ny*nh)
     a.nouter = p->n[1];
-    a.s_outer = p->nh;
-    a.s_r = (long long)p->n[1] * p->nh;
+    a.so = Stride2{0, Bnh, 30};
+    a.sr = Stride2{blk, p->nh, p->lgB};
   }
   a.T = c.T;
   a.lgT = c.lgT;

@@ -7,6 +7,7 @@ struct ff_plan {
   int n[3];
   double lo[3], hi[3];
   int nh;
+  int lgB;       // spectrum layout: x split in blocks of 2^lgB planes
   long long N;   // nx*ny*nz
   long long Nh;  // nx*ny*nh
   double ddi, ddj, ddk;

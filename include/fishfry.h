@@ -47,6 +47,10 @@ typedef enum ff_precision {
 #define FF_PRESERVE_INPUT 1 /* FP64: the plan owns a third work array (ff_plan_bytes()
                                more device memory) and solve leaves density intact;
                                FP32 modes always preserve density                   */
+#define FF_REFERENCE_LAYOUT 2 /* keep the intermediate half spectrum in the reference's
+                               [i][j][kh] layout (default: x in blocks of up to 16
+                               planes once an x-plane exceeds 2 MiB; see
+                               ff_plan_describe "block")                              */
 
 typedef struct ff_options {
   int precision;    /* ff_precision                                             */
@@ -108,7 +112,9 @@ ff_status ff_plan_describe(const ff_plan* plan, char* buf, size_t len);
 /* One pass of the solve, for per-pass parity checks and timing:
  * 0 z forward (real src -> complex dst), 1 y forward, 2 x forward*filter*
  * inverse, 3 y inverse, 4 z inverse + scale (complex src -> real dst).
- * src/dst follow the FP64 layouts of ParisPencil.cpp (D2Z output [i][j][kh]). */
+ * Real arrays are [i][j][k]; the half spectrum is the plan's layout: element
+ * (i, j, kh) at ((i/B)*ny + j)*B*nh + (i%B)*nh + kh, B from ff_plan_describe
+ * (B = 1, the reference's D2Z output layout, with FF_REFERENCE_LAYOUT). */
 ff_status ff_solve_pass(const ff_plan* plan, int pass, const double* src, double* dst, void* stream);
 
 /* ----------------------------------------------------------------------- */

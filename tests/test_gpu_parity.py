@@ -5,6 +5,7 @@ Bars: FP64 passes and solves agree with the oracle to a relative L2 of 1e-13
 (the acceptance bar for parity with the reference's FP64 solve is 1e-12); the
 analytic-phi norms stay at FP64 round-off.
 """
+import json
 import math
 import sys
 
@@ -50,10 +51,21 @@ def get_real(buf, n):
     return buf[:N].cpu().numpy().reshape(n)
 
 
-def get_half(buf, n):
+def block_of(plan):
+    return json.loads(plan.describe())["block"]
+
+
+def get_half(buf, n, B=1):
+    """Half spectrum [i][j][kh] from the plan's layout ((i/B)*ny + j)*B*nh + (i%B)*nh + kh."""
     nh = n[2] // 2 + 1
     M = n[0] * n[1] * nh
-    return buf[: 2 * M].cpu().numpy().view(np.complex128).reshape(n[0], n[1], nh)
+    a = buf[: 2 * M].cpu().numpy().view(np.complex128)
+    return a.reshape(n[0] // B, n[1], B, nh).transpose(0, 2, 1, 3).reshape(n[0], n[1], nh)
+
+
+def put_half(buf, spec, B=1):
+    n0, n1, nh = spec.shape
+    put(buf, np.ascontiguousarray(spec.reshape(n0 // B, B, n1, nh).transpose(0, 2, 1, 3)))
 
 
 def gpu_solve(f, **kw):
@@ -68,14 +80,15 @@ def gpu_solve(f, **kw):
 
 
 PASS_SHAPES = [(16, 24, 20), (32, 32, 32), (12, 10, 9), (64, 48, 40), (5, 6, 4), (9, 8, 12), (2, 8, 16),
-               (30, 25, 27)]
+               (30, 25, 27), (32, 1024, 512), (24, 600, 1000)]
 
 
 @pytest.mark.parametrize("n", PASS_SHAPES)
 def test_each_pass_matches_oracle(n):
-    say("pass parity", n)
     rng = np.random.default_rng(sum(n))
     plan, nb, a, b = make(n)
+    B = block_of(plan)
+    say("pass parity", n, "block", B)
     nh = n[2] // 2 + 1
     lo, hi, _ = oracle.grid(n)
     # pass 0: z forward (D2Z)
@@ -83,26 +96,23 @@ def test_each_pass_matches_oracle(n):
     put(a, f)
     plan.solve_pass(0, a, b)
     torch.cuda.synchronize()
-    assert rel(get_half(b, n), oracle.rfft(f)) < TOL
+    assert rel(get_half(b, n, B), oracle.rfft(f)) < TOL
     spec = rng.standard_normal((n[0], n[1], nh)) + 1j * rng.standard_normal((n[0], n[1], nh))
     # pass 1: y forward
-    put(a, spec)
+    put_half(a, spec, B)
     plan.solve_pass(1, a, b)
     torch.cuda.synchronize()
-    assert rel(get_half(b, n), oracle.fft(spec, -1, axis=1)) < TOL
+    assert rel(get_half(b, n, B), oracle.fft(spec, -1, axis=1)) < TOL
     # pass 2: x forward * filter * inverse
-    put(a, spec)
     plan.solve_pass(2, a, b)
     torch.cuda.synchronize()
     want = oracle.fft(oracle.filter_half(oracle.fft(spec, -1, axis=0), n[2], lo, hi), +1, axis=0)
-    assert rel(get_half(b, n), want) < TOL
+    assert rel(get_half(b, n, B), want) < TOL
     # pass 3: y inverse
-    put(a, spec)
     plan.solve_pass(3, a, b)
     torch.cuda.synchronize()
-    assert rel(get_half(b, n), oracle.fft(spec, +1, axis=1)) < TOL
+    assert rel(get_half(b, n, B), oracle.fft(spec, +1, axis=1)) < TOL
     # pass 4: z inverse (Z2D) and the 1/N scale; Im of DC / Nyquist ignored
-    put(a, spec)
     plan.solve_pass(4, a, b)
     torch.cuda.synchronize()
     want = oracle.irfft(spec, n[2]) * (1.0 / (float(n[0]) * float(n[1]) * float(n[2])))
@@ -110,9 +120,29 @@ def test_each_pass_matches_oracle(n):
     plan.close()
 
 
+def test_layouts_agree_bitwise():
+    """The blocked spectrum layout changes addresses only, not arithmetic."""
+    n = (32, 1024, 512)
+    rng = np.random.default_rng(4)
+    f = rng.standard_normal(n)
+    outs = []
+    for flags in (0, ff.FF_REFERENCE_LAYOUT):
+        plan, nb, a, b = make(n, flags=flags)
+        say("layout", flags, "block", block_of(plan))
+        assert block_of(plan) == (16 if flags == 0 else 1)
+        put(a, f)
+        plan.solve(nb, a, b)
+        torch.cuda.synchronize()
+        outs.append(get_real(b, n))
+        plan.close()
+    assert np.array_equal(outs[0], outs[1])
+    assert rel(outs[0], oracle.solve(f)) < TOL
+
+
 SOLVE_SHAPES = [(8, 8, 8), (12, 10, 9), (16, 24, 20), (32, 32, 32), (64, 64, 64), (6, 9, 15), (5, 6, 4),
                 (9, 8, 12), (48, 40, 36), (128, 64, 96), (2, 2, 2), (2, 3, 4), (2, 8, 16), (16, 2, 9),
-                (30, 25, 27), (100, 96, 80), (64, 64, 2), (4, 4, 2), (3, 3, 3), (250, 2, 6), (96, 160, 48)]
+                (30, 25, 27), (100, 96, 80), (64, 64, 2), (4, 4, 2), (3, 3, 3), (250, 2, 6), (96, 160, 48),
+                (48, 600, 1000), (40, 1024, 300)]
 
 
 @pytest.mark.parametrize("n", SOLVE_SHAPES)
