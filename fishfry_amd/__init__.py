@@ -38,7 +38,7 @@ FF_ERR_NOT_FOUND = 8
 FF_FP64 = 0
 FF_FP32_REFINE = 1
 FF_PRESERVE_INPUT = 1
-FF_REFERENCE_LAYOUT = 2
+FF_BLOCKED_LAYOUT = 2
 FF_SOLVE_STAMPS = 14
 FF_MAX_MODES = 32
 
@@ -60,6 +60,12 @@ class Options(ctypes.Structure):
 
 class Stamp(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 64), ("seconds", ctypes.c_double)]
+
+
+class Tune(ctypes.Structure):
+    """ff_tune: per-pass override of elements per thread (E) and lines per workgroup (T)."""
+    _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_int), ("precision", ctypes.c_int), ("E", ctypes.c_int),
+                ("T", ctypes.c_int)]
 
 
 class Field(ctypes.Structure):
@@ -110,6 +116,12 @@ def lib():
     L.ff_plan_describe.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
     L.ff_solve_pass.restype = ctypes.c_int
     L.ff_solve_pass.argtypes = [P, ctypes.c_int, P, P, P]
+    L.ff_tune_set.restype = ctypes.c_int
+    L.ff_tune_set.argtypes = [ctypes.POINTER(Tune)]
+    L.ff_tune_reset.restype = None
+    L.ff_tune_reset.argtypes = []
+    L.ff_tune_query.restype = ctypes.c_int
+    L.ff_tune_query.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Tune)]
     L.ff_field_default.restype = None
     L.ff_field_default.argtypes = [i3, ctypes.POINTER(Field)]
     L.ff_field_init.restype = ctypes.c_int
@@ -240,6 +252,22 @@ class ParisPencil:
         self.close()
 
 
+def tune_set(kind: int, n: int, precision: int, E: int, T: int) -> None:
+    """Override E / T for passes of kind (0 z rows, 1 y, 2 x) and length n."""
+    t = Tune(kind, n, precision, E, T)
+    check(lib().ff_tune_set(ctypes.byref(t)), "tune_set")
+
+
+def tune_reset() -> None:
+    lib().ff_tune_reset()
+
+
+def tune_query(kind: int, n: int, precision: int = 0, ncols: int = 1 << 20) -> Tune:
+    t = Tune()
+    check(lib().ff_tune_query(kind, n, precision, ncols, ctypes.byref(t)), "tune_query")
+    return t
+
+
 def field_default(n: Sequence[int]) -> Field:
     """The benchmark's f on the 1x1x1 grid (FishFry.hpp:108-139)."""
     f = Field()
@@ -283,5 +311,5 @@ from .driver import FishFry  # noqa: E402  (needs the names above)
 __all__ = [
     "ParisPencil", "FishFry", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
     "field_norms", "reference_bytes", "benchmark_grid", "FishFryError", "lib", "LIB_PATH",
-    "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_REFERENCE_LAYOUT",
+    "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_BLOCKED_LAYOUT",
 ]

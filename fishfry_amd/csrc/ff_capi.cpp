@@ -154,6 +154,7 @@ ff_status ff_plan_describe(const ff_plan* p, char* buf, size_t len) {
     return r;
   };
   s += line("z", p->zr[0].line, p->zr[0].T, p->zr[0].lds, p->zr[0].threads) + ",";
+  s += line("zi", p->zi[0].line, p->zi[0].T, p->zi[0].lds, p->zi[0].threads) + ",";
   s += line("y", p->cy[0].line, p->cy[0].T, p->cy[0].lds, p->cy[0].threads) + ",";
   s += line("x", p->cx[0].line, p->cx[0].T, p->cx[0].lds, p->cx[0].threads);
   s += "}";
@@ -175,6 +176,36 @@ ff_status ff_solve_pass(const ff_plan* p, int pass, const double* src, double* d
     case 4: return run_zinv(p, 0, 0, EPI_STORE, src, dst, nullptr, nullptr, s);
     default: return set_error(FF_ERR_INVALID, "ff_solve_pass: pass must be 0..4");
   }
+}
+
+ff_status ff_tune_set(const ff_tune* t) {
+  clear_error();
+  return tune_set(t);
+}
+
+void ff_tune_reset(void) { tune_reset(); }
+
+ff_status ff_tune_query(int kind, int n, int precision, int ncols, ff_tune* out) {
+  clear_error();
+  if (!out || kind < 0 || kind > 3 || n < 1 || (precision != 0 && precision != 1) || ncols < 1)
+    return set_error(FF_ERR_INVALID, "ff_tune_query: bad arguments");
+  const int eb = precision ? 8 : 16;
+  int E = 0, T = 0;
+  if (kind == 0 || kind == 3) {
+    RowCfg c;
+    if (!make_row_cfg(2 * n, eb, c, kind, precision))
+      return set_error(FF_ERR_UNSUPPORTED, "ff_tune_query: no plan for that length");
+    E = c.line.E;
+    T = c.T;
+  } else {
+    ColCfg c;
+    if (!make_col_cfg(n, ncols, eb, c, kind, precision))
+      return set_error(FF_ERR_UNSUPPORTED, "ff_tune_query: no plan for that length");
+    E = c.line.E;
+    T = c.T;
+  }
+  *out = ff_tune{kind, n, precision, E, T};
+  return FF_OK;
 }
 
 void ff_field_default(const int n[3], ff_field* out) {

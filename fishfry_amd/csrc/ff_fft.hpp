@@ -23,7 +23,14 @@
 #include "ff_consts.hpp"
 
 #define FF_MAXP 16
-#define FF_MAX_THREADS 256
+#define FF_MAX_THREADS 1024
+
+namespace ff {
+// Workgroup-size bound per elements-per-thread, chosen so the register file
+// holds the kernel without spills: 1024 threads leave 128 VGPRs per lane
+// (E <= 8), 512 leave 256 (E <= 16), 256 leave 512 (E <= 30).
+__host__ __device__ constexpr int max_threads_for(int E) { return E <= 8 ? 1024 : (E <= 16 ? 512 : 256); }
+}  // namespace ff
 
 namespace ff {
 

@@ -57,8 +57,16 @@ struct RowCfg {
   uint32_t Linv = 0;
 };
 
-bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int forceE = 0, int forceT = 0);
-bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int forceE = 0, int forceT = 0);
+// kind: 0 z forward rows, 1 y columns, 2 x columns, 3 z inverse rows;
+// prec: 0 FP64, 1 FP32.  Tuning
+// overrides (ff_tune_set) take precedence over the built-in heuristic.
+bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int kind, int prec);
+bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int kind, int prec);
+bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T);
+bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T);
+bool tune_lookup(int kind, int n, int prec, int& E, int& T);
+ff_status tune_set(const ff_tune* t);
+void tune_reset();
 
 hipError_t launch_col(int E, int f32, int mode, const Launch& l, const ColArgs& a);
 hipError_t launch_r2c(int E, int f32, int in_f32, int odd, const Launch& l, const RowArgs& a);

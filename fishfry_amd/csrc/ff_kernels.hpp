@@ -64,7 +64,7 @@ struct ColArgs {
 enum { COL_FWD = 0, COL_INV = 1, COL_FILTER = 2, COL_SYMBOL = 3 };
 
 template <typename F, int E, int MODE>
-__global__ __launch_bounds__(FF_MAX_THREADS) void k_col(const ColArgs a) {
+__global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
   const int tid = threadIdx.x;
@@ -155,7 +155,7 @@ __device__ __forceinline__ long long spec_row(const RowArgs& a, long long row) {
 // Even nz: pack (x[2r], x[2r+1]) as one complex, an nz/2-point transform, then
 // split the even/odd spectra.  Odd nz: a full nz-point transform of (x, 0).
 template <typename F, typename IN, int E, bool ODD>
-__global__ __launch_bounds__(FF_MAX_THREADS) void k_r2c(const RowArgs a) {
+__global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
   const int tid = threadIdx.x;
@@ -259,7 +259,7 @@ __device__ __forceinline__ void c2r_out2(OUT* out, const double* f, long long r,
 // The imaginary parts of the DC and (even nz) Nyquist terms are ignored,
 // i.e. the rows are read as Hermitian half spectra.
 template <typename F, typename OUT, int E, bool ODD, int EPI>
-__global__ __launch_bounds__(FF_MAX_THREADS) void k_c2r(const RowArgs a) {
+__global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
   const int tid = threadIdx.x;

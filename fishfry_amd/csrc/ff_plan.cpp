@@ -5,6 +5,8 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <tuple>
 
 #include "ff_internal.hpp"
 
@@ -161,7 +163,7 @@ bool plan_line(int n, LineCfg& out, int forceE) {
   for (int E : kEValues) {
     if (E == 1) continue;
     if (forceE && E != forceE) continue;
-    if (n % E || n / E > FF_MAX_THREADS) continue;
+    if (n % E || n / E > max_threads_for(E)) continue;
     std::vector<int> seq;
     if (!radix_seq(n, E, seq)) continue;
     const double cost = (double)seq.size() + 0.04 * E;
@@ -182,17 +184,37 @@ static int floor_pow2(int v) {
   return p;
 }
 
-bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int forceE, int forceT) {
-  if (!plan_line(n, out.line, forceE)) return false;
+// ---- tuning overrides -------------------------------------------------------
+
+static std::mutex& tune_mutex() {
+  static std::mutex m;
+  return m;
+}
+static std::map<std::tuple<int, int, int>, std::pair<int, int>>& tune_table() {
+  static std::map<std::tuple<int, int, int>, std::pair<int, int>> t;
+  return t;
+}
+
+bool tune_lookup(int kind, int n, int prec, int& E, int& T) {
+  std::lock_guard<std::mutex> g(tune_mutex());
+  auto it = tune_table().find({kind, n, prec});
+  if (it == tune_table().end()) return false;
+  E = it->second.first;
+  T = it->second.second;
+  return true;
+}
+
+bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T) {
+  if (!plan_line(n, out.line, E)) return false;
   const int L = out.line.lp.L;
-  int T = forceT ? forceT : floor_pow2(FF_MAX_THREADS / L);
-  if (!forceT) {
-    // no wider than the (power-of-two rounded) number of columns
+  if (T <= 0) {
+    // default: about 256 threads per workgroup, no wider than the columns
+    T = floor_pow2(std::max(1, 256 / L));
     int cap = 1;
     while (cap < ncols) cap *= 2;
     if (T > cap) T = cap;
   }
-  if (T < 1 || (T & (T - 1)) || T * L > FF_MAX_THREADS) return false;
+  if (T < 1 || (T & (T - 1)) || T * L > max_threads_for(out.line.E)) return false;
   out.T = T;
   out.lgT = 0;
   while ((1 << out.lgT) < T) out.lgT++;
@@ -203,12 +225,12 @@ bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int forceE, int
   return out.lds <= 160 * 1024;
 }
 
-bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int forceE, int forceT) {
+bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T) {
   const int M = (nz % 2 == 0) ? nz / 2 : nz;
-  if (!plan_line(M, out.line, forceE)) return false;
+  if (!plan_line(M, out.line, E)) return false;
   const int L = out.line.lp.L;
-  int T = forceT ? forceT : FF_MAX_THREADS / L;
-  if (T < 1 || T * L > FF_MAX_THREADS) return false;
+  if (T <= 0) T = std::max(1, 256 / L);
+  if (T < 1 || T * L > max_threads_for(out.line.E)) return false;
   out.T = T;
   out.ps = 4;
   out.pitch = (M + 1) + (M >> out.ps) + 1;
@@ -216,6 +238,60 @@ bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int forceE, int forceT) {
   out.Linv = L > 1 ? (uint32_t)(((1ull << 32) + (uint64_t)L - 1) / (uint64_t)L) : 0u;
   out.lds = (size_t)T * out.pitch * elem_bytes;
   return out.lds <= 160 * 1024;
+}
+
+bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int kind, int prec) {
+  int E = 0, T = 0;
+  if (tune_lookup(kind, n, prec, E, T) && col_cfg_with(n, ncols, elem_bytes, out, E, T)) return true;
+  return col_cfg_with(n, ncols, elem_bytes, out, 0, 0);
+}
+
+// Built-in defaults measured on MI355X at 1024^3 (FP64): the z inverse is
+// fastest with 4 elements per thread, the z forward with 8.
+static bool builtin_default(int kind, int n, int prec, int& E, int& T) {
+  if (prec == 0 && kind == 3 && n == 512) {
+    E = 4;
+    T = 4;
+    return true;
+  }
+  return false;
+}
+
+bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int kind, int prec) {
+  const int M = (nz % 2 == 0) ? nz / 2 : nz;
+  int E = 0, T = 0;
+  if (tune_lookup(kind, M, prec, E, T) && row_cfg_with(nz, elem_bytes, out, E, T)) return true;
+  if (builtin_default(kind, M, prec, E, T) && row_cfg_with(nz, elem_bytes, out, E, T)) return true;
+  return row_cfg_with(nz, elem_bytes, out, 0, 0);
+}
+
+ff_status tune_set(const ff_tune* t) {
+  if (!t) return set_error(FF_ERR_INVALID, "ff_tune_set: null");
+  if (t->kind < 0 || t->kind > 3 || t->n < 1 || (t->precision != 0 && t->precision != 1))
+    return set_error(FF_ERR_INVALID, "ff_tune_set: kind 0..3, n >= 1, precision 0/1");
+  if (t->E == 0) {
+    std::lock_guard<std::mutex> g(tune_mutex());
+    tune_table().erase({t->kind, t->n, t->precision});
+    return FF_OK;
+  }
+  const int eb = t->precision ? 8 : 16;
+  bool ok;
+  if (t->kind == 0 || t->kind == 3) {
+    RowCfg c;
+    ok = row_cfg_with(2 * t->n, eb, c, t->E, t->T);
+  } else {
+    ColCfg c;
+    ok = col_cfg_with(t->n, 1 << 20, eb, c, t->E, t->T);
+  }
+  if (!ok) return set_error(FF_ERR_INVALID, "ff_tune_set: no valid configuration for that E/T");
+  std::lock_guard<std::mutex> g(tune_mutex());
+  tune_table()[{t->kind, t->n, t->precision}] = {t->E, t->T};
+  return FF_OK;
+}
+
+void tune_reset() {
+  std::lock_guard<std::mutex> g(tune_mutex());
+  tune_table().clear();
 }
 
 // ---- dispatch ---------------------------------------------------------------

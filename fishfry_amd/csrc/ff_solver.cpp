@@ -76,7 +76,7 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   if (o.precision != FF_FP64 && o.precision != FF_FP32_REFINE)
     return set_error(FF_ERR_INVALID, "options.precision must be FF_FP64 or FF_FP32_REFINE");
   if (o.refine_steps < 0 || o.refine_steps > 8) return set_error(FF_ERR_INVALID, "refine_steps must be 0..8");
-  if (o.flags & ~(FF_PRESERVE_INPUT | FF_REFERENCE_LAYOUT))
+  if (o.flags & ~(FF_PRESERVE_INPUT | FF_BLOCKED_LAYOUT))
     return set_error(FF_ERR_INVALID, "unknown bits in options.flags");
 
   ff_plan* p = new (std::nothrow) ff_plan();
@@ -96,10 +96,9 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   // Block x when an x-plane of the half spectrum spans more than a 2 MiB
   // page: B = the largest power of two <= 16 dividing nx.
   p->lgB = 0;
-  if ((long long)n[1] * p->nh * 16 > (2LL << 20)) {
+  if ((o.flags & FF_BLOCKED_LAYOUT) && (long long)n[1] * p->nh * 16 > (2LL << 20)) {
     while (p->lgB < 4 && n[0] % (2 << p->lgB) == 0) p->lgB++;
   }
-  if (o.flags & FF_REFERENCE_LAYOUT) p->lgB = 0;
 
   ff_status st = hip_check(hipGetDevice(&p->device), "hipGetDevice");
   if (st != FF_OK) {
@@ -109,8 +108,8 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   const int nprec = (o.precision == FF_FP64) ? 1 : 2;
   for (int prec = 0; prec < nprec; ++prec) {
     const int eb = prec ? 8 : 16;
-    if (!make_row_cfg(n[2], eb, p->zr[prec]) || !make_col_cfg(n[1], p->nh, eb, p->cy[prec]) ||
-        !make_col_cfg(n[0], p->nh, eb, p->cx[prec])) {
+    if (!make_row_cfg(n[2], eb, p->zr[prec], 0, prec) || !make_row_cfg(n[2], eb, p->zi[prec], 3, prec) || !make_col_cfg(n[1], p->nh, eb, p->cy[prec], 1, prec) ||
+        !make_col_cfg(n[0], p->nh, eb, p->cx[prec], 2, prec)) {
       delete p;
       char buf[160];
       snprintf(buf, sizeof buf,
@@ -181,8 +180,8 @@ void plan_destroy(ff_plan* p) {
 
 // ---- passes -------------------------------------------------------------------
 
-static RowArgs row_args(const ff_plan* p, int prec) {
-  const RowCfg& c = p->zr[prec];
+static RowArgs row_args(const ff_plan* p, int prec, bool inverse) {
+  const RowCfg& c = inverse ? p->zi[prec] : p->zr[prec];
   RowArgs a;
   memset(&a, 0, sizeof a);
   a.nrows = (long long)p->n[0] * p->n[1];
@@ -205,7 +204,7 @@ static RowArgs row_args(const ff_plan* p, int prec) {
 }
 
 ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void* dst, hipStream_t s) {
-  RowArgs a = row_args(p, prec);
+  RowArgs a = row_args(p, prec, false);
   a.src = src;
   a.dst = dst;
   const RowCfg& c = p->zr[prec];
@@ -215,7 +214,7 @@ ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void
 
 ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void* src, void* dst, const double* f,
                    const double* shiftp, hipStream_t s) {
-  RowArgs a = row_args(p, prec);
+  RowArgs a = row_args(p, prec, true);
   a.src = src;
   a.dst = dst;
   a.aux = f;
@@ -223,7 +222,7 @@ ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void
     a.shiftp = shiftp;
     a.shift = a.scale;  // shift = (sum of phi) / N
   }
-  const RowCfg& c = p->zr[prec];
+  const RowCfg& c = p->zi[prec];
   const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
   return hip_check(launch_c2r(c.line.E, prec, out_kind, p->n[2] % 2, epi, l, a), "z inverse launch");
 }

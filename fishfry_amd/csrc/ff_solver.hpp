@@ -15,7 +15,8 @@ struct ff_plan {
   ff_options opt;
   int device;
   // pass configurations per precision (0 = FP64, 1 = FP32)
-  ff::RowCfg zr[2];
+  ff::RowCfg zr[2];  // z forward rows
+  ff::RowCfg zi[2];  // z inverse rows
   ff::ColCfg cy[2], cx[2];
   const void* tw_z[2];
   const void* tw_zpost[2];

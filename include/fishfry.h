@@ -47,15 +47,14 @@ typedef enum ff_precision {
 #define FF_PRESERVE_INPUT 1 /* FP64: the plan owns a third work array (ff_plan_bytes()
                                more device memory) and solve leaves density intact;
                                FP32 modes always preserve density                   */
-#define FF_REFERENCE_LAYOUT 2 /* keep the intermediate half spectrum in the reference's
-                               [i][j][kh] layout (default: x in blocks of up to 16
-                               planes once an x-plane exceeds 2 MiB; see
-                               ff_plan_describe "block")                              */
+#define FF_BLOCKED_LAYOUT 2 /* store the intermediate half spectrum with x in blocks of
+                               up to 16 planes once an x-plane exceeds 2 MiB (default:
+                               the reference's [i][j][kh]; see ff_plan_describe "block") */
 
 typedef struct ff_options {
   int precision;    /* ff_precision                                             */
   int refine_steps; /* FF_FP32_REFINE only: 0..8                                */
-  int flags;        /* 0 or FF_PRESERVE_INPUT                                   */
+  int flags;        /* 0 or FF_PRESERVE_INPUT | FF_BLOCKED_LAYOUT               */
 } ff_options;
 
 /* One phase time stamp: the reference's TimeStamp (TimeStamp.hpp:7), a
@@ -114,7 +113,28 @@ ff_status ff_plan_describe(const ff_plan* plan, char* buf, size_t len);
  * inverse, 3 y inverse, 4 z inverse + scale (complex src -> real dst).
  * Real arrays are [i][j][k]; the half spectrum is the plan's layout: element
  * (i, j, kh) at ((i/B)*ny + j)*B*nh + (i%B)*nh + kh, B from ff_plan_describe
- * (B = 1, the reference's D2Z output layout, with FF_REFERENCE_LAYOUT). */
+ * (B = 1, the reference's D2Z output layout, unless FF_BLOCKED_LAYOUT). */
+
+/* Tuning overrides consulted when a plan is created (process-wide, thread
+ * safe).  kind: 0 z forward rows, 1 y columns, 2 x columns, 3 z inverse
+ * rows; n: transform length
+ * (nz/2 for even-nz rows); precision: ff_precision of the pass (0 FP64,
+ * 1 FP32); E: elements per thread; T: lines per workgroup (power of two for
+ * columns).  E = 0 removes the override.  Invalid combinations are rejected
+ * when the override is set. */
+typedef struct ff_tune {
+  int kind;
+  int n;
+  int precision;
+  int E;
+  int T;
+} ff_tune;
+
+ff_status ff_tune_set(const ff_tune* t);
+void ff_tune_reset(void);
+/* The configuration a plan created now would use for (kind, n, precision);
+ * ncols is the column count of the pass (nh), used to cap T. */
+ff_status ff_tune_query(int kind, int n, int precision, int ncols, ff_tune* out);
 ff_status ff_solve_pass(const ff_plan* plan, int pass, const double* src, double* dst, void* stream);
 
 /* ----------------------------------------------------------------------- */

@@ -69,6 +69,7 @@ def put_half(buf, spec, B=1):
 
 
 def gpu_solve(f, **kw):
+    kw.setdefault("flags", 0)
     n = f.shape
     plan, nb, a, b = make(n, **kw)
     put(a, f)
@@ -126,10 +127,10 @@ def test_layouts_agree_bitwise():
     rng = np.random.default_rng(4)
     f = rng.standard_normal(n)
     outs = []
-    for flags in (0, ff.FF_REFERENCE_LAYOUT):
+    for flags in (0, ff.FF_BLOCKED_LAYOUT):
         plan, nb, a, b = make(n, flags=flags)
         say("layout", flags, "block", block_of(plan))
-        assert block_of(plan) == (16 if flags == 0 else 1)
+        assert block_of(plan) == (16 if flags else 1)
         put(a, f)
         plan.solve(nb, a, b)
         torch.cuda.synchronize()
