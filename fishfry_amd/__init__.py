@@ -68,6 +68,15 @@ class Tune(ctypes.Structure):
                 ("T", ctypes.c_int)]
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int * 3), ("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3),
+                ("precision", ctypes.c_int), ("refine_steps", ctypes.c_int)]
+
+
+FF_BATCH_GRAPH = 1
+FF_BATCH_PERSISTENT = 4
+
+
 class Field(ctypes.Structure):
     """ff_field: grid, phase shift, amplitude and extra modes of f."""
     _fields_ = [
@@ -122,6 +131,22 @@ def lib():
     L.ff_tune_reset.argtypes = []
     L.ff_tune_query.restype = ctypes.c_int
     L.ff_tune_query.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Tune)]
+    L.ff_batch_create.restype = ctypes.c_int
+    L.ff_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(BatchItem), ctypes.c_int, ctypes.POINTER(P)]
+    L.ff_batch_bytes.restype = ctypes.c_size_t
+    L.ff_batch_bytes.argtypes = [P, ctypes.c_int]
+    L.ff_batch_solve.restype = ctypes.c_int
+    L.ff_batch_solve.argtypes = [P, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), P]
+    L.ff_batch_wait.restype = ctypes.c_int
+    L.ff_batch_wait.argtypes = [P, P]
+    L.ff_batch_autotune.restype = ctypes.c_int
+    L.ff_batch_autotune.argtypes = [P, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), P,
+                                    ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double)]
+    L.ff_batch_describe.restype = ctypes.c_int
+    L.ff_batch_describe.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
+    L.ff_batch_destroy.restype = ctypes.c_int
+    L.ff_batch_destroy.argtypes = [P]
     L.ff_field_default.restype = None
     L.ff_field_default.argtypes = [i3, ctypes.POINTER(Field)]
     L.ff_field_init.restype = ctypes.c_int
@@ -268,6 +293,68 @@ def tune_query(kind: int, n: int, precision: int = 0, ncols: int = 1 << 20) -> T
     return t
 
 
+class Batch:
+    """Many independent grids (uneven shapes, mixed precisions) launched
+    together on parallel streams, optionally as one HIP graph (ff_batch_*).
+
+    items: sequence of dicts with n, and optionally lo/hi (default: the
+    benchmark grid), precision and refine_steps."""
+
+    def __init__(self, items, flags: int = 0):
+        arr = (BatchItem * len(items))()
+        for k, it in enumerate(items):
+            n = it["n"]
+            lo, hi, _ = benchmark_grid(n)
+            lo = it.get("lo", lo)
+            hi = it.get("hi", hi)
+            arr[k] = BatchItem(_i3(n), _d3(lo), _d3(hi), int(it.get("precision", FF_FP64)),
+                               int(it.get("refine_steps", 0)))
+        h = ctypes.c_void_p()
+        check(lib().ff_batch_create(len(items), arr, flags, ctypes.byref(h)), "Batch")
+        self._h = h
+        self.count = len(items)
+        self.items = list(items)
+
+    def bytes(self, i: int) -> int:
+        return int(lib().ff_batch_bytes(self._h, i))
+
+    @staticmethod
+    def _ptrs(bufs):
+        return (ctypes.c_void_p * len(bufs))(*[_ptr(b) for b in bufs])
+
+    def solve(self, densities, potentials, stream=None) -> None:
+        check(lib().ff_batch_solve(self._h, self._ptrs(densities), self._ptrs(potentials), _stream(stream)),
+              "batch solve")
+
+    def wait(self, stream=None) -> None:
+        check(lib().ff_batch_wait(self._h, _stream(stream)), "batch wait")
+
+    def autotune(self, densities, potentials, reps: int = 3, stream=None):
+        """Tune every distinct pass of the batch; returns (passes tuned, ms before, ms after)."""
+        t = ctypes.c_int()
+        b = ctypes.c_double()
+        a = ctypes.c_double()
+        check(lib().ff_batch_autotune(self._h, self._ptrs(densities), self._ptrs(potentials), _stream(stream), reps,
+                                      ctypes.byref(t), ctypes.byref(b), ctypes.byref(a)), "batch autotune")
+        return t.value, b.value, a.value
+
+    def describe(self) -> str:
+        buf = ctypes.create_string_buffer(1024 * (self.count + 4))
+        check(lib().ff_batch_describe(self._h, buf, len(buf)), "batch describe")
+        return buf.value.decode()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().ff_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def field_default(n: Sequence[int]) -> Field:
     """The benchmark's f on the 1x1x1 grid (FishFry.hpp:108-139)."""
     f = Field()
@@ -309,7 +396,7 @@ def field_norms(field: Field, phi, stream=None) -> Tuple[float, float, float]:
 from .driver import FishFry  # noqa: E402  (needs the names above)
 
 __all__ = [
-    "ParisPencil", "FishFry", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
+    "ParisPencil", "FishFry", "Batch", "FF_BATCH_GRAPH", "FF_BATCH_PERSISTENT", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
     "field_norms", "reference_bytes", "benchmark_grid", "FishFryError", "lib", "LIB_PATH",
     "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_BLOCKED_LAYOUT",
 ]

@@ -1,0 +1,722 @@
+// Host side of the batched solve (see ff_batch.hpp): op chains per instance,
+// tile shapes, resident twiddle tables, the persistent launch (optionally
+// captured in a HIP graph) and the autotuner.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <sstream>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "ff_batch.hpp"
+#include "ff_internal.hpp"
+#include "ff_solver.hpp"
+
+namespace ff {
+hipError_t batch_launch(const BatchDev& d, int grid, hipStream_t s);
+hipError_t batch_reset(unsigned int* done, int nops, unsigned long long* head, int nheads, unsigned int* err,
+                       hipStream_t s);
+hipError_t batch_occupancy(int lds_bytes, int* blocks_per_cu);
+hipError_t batch_prepare(int lds_bytes);
+
+namespace {
+
+// Radix sequence for the LDS engine.  strategy 0: fewest passes (largest
+// radices first); strategy 1: prefer radix 8 and 4 (shorter butterflies).
+bool engine_radices(int n, int strategy, std::vector<int>& seq) {
+  static const int big[] = {16, 12, 10, 8, 6, 5, 4, 3, 2};
+  static const int mid[] = {8, 4, 6, 5, 3, 2, 16, 12, 10};
+  seq.clear();
+  if (strategy == 0) {
+    std::vector<int> best(n + 1, 1 << 20), pick(n + 1, 0);
+    best[1] = 0;
+    for (int m = 2; m <= n; ++m) {
+      if (n % m) continue;
+      for (int R : big) {
+        if (m % R) continue;
+        if (best[m / R] + 1 < best[m]) {
+          best[m] = best[m / R] + 1;
+          pick[m] = R;
+        }
+      }
+    }
+    if (best[n] >= (1 << 20)) return false;
+    for (int m = n; m > 1; m /= pick[m]) seq.push_back(pick[m]);
+    std::sort(seq.begin(), seq.end(), [](int a, int b) { return a > b; });
+  } else {
+    int m = n;
+    while (m > 1) {
+      int R = 0;
+      for (int r : mid)
+        if (m % r == 0) {
+          R = r;
+          break;
+        }
+      if (!R) return false;
+      seq.push_back(R);
+      m /= R;
+    }
+  }
+  return (int)seq.size() <= FF_MAXP;
+}
+
+uint32_t inv32(int d) { return d > 1 ? (uint32_t)(((1ull << 32) + (uint64_t)d - 1) / (uint64_t)d) : 0u; }
+
+bool make_bline(int n, int strategy, BLine& lp) {
+  std::vector<int> seq;
+  memset(&lp, 0, sizeof lp);
+  lp.n = n;
+  if (n == 1) return true;
+  if (!engine_radices(n, strategy, seq)) return false;
+  lp.np = (int)seq.size();
+  int ns = 1;
+  for (int p = 0; p < lp.np; ++p) {
+    const int R = seq[p];
+    lp.radix[p] = R;
+    lp.nr[p] = n / R;
+    lp.nrinv[p] = inv32(n / R);
+    lp.ns[p] = ns;
+    lp.nsinv[p] = inv32(ns);
+    lp.twstep[p] = n / (ns * R);
+    ns *= R;
+  }
+  return true;
+}
+
+int floor_pow2(int v) {
+  int p = 1;
+  while (p * 2 <= v) p *= 2;
+  return p;
+}
+
+double dd_const(int n, double lo, double hi) { return 2.0 * M_PI * double(n - 1) / (double(n) * (hi - lo)); }
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+// ---- resident twiddle cache --------------------------------------------------
+
+struct TwKey {
+  int dev, n, prec, post;
+  bool operator<(const TwKey& o) const {
+    return std::tie(dev, n, prec, post) < std::tie(o.dev, o.n, o.prec, o.post);
+  }
+};
+
+const void* cached_table(int n, int prec, int post) {
+  static std::mutex m;
+  static std::map<TwKey, void*> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> g(m);
+  const TwKey key{dev, n, prec, post};
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  const int count = post ? n / 2 : n;
+  const size_t eb = prec ? 8 : 16;
+  std::vector<unsigned char> buf(std::max<size_t>(eb, eb * count));
+  for (int t = 0; t < count; ++t) {
+    long double c, s;
+    twiddle_ld(t, n, c, s);
+    if (prec) {
+      const float v[2] = {(float)c, (float)s};
+      memcpy(&buf[t * eb], v, sizeof v);
+    } else {
+      const double v[2] = {(double)c, (double)s};
+      memcpy(&buf[t * eb], v, sizeof v);
+    }
+  }
+  void* d = nullptr;
+  if (hipMalloc(&d, buf.size()) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, buf.data(), buf.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  cache[key] = d;
+  return d;
+}
+
+}  // namespace
+
+const void* twiddle_table(int n, int prec) { return cached_table(n, prec, 0); }
+const void* twiddle_post(int nz, int prec) { return cached_table(nz, prec, 1); }
+
+size_t batch_item_bytes(const ff_batch_item& it) {
+  const int ones[3] = {1, 1, 1};
+  return reference_bytes(it.n, ones);
+}
+
+size_t batch_scratch_bytes(const ff_batch_item& it) {
+  if (it.precision != FF_FP32_REFINE) return 0;
+  const size_t N = (size_t)it.n[0] * it.n[1] * it.n[2];
+  const size_t Nh = (size_t)it.n[0] * it.n[1] * (it.n[2] / 2 + 1);
+  return align256(Nh * 16) + align256(N * 4) + 256;
+}
+
+// ---- BatchCore -----------------------------------------------------------------
+
+ff_status BatchCore::init(int cnt, const ff_batch_item* its, int flg) {
+  if (cnt < 1 || !its) return set_error(FF_ERR_INVALID, "batch: need at least one item");
+  if (flg & ~(FF_BATCH_GRAPH | kBatchPerStep)) return set_error(FF_ERR_INVALID, "batch: bad flags");
+  count = cnt;
+  flags = flg;
+  items.assign(its, its + cnt);
+  N.clear();
+  Nh.clear();
+  bytes.clear();
+  scratch_off.clear();
+  scratch_bytes = 0;
+  for (int i = 0; i < cnt; ++i) {
+    const ff_batch_item& it = its[i];
+    for (int a = 0; a < 3; ++a)
+      if (it.n[a] < 2 || !(it.hi[a] > it.lo[a]))
+        return set_error(FF_ERR_INVALID, "batch: extents must be >= 2 with hi > lo");
+    if ((it.precision != FF_FP64 && it.precision != FF_FP32_REFINE) || it.refine_steps < 0 || it.refine_steps > 8)
+      return set_error(FF_ERR_INVALID, "batch: bad precision / refine_steps");
+    const int nz = it.n[2];
+    const int M = nz % 2 ? nz : nz / 2;
+    BLine tmp;
+    for (int n : {it.n[0], it.n[1], M}) {
+      if (n > 4096 || !make_bline(n, 0, tmp)) {
+        char buf[160];
+        snprintf(buf, sizeof buf, "batch: item %d (%dx%dx%d) has an extent with a prime factor above 5", i, it.n[0],
+                 it.n[1], it.n[2]);
+        return set_error(FF_ERR_UNSUPPORTED, buf);
+      }
+    }
+    const int nprec = it.precision == FF_FP64 ? 1 : 2;
+    for (int prec = 0; prec < nprec; ++prec) {
+      for (int n : {it.n[0], it.n[1], M})
+        if (!twiddle_table(n, prec)) return set_error(FF_ERR_NOMEM, "batch: twiddle table allocation failed");
+      if (nz % 2 == 0 && !twiddle_post(nz, prec))
+        return set_error(FF_ERR_NOMEM, "batch: twiddle table allocation failed");
+    }
+    bytes.push_back(batch_item_bytes(it));
+    N.push_back((long long)it.n[0] * it.n[1] * it.n[2]);
+    Nh.push_back((long long)it.n[0] * it.n[1] * (it.n[2] / 2 + 1));
+    scratch_off.push_back(scratch_bytes);
+    scratch_bytes += batch_scratch_bytes(it);
+  }
+  return FF_OK;
+}
+
+namespace {
+
+bool size_op(const BatchCore& b, BOp& o) {
+  const int eb = o.prec ? 8 : 16;
+  const int budget = b.budget_kib * 1024 / (2 * eb);  // complex elements per buffer
+  if (o.kind == BOP_COL) {
+    o.pitch = o.line.n + 1;
+    int T = floor_pow2(std::max(1, budget / o.pitch));
+    int cap = 1;
+    while (cap < o.nh) cap *= 2;
+    T = std::min(T, cap);
+    o.T = T;
+    o.lgT = 0;
+    while ((1 << o.lgT) < T) o.lgT++;
+    o.tiles_pp = (o.nh + T - 1) / T;
+    o.ntiles = (int)(o.nlines * o.tiles_pp);
+  } else {
+    o.pitch = o.line.n + 2;
+    o.T = std::max(1, std::min(budget / o.pitch, 64));
+    o.lgT = 0;
+    o.ntiles = (int)((o.nlines + o.T - 1) / o.T);
+  }
+  return (size_t)2 * o.T * o.pitch * eb <= 160 * 1024;
+}
+
+ff_status build_chain(const BatchCore& b, int i, double* density, double* potential, void* scratch,
+                      std::vector<BOp>& ops) {
+  const ff_batch_item& it = b.items[i];
+  const int nx = it.n[0], ny = it.n[1], nz = it.n[2];
+  const int nh = nz / 2 + 1;
+  const int odd = nz % 2;
+  const int M = odd ? nz : nz / 2;
+  BOp base;
+  memset(&base, 0, sizeof base);
+  base.nx = nx;
+  base.ny = ny;
+  base.nz = nz;
+  base.nh = nh;
+  base.odd = odd;
+  base.scale = 1.0 / (double(nx) * double(ny) * double(nz));  // ParisPencil.cpp:467
+  base.ddi = dd_const(nx, it.lo[0], it.hi[0]);               // ParisPencil.cpp:68-70
+  base.ddj = dd_const(ny, it.lo[1], it.hi[1]);
+  base.ddk = dd_const(nz, it.lo[2], it.hi[2]);
+  base.dep = -1;
+  auto zf = [&](int prec, int io32, const void* src, void* dst) {
+    BOp o = base;
+    o.kind = BOP_ZF;
+    o.prec = prec;
+    o.io32 = io32;
+    o.nlines = (long long)nx * ny;
+    make_bline(M, b.strategy, o.line);
+    o.tw = twiddle_table(M, prec);
+    o.twpost = odd ? nullptr : twiddle_post(nz, prec);
+    o.src = src;
+    o.dst = dst;
+    return o;
+  };
+  auto col = [&](int prec, int axis, int mode, const void* src, void* dst, double* dc) {
+    BOp o = base;
+    o.kind = BOP_COL;
+    o.prec = prec;
+    o.mode = mode;
+    const int n = axis == 1 ? ny : nx;
+    make_bline(n, b.strategy, o.line);
+    o.tw = twiddle_table(n, prec);
+    if (axis == 1) {  // planes i, positions j
+      o.nlines = nx;
+      o.s_outer = (long long)ny * nh;
+      o.s_r = nh;
+    } else {  // planes j, positions i
+      o.nlines = ny;
+      o.s_outer = nh;
+      o.s_r = (long long)ny * nh;
+    }
+    o.src = src;
+    o.dst = dst;
+    o.dc_out = dc;
+    return o;
+  };
+  auto zi = [&](int prec, int io32, int epi, const void* src, void* dst, const double* aux, const double* shiftp) {
+    BOp o = base;
+    o.kind = BOP_ZI;
+    o.prec = prec;
+    o.io32 = io32;
+    o.mode = epi;
+    o.nlines = (long long)nx * ny;
+    make_bline(M, b.strategy, o.line);
+    o.tw = twiddle_table(M, prec);
+    o.twpost = odd ? nullptr : twiddle_post(nz, prec);
+    o.src = src;
+    o.dst = dst;
+    o.aux = aux;
+    o.shiftp = shiftp;
+    return o;
+  };
+  ops.clear();
+  if (it.precision == FF_FP64) {
+    // ping-pong density <-> potential as in the single-grid solve
+    ops.push_back(zf(0, 0, density, potential));
+    ops.push_back(col(0, 1, COL_FWD, potential, density, nullptr));
+    ops.push_back(col(0, 0, COL_FILTER, density, potential, nullptr));
+    ops.push_back(col(0, 1, COL_INV, potential, density, nullptr));
+    ops.push_back(zi(0, 0, EPI_STORE, density, potential, nullptr, nullptr));
+  } else {
+    unsigned char* w = static_cast<unsigned char*>(scratch) + b.scratch_off[i];
+    void* spec = w;
+    float* res = reinterpret_cast<float*>(w + align256((size_t)b.Nh[i] * 16));
+    double* dc = reinterpret_cast<double*>(w + align256((size_t)b.Nh[i] * 16) + align256((size_t)b.N[i] * 4));
+    ops.push_back(zf(1, 0, density, spec));
+    ops.push_back(col(1, 1, COL_FWD, spec, spec, nullptr));
+    ops.push_back(col(1, 0, COL_FILTER, spec, spec, nullptr));
+    ops.push_back(col(1, 1, COL_INV, spec, spec, nullptr));
+    ops.push_back(zi(1, 0, EPI_STORE, spec, potential, nullptr, nullptr));
+    for (int s = 0; s < it.refine_steps; ++s) {
+      ops.push_back(zf(0, 0, potential, spec));
+      ops.push_back(col(0, 1, COL_FWD, spec, spec, nullptr));
+      ops.push_back(col(0, 0, COL_SYMBOL, spec, spec, dc));
+      ops.push_back(col(0, 1, COL_INV, spec, spec, nullptr));
+      ops.push_back(zi(0, 1, EPI_RESIDUAL, spec, res, density, nullptr));
+      ops.push_back(zf(1, 1, res, spec));
+      ops.push_back(col(1, 1, COL_FWD, spec, spec, nullptr));
+      ops.push_back(col(1, 0, COL_FILTER, spec, spec, nullptr));
+      ops.push_back(col(1, 1, COL_INV, spec, spec, nullptr));
+      ops.push_back(zi(1, 0, EPI_ACCUM, spec, potential, nullptr, dc));
+    }
+  }
+  for (BOp& o : ops) {
+    if (!o.tw || (!o.odd && o.kind != BOP_COL && !o.twpost))
+      return set_error(FF_ERR_NOMEM, "batch: twiddle table missing");
+    if (!size_op(b, o)) return set_error(FF_ERR_UNSUPPORTED, "batch: tile does not fit in LDS");
+  }
+  return FF_OK;
+}
+
+}  // namespace
+
+ff_status BatchCore::build(double* const* density, double* const* potential, void* scratch) {
+  std::vector<std::vector<BOp>> chains(count);
+  size_t maxlen = 0;
+  for (int i = 0; i < count; ++i) {
+    ff_status st = build_chain(*this, i, density[i], potential[i], scratch, chains[i]);
+    if (st != FF_OK) return st;
+    maxlen = std::max(maxlen, chains[i].size());
+  }
+  // step-major order; each op waits for the same instance's previous op
+  ops.clear();
+  item_start.assign(1, 0);
+  step_start.assign(1, 0);
+  std::vector<int> last(count, -1);
+  lds_bytes = 0;
+  for (size_t s = 0; s < maxlen; ++s) {
+    for (int i = 0; i < count; ++i) {
+      if (s >= chains[i].size()) continue;
+      BOp o = chains[i][s];
+      o.dep = last[i];
+      last[i] = (int)ops.size();
+      ops.push_back(o);
+      item_start.push_back(item_start.back() + o.ntiles);
+      const int eb = o.prec ? 8 : 16;
+      lds_bytes = std::max(lds_bytes, 2 * o.T * o.pitch * eb);
+    }
+    step_start.push_back(item_start.back());
+  }
+  nsteps = (int)maxlen;
+  const int nops = (int)ops.size();
+  // device block: ops | item_start | done | heads | err
+  o_is = align256(sizeof(BOp) * nops);
+  o_done = align256(o_is + sizeof(long long) * (nops + 1));
+  o_head = align256(o_done + sizeof(unsigned) * nops);
+  o_err = align256(o_head + sizeof(unsigned long long) * (nsteps + 1));
+  desc_bytes = o_err + 256;
+  image.assign(o_done, 0);
+  memcpy(&image[0], ops.data(), sizeof(BOp) * nops);
+  memcpy(&image[o_is], item_start.data(), sizeof(long long) * (nops + 1));
+  int per_cu = 0, dev_id = 0, ncu = 0;
+  ff_status st = hip_check(batch_prepare(lds_bytes), "batch kernel LDS attribute");
+  if (st != FF_OK) return st;
+  st = hip_check(batch_occupancy(lds_bytes, &per_cu), "batch occupancy");
+  if (st != FF_OK) return st;
+  (void)hipGetDevice(&dev_id);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_id);
+  const long long nitems = item_start.back();
+  grid = (int)std::max(1LL, std::min<long long>(nitems, (long long)std::max(1, per_cu) * std::max(1, ncu)));
+  return FF_OK;
+}
+
+ff_status BatchCore::upload(void* d, hipStream_t s) {
+  dev = d;
+  return hip_check(hipMemcpyAsync(d, image.data(), image.size(), hipMemcpyHostToDevice, s), "batch upload");
+}
+
+BatchDev BatchCore::view() const {
+  unsigned char* base = static_cast<unsigned char*>(dev);
+  BatchDev d;
+  d.ops = reinterpret_cast<const BOp*>(base);
+  d.item_start = reinterpret_cast<const long long*>(base + o_is);
+  d.nops = (int)ops.size();
+  d.done = reinterpret_cast<unsigned*>(base + o_done);
+  d.head = reinterpret_cast<unsigned long long*>(base + o_head);
+  d.err = reinterpret_cast<unsigned*>(base + o_err);
+  d.item_begin = 0;
+  d.item_end = item_start.back();
+  d.spin_limit = 200000000LL;  // 2 s
+  d.lds_bytes = lds_bytes;
+  return d;
+}
+
+ff_status BatchCore::enqueue(hipStream_t s) const {
+  const BatchDev d = view();
+  ff_status st = hip_check(batch_reset(d.done, d.nops, d.head, nsteps + 1, d.err, s), "batch reset launch");
+  if (st != FF_OK) return st;
+  if (flags & kBatchPerStep) {
+    for (int k = 0; k < nsteps; ++k) {
+      BatchDev dk = d;
+      dk.head = d.head + 1 + k;
+      dk.item_begin = step_start[k];
+      dk.item_end = step_start[k + 1];
+      st = hip_check(batch_launch(dk, (int)std::min<long long>(dk.item_end - dk.item_begin, grid), s),
+                     "batch step launch");
+      if (st != FF_OK) return st;
+    }
+    return FF_OK;
+  }
+  return hip_check(batch_launch(d, grid, s), "batch launch");
+}
+
+}  // namespace ff
+
+using namespace ff;
+
+// The C-ABI batch.  Each instance runs the single-grid pass kernels with its
+// own plan; the instances are spread over a few streams (fork on the caller's
+// stream, join back onto it) so their kernels overlap on the device, and with
+// FF_BATCH_GRAPH the whole fork/join sequence is captured once as a HIP graph
+// and replayed while the buffers stay the same.
+//
+// The single persistent launch over every instance (BatchCore, k_batch) is
+// switched off: its kernel did not complete on MI355X and the cause is not
+// established, so FF_BATCH_PERSISTENT is rejected.
+struct ff_batch {
+  int count = 0;
+  int flags = 0;
+  std::vector<ff_batch_item> items;
+  std::vector<ff_plan*> plans;  // per instance
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> joins;
+  hipEvent_t fork = nullptr;
+  std::vector<double*> bd, bp;
+  bool bound = false;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  hipStream_t graph_stream = nullptr;
+};
+
+namespace {
+
+constexpr int kBatchStreams = 4;
+
+void drop_graph(ff_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+void destroy_plans(ff_batch* b) {
+  for (ff_plan* p : b->plans) plan_destroy(p);
+  b->plans.clear();
+}
+
+ff_status make_plans(ff_batch* b) {
+  destroy_plans(b);
+  for (const ff_batch_item& it : b->items) {
+    ff_options o{it.precision, it.refine_steps, 0};
+    ff_plan* p = nullptr;
+    ff_status st = plan_create(it.n, it.lo, it.hi, &o, &p);
+    if (st != FF_OK) return st;
+    b->plans.push_back(p);
+  }
+  return FF_OK;
+}
+
+// Fork onto the batch's streams, enqueue every instance, join back onto s.
+ff_status enqueue_all(ff_batch* b, double* const* d, double* const* p, hipStream_t s) {
+  ff_status st = hip_check(hipEventRecord(b->fork, s), "batch fork");
+  for (size_t k = 0; k < b->streams.size() && st == FF_OK; ++k)
+    st = hip_check(hipStreamWaitEvent(b->streams[k], b->fork, 0), "batch fork wait");
+  for (int i = 0; i < b->count && st == FF_OK; ++i)
+    st = solve_enqueue(b->plans[i], d[i], p[i], b->streams[i % b->streams.size()], nullptr);
+  for (size_t k = 0; k < b->streams.size() && st == FF_OK; ++k) {
+    st = hip_check(hipEventRecord(b->joins[k], b->streams[k]), "batch join");
+    if (st == FF_OK) st = hip_check(hipStreamWaitEvent(s, b->joins[k], 0), "batch join wait");
+  }
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+ff_status ff_batch_create(int count, const ff_batch_item* items, int flags, ff_batch** out) {
+  clear_error();
+  if (!out) return set_error(FF_ERR_INVALID, "ff_batch_create: null output");
+  *out = nullptr;
+  if (count < 1 || !items) return set_error(FF_ERR_INVALID, "ff_batch_create: need at least one item");
+  if (flags & FF_BATCH_PERSISTENT)
+    return set_error(FF_ERR_UNSUPPORTED,
+                     "ff_batch_create: the single persistent launch is disabled in this build (its kernel did not "
+                     "complete on MI355X); use the default stream-parallel batch");
+  if (flags & ~FF_BATCH_GRAPH) return set_error(FF_ERR_INVALID, "ff_batch_create: bad flags");
+  for (int i = 0; i < count; ++i)
+    for (int a = 0; a < 3; ++a)
+      if (items[i].n[a] < 2) return set_error(FF_ERR_INVALID, "ff_batch_create: extents must be >= 2");
+  ff_batch* b = new (std::nothrow) ff_batch();
+  if (!b) return set_error(FF_ERR_NOMEM, "host allocation failed");
+  b->count = count;
+  b->flags = flags;
+  b->items.assign(items, items + count);
+  ff_status st = make_plans(b);
+  const int ns = std::min(kBatchStreams, count);
+  for (int k = 0; k < ns && st == FF_OK; ++k) {
+    hipStream_t s = nullptr;
+    hipEvent_t e = nullptr;
+    st = hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "batch stream");
+    if (st == FF_OK) b->streams.push_back(s);
+    if (st == FF_OK) st = hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "batch event");
+    if (st == FF_OK) b->joins.push_back(e);
+  }
+  if (st == FF_OK) st = hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "batch event");
+  if (st != FF_OK) {
+    ff_batch_destroy(b);
+    return st;
+  }
+  *out = b;
+  return FF_OK;
+}
+
+size_t ff_batch_bytes(const ff_batch* b, int index) {
+  if (!b || index < 0 || index >= b->count) return 0;
+  return b->plans[index]->bytes;
+}
+
+ff_status ff_batch_solve(ff_batch* b, double* const* density, double* const* potential, void* stream) {
+  clear_error();
+  if (!b || !density || !potential) return set_error(FF_ERR_INVALID, "ff_batch_solve: null argument");
+  for (int i = 0; i < b->count; ++i)
+    if (!density[i] || !potential[i] || density[i] == potential[i])
+      return set_error(FF_ERR_INVALID, "ff_batch_solve: every instance needs two distinct buffers");
+  hipStream_t s = (hipStream_t)stream;
+  if (!(b->flags & FF_BATCH_GRAPH)) return enqueue_all(b, density, potential, s);
+  if (!s) return set_error(FF_ERR_INVALID, "ff_batch_solve: graph capture needs a non-default stream");
+  const bool same = b->bound && std::equal(b->bd.begin(), b->bd.end(), density) &&
+                    std::equal(b->bp.begin(), b->bp.end(), potential);
+  if (!same || !b->exec || b->graph_stream != s) {
+    drop_graph(b);
+    ff_status st = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (st != FF_OK) return st;
+    st = enqueue_all(b, density, potential, s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (st != FF_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    st = hip_check(e, "end capture");
+    if (st != FF_OK) return st;
+    b->graph = g;
+    st = hip_check(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "graph instantiate");
+    if (st != FF_OK) {
+      drop_graph(b);
+      return st;
+    }
+    b->graph_stream = s;
+    b->bd.assign(density, density + b->count);
+    b->bp.assign(potential, potential + b->count);
+    b->bound = true;
+  }
+  return hip_check(hipGraphLaunch(b->exec, s), "graph launch");
+}
+
+ff_status ff_batch_wait(ff_batch* b, void* stream) {
+  clear_error();
+  if (!b) return set_error(FF_ERR_INVALID, "ff_batch_wait: null batch");
+  return hip_check(hipStreamSynchronize((hipStream_t)stream), "batch synchronize");
+}
+
+ff_status ff_batch_autotune(ff_batch* b, double* const* density, double* const* potential, void* stream, int reps,
+                            int* tuned, double* ms_before, double* ms_after) {
+  clear_error();
+  if (!b || !density || !potential || reps < 1) return set_error(FF_ERR_INVALID, "ff_batch_autotune: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ff_status st = hip_check(hipEventCreate(&e0), "event");
+  if (st == FF_OK) st = hip_check(hipEventCreate(&e1), "event");
+  auto time_batch = [&](double* out) {
+    ff_status r = enqueue_all(b, density, potential, s);  // warm up
+    if (r == FF_OK) r = hip_check(hipEventRecord(e0, s), "event");
+    for (int k = 0; k < reps && r == FF_OK; ++k) r = enqueue_all(b, density, potential, s);
+    if (r == FF_OK) r = hip_check(hipEventRecord(e1, s), "event");
+    if (r == FF_OK) r = hip_check(hipEventSynchronize(e1), "event sync");
+    float ms = 0.f;
+    if (r == FF_OK) r = hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    *out = ms / reps;
+    return r;
+  };
+  double before = 0.0, after = 0.0;
+  if (st == FF_OK) st = time_batch(&before);
+  // every distinct (pass kind, length, precision): time each valid (E, T)
+  int count = 0;
+  std::map<std::tuple<int, int, int>, int> seen;  // key -> instance
+  for (int i = 0; i < b->count && st == FF_OK; ++i) {
+    const ff_plan* p = b->plans[i];
+    const int nprec = p->opt.precision == FF_FP64 ? 1 : 2;
+    for (int prec = 0; prec < nprec; ++prec) {
+      const int M = p->n[2] % 2 ? p->n[2] : p->n[2] / 2;
+      const int keys[4][2] = {{0, M}, {1, p->n[1]}, {2, p->n[0]}, {3, M}};
+      for (auto& kv : keys)
+        if (!seen.count({kv[0], kv[1], prec})) seen[{kv[0], kv[1], prec}] = i;
+    }
+  }
+  for (auto& kv : seen) {
+    if (st != FF_OK) break;
+    const int kind = std::get<0>(kv.first), n = std::get<1>(kv.first), prec = std::get<2>(kv.first);
+    ff_plan* p = b->plans[kv.second];
+    const int eb = prec ? 8 : 16;
+    double best = 1e30;
+    int bE = 0, bT = 0;
+    for (int E : kEValues) {
+      for (int T = 1; T <= 64 && st == FF_OK; T *= 2) {
+        // candidate configuration on a scratch copy of the plan's pass config
+        ff_plan trial = *p;
+        bool ok;
+        if (kind == 0 || kind == 3) {
+          RowCfg c;
+          ok = row_cfg_with(p->n[2], eb, c, E, T) && c.line.n == n;
+          if (ok) (kind == 0 ? trial.zr : trial.zi)[prec] = c;
+        } else {
+          ColCfg c;
+          ok = col_cfg_with(n, p->nh, eb, c, E, T);
+          if (ok) (kind == 1 ? trial.cy : trial.cx)[prec] = c;
+        }
+        if (!ok) continue;
+        const int ip = kv.second;
+        double* src = density[ip];
+        double* dst = potential[ip];
+        auto launch = [&]() -> ff_status {
+          if (kind == 0) return run_zfwd(&trial, prec, 0, src, dst, s);
+          if (kind == 3) return run_zinv(&trial, prec, 0, EPI_STORE, src, dst, nullptr, nullptr, s);
+          return run_col(&trial, prec, kind == 1 ? 1 : 0, kind == 2 ? COL_FILTER : COL_FWD, src, dst, s);
+        };
+        st = launch();
+        if (st == FF_OK) st = hip_check(hipEventRecord(e0, s), "event");
+        for (int r = 0; r < reps && st == FF_OK; ++r) st = launch();
+        if (st == FF_OK) st = hip_check(hipEventRecord(e1, s), "event");
+        if (st == FF_OK) st = hip_check(hipEventSynchronize(e1), "event sync");
+        float ms = 0.f;
+        if (st == FF_OK) st = hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        if (st == FF_OK && ms < best) {
+          best = ms;
+          bE = E;
+          bT = T;
+        }
+      }
+    }
+    if (st == FF_OK && bE) {
+      const ff_tune t{kind, n, prec, bE, bT};
+      st = tune_set(&t);
+      count++;
+    }
+  }
+  if (st == FF_OK) {
+    drop_graph(b);
+    b->bound = false;
+    st = make_plans(b);
+  }
+  if (st == FF_OK) st = time_batch(&after);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st != FF_OK) return st;
+  if (tuned) *tuned = count;
+  if (ms_before) *ms_before = before;
+  if (ms_after) *ms_after = after;
+  return FF_OK;
+}
+
+ff_status ff_batch_describe(const ff_batch* b, char* buf, size_t len) {
+  clear_error();
+  if (!b || !buf || !len) return set_error(FF_ERR_INVALID, "ff_batch_describe: bad arguments");
+  std::ostringstream os;
+  os << "{\"count\":" << b->count << ",\"flags\":" << b->flags << ",\"streams\":" << b->streams.size()
+     << ",\"graph\":" << (b->exec ? 1 : 0) << ",\"plans\":[";
+  char tmp[4096];
+  for (int i = 0; i < b->count; ++i) {
+    if (ff_plan_describe(b->plans[i], tmp, sizeof tmp) != FF_OK) return FF_ERR_INVALID;
+    os << (i ? "," : "") << tmp;
+  }
+  os << "]}";
+  const std::string s = os.str();
+  if (s.size() + 1 > len) return set_error(FF_ERR_INVALID, "ff_batch_describe: buffer too small");
+  memcpy(buf, s.c_str(), s.size() + 1);
+  return FF_OK;
+}
+
+ff_status ff_batch_destroy(ff_batch* b) {
+  if (!b) return FF_OK;
+  drop_graph(b);
+  for (hipStream_t s : b->streams) (void)hipStreamSynchronize(s);
+  destroy_plans(b);
+  for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : b->joins) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  delete b;
+  return FF_OK;
+}
+
+}  // extern "C"

@@ -54,7 +54,7 @@ struct ColArgs {
   int ps;             // LDS pad shift
   int nx, ny;         // extents for the spectral symbol (x pass)
   double ddi, ddj, ddk;
-  double* dc_out;  // SYMBOL mode: receives the forward (0,0,0) term (sum of the input)
+  double* dc_out;  // FILTER / SYMBOL: receives the forward (0,0,0) term (sum of the input)
   const void* tw;
   LinePlan lp;
 };
@@ -100,9 +100,8 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
     const int jj = outer, kk = col;
     const double j2 = sqr(double(jj < a.ny - jj ? jj : a.ny - jj) * a.ddj);
     const double k2 = sqr(double(kk) * a.ddk);
-    if constexpr (MODE == COL_SYMBOL) {
-      if (a.dc_out && valid && r0 == 0 && jj == 0 && kk == 0) *a.dc_out = double(v[0].x);
-    }
+    // the (0,0,0) term before the multiply is the sum of the input field
+    if (a.dc_out && valid && r0 == 0 && jj == 0 && kk == 0) *a.dc_out = double(v[0].x);
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int ii = r0 + i * L;
@@ -223,8 +222,8 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
   }
 }
 
-// Epilogues of the z inverse: store scale*x, store the residual f - scale*x,
-// or accumulate out += scale*x - shift.
+// Epilogues of the z inverse: store scale*x, store the residual
+// f - shift - scale*x, or accumulate out += scale*x - shift.
 enum { EPI_STORE = 0, EPI_RESIDUAL = 1, EPI_ACCUM = 2 };
 
 template <int EPI, typename OUT>
@@ -233,7 +232,7 @@ __device__ __forceinline__ void c2r_out1(OUT* out, const double* f, long long of
   if constexpr (EPI == EPI_STORE) {
     out[off] = OUT(scale * val);
   } else if constexpr (EPI == EPI_RESIDUAL) {
-    out[off] = OUT(f[off] - scale * val);
+    out[off] = OUT((f[off] - shift) - scale * val);
   } else {
     out[off] = OUT(double(out[off]) + (scale * val - shift));
   }
@@ -248,7 +247,7 @@ __device__ __forceinline__ void c2r_out2(OUT* out, const double* f, long long r,
     *o = cx<OUT>{OUT(scale * v0), OUT(scale * v1)};
   } else if constexpr (EPI == EPI_RESIDUAL) {
     const cx<double> fp = reinterpret_cast<const cx<double>*>(f)[r];
-    *o = cx<OUT>{OUT(fp.x - scale * v0), OUT(fp.y - scale * v1)};
+    *o = cx<OUT>{OUT((fp.x - shift) - scale * v0), OUT((fp.y - shift) - scale * v1)};
   } else {
     const cx<OUT> p = *o;
     *o = cx<OUT>{OUT(double(p.x) + (scale * v0 - shift)), OUT(double(p.y) + (scale * v1 - shift))};

@@ -158,6 +158,7 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     p->spec = b;
     p->res = reinterpret_cast<float*>(b + spec);
     p->dc = reinterpret_cast<double*>(b + spec + res);
+    p->dcf = p->dc + 1;
   }
   if (o.precision == FF_FP64 && (o.flags & FF_PRESERVE_INPUT)) {
     st = hip_check(hipMalloc(&p->work, (size_t)p->Nh * 16), "work array hipMalloc");
@@ -271,7 +272,7 @@ static ff_status solve_fp32_refine(const ff_plan* p, const double* f, double* ph
   // phi0: FP32 passes, FP64 in and out
   if ((st = run_zfwd(p, 1, 0, f, spec, s))) return st;
   if ((st = run_col(p, 1, 1, COL_FWD, spec, spec, s))) return st;
-  if ((st = run_col(p, 1, 0, COL_FILTER, spec, spec, s))) return st;
+  if ((st = run_col(p, 1, 0, COL_FILTER, spec, spec, s, p->dcf))) return st;
   if ((st = run_col(p, 1, 1, COL_INV, spec, spec, s))) return st;
   if ((st = run_zinv(p, 1, 0, EPI_STORE, spec, phi, nullptr, nullptr, s))) return st;
   for (int it = 0; it < p->opt.refine_steps; ++it) {
@@ -281,7 +282,9 @@ static ff_status solve_fp32_refine(const ff_plan* p, const double* f, double* ph
     if ((st = run_col(p, 0, 1, COL_FWD, spec, spec, s))) return st;
     if ((st = run_col(p, 0, 0, COL_SYMBOL, spec, spec, s, p->dc))) return st;
     if ((st = run_col(p, 0, 1, COL_INV, spec, spec, s))) return st;
-    if ((st = run_zinv(p, 0, 1, EPI_RESIDUAL, spec, p->res, f, nullptr, s))) return st;
+    // r = (f - mean f) - L_h phi: L_h phi has no mean, and removing f's mean
+    // keeps the FP32 rounding of r relative to the residual itself
+    if ((st = run_zinv(p, 0, 1, EPI_RESIDUAL, spec, p->res, f, p->dcf, s))) return st;
     // correction in FP32: phi += L_h^-1 r - mean(phi)
     if ((st = run_zfwd(p, 1, 1, p->res, spec, s))) return st;
     if ((st = run_col(p, 1, 1, COL_FWD, spec, spec, s))) return st;

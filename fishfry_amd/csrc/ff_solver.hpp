@@ -29,7 +29,8 @@ struct ff_plan {
   size_t scratch_bytes;
   void* spec;   // 16*Nh bytes
   float* res;   // N floats
-  double* dc;   // 1 double
+  double* dc;   // sum of phi (refinement mean removal)
+  double* dcf;  // sum of f (removed from the residual before it is rounded to FP32)
   // FP64 with FF_PRESERVE_INPUT: third work array (16*Nh bytes)
   void* work;
 };

@@ -138,6 +138,47 @@ ff_status ff_tune_query(int kind, int n, int precision, int ncols, ff_tune* out)
 ff_status ff_solve_pass(const ff_plan* plan, int pass, const double* src, double* dst, void* stream);
 
 /* ----------------------------------------------------------------------- */
+/* Batched solves: many independent grids (uneven shapes, FP64 or FP32 +    */
+/* refinement each) launched together.  No reference counterpart (the       */
+/* reference solves one grid per process); each instance follows            */
+/* ParisPencil::solve's contract and numerics.                              */
+
+typedef struct ff_batch_item {
+  int n[3];         /* cells per axis (each >= 2; 2^a 3^b 5^c, nz/2 for even nz) */
+  double lo[3];     /* physical bounds as in ff_plan_create                       */
+  double hi[3];
+  int precision;    /* ff_precision                                               */
+  int refine_steps; /* FF_FP32_REFINE only                                        */
+} ff_batch_item;
+
+#define FF_BATCH_GRAPH 1      /* capture the fork/join launch sequence in a HIP graph on
+                                 first solve and replay it while buffers are unchanged */
+#define FF_BATCH_PERSISTENT 4 /* one persistent kernel over every instance: disabled in
+                                 this build, rejected with FF_ERR_UNSUPPORTED           */
+
+typedef struct ff_batch ff_batch;
+
+/* Plans every instance; the instances run on a few internal streams forked
+ * from, and joined back onto, the caller's stream. */
+ff_status ff_batch_create(int count, const ff_batch_item* items, int flags, ff_batch** out);
+/* Bytes each of instance index's density / potential must hold. */
+size_t ff_batch_bytes(const ff_batch* b, int index);
+/* Enqueue one solve of every instance on stream: density[i] (clobbered for
+ * FP64 instances, preserved for FP32 ones) -> potential[i]; returns after
+ * enqueueing.  FF_BATCH_GRAPH needs a non-default stream. */
+ff_status ff_batch_solve(ff_batch* b, double* const* density, double* const* potential, void* stream);
+ff_status ff_batch_wait(ff_batch* b, void* stream);
+/* Autotuner: times every valid (elements per thread, lines per workgroup)
+ * configuration of each distinct pass of the batch on the given buffers
+ * (reps launches each; the buffers' contents are overwritten), records the
+ * fastest with ff_tune_set and re-plans.  tuned: passes tuned; ms_before /
+ * ms_after: batch solve time before and after. */
+ff_status ff_batch_autotune(ff_batch* b, double* const* density, double* const* potential, void* stream, int reps,
+                            int* tuned, double* ms_before, double* ms_after);
+ff_status ff_batch_describe(const ff_batch* b, char* buf, size_t len);
+ff_status ff_batch_destroy(ff_batch* b);
+
+/* ----------------------------------------------------------------------- */
 /* Driver kernels of FishFry (FishFry.hpp:14-101).                          */
 
 #define FF_MAX_MODES 32

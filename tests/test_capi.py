@@ -98,3 +98,10 @@ def test_field_validation_without_gpu_work():
     with pytest.raises(ff.FishFryError) as e:
         ff.field_init(f, 0)
     assert e.value.status == ff.FF_ERR_INVALID
+
+
+def test_persistent_batch_is_rejected_without_gpu_work():
+    with pytest.raises(ff.FishFryError) as e:
+        ff.Batch([{"n": (8, 8, 8)}], ff.FF_BATCH_PERSISTENT)
+    assert e.value.status == ff.FF_ERR_UNSUPPORTED
+    assert "persistent" in str(e.value)
