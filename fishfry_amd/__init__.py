@@ -73,6 +73,28 @@ class BatchItem(ctypes.Structure):
                 ("precision", ctypes.c_int), ("refine_steps", ctypes.c_int)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("streams", ctypes.c_int), ("max_batch", ctypes.c_int),
+                ("batch_window_us", ctypes.c_double), ("flags", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int * 3), ("precision", ctypes.c_int), ("refine_steps", ctypes.c_int),
+                ("field", ctypes.c_void_p), ("f_host", ctypes.c_void_p), ("phi_host", ctypes.c_void_p),
+                ("want_norms", ctypes.c_int)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("latency_s", ctypes.c_double), ("queue_s", ctypes.c_double),
+                ("norms", ctypes.c_double * 3), ("points", ctypes.c_longlong)]
+
+
+class ServingStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_longlong), ("rejected", ctypes.c_longlong), ("groups", ctypes.c_longlong),
+                ("p50_s", ctypes.c_double), ("p99_s", ctypes.c_double), ("points_per_s", ctypes.c_double),
+                ("arena_bytes", ctypes.c_size_t), ("arena_peak_bytes", ctypes.c_size_t)]
+
+
 FF_BATCH_GRAPH = 1
 FF_BATCH_PERSISTENT = 4
 
@@ -147,6 +169,18 @@ def lib():
     L.ff_batch_describe.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
     L.ff_batch_destroy.restype = ctypes.c_int
     L.ff_batch_destroy.argtypes = [P]
+    L.ff_server_create.restype = ctypes.c_int
+    L.ff_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]
+    L.ff_server_footprint.restype = ctypes.c_size_t
+    L.ff_server_footprint.argtypes = [ctypes.POINTER(Request)]
+    L.ff_server_submit.restype = ctypes.c_int
+    L.ff_server_submit.argtypes = [P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]
+    L.ff_server_wait.restype = ctypes.c_int
+    L.ff_server_wait.argtypes = [P, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(Result)]
+    L.ff_server_stats.restype = ctypes.c_int
+    L.ff_server_stats.argtypes = [P, ctypes.POINTER(ServingStats)]
+    L.ff_server_destroy.restype = ctypes.c_int
+    L.ff_server_destroy.argtypes = [P]
     L.ff_field_default.restype = None
     L.ff_field_default.argtypes = [i3, ctypes.POINTER(Field)]
     L.ff_field_init.restype = ctypes.c_int
@@ -197,7 +231,7 @@ def _stream(stream) -> int:
 
 
 def reference_bytes(n: Sequence[int], m: Sequence[int] = (1, 1, 1)) -> int:
-    """The reference's per-array byte formula (ParisPencil.cpp:108-134)."""
+    """The reference's per-array byte formula (ParisPencil.cpp:49-75)."""
     return int(lib().ff_reference_bytes(_i3(n), _i3(m)))
 
 
@@ -355,6 +389,73 @@ class Batch:
             pass
 
 
+class Server:
+    """Serving loop (ff_server_*): admission by HBM footprint, size-class
+    groups, continuous dispatch over several streams."""
+
+    def __init__(self, hbm_budget: int = 8 << 30, streams: int = 4, max_batch: int = 16,
+                 batch_window_us: float = 200.0):
+        cfg = ServerConfig(hbm_budget, streams, max_batch, batch_window_us, 0)
+        h = ctypes.c_void_p()
+        check(lib().ff_server_create(ctypes.byref(cfg), ctypes.byref(h)), "Server")
+        self._h = h
+        self._keep = {}
+
+    @staticmethod
+    def request(n, precision=FF_FP64, refine_steps=0, field: Optional[Field] = None, f_host=None, phi_host=None,
+                want_norms=False) -> Request:
+        r = Request()
+        r.n = _i3(n)
+        r.precision = precision
+        r.refine_steps = refine_steps
+        r.field = ctypes.cast(ctypes.pointer(field), ctypes.c_void_p) if field is not None else None
+        r.f_host = f_host.ctypes.data if f_host is not None else None
+        r.phi_host = phi_host.ctypes.data if phi_host is not None else None
+        r.want_norms = 1 if want_norms else 0
+        return r
+
+    def footprint(self, n, precision=FF_FP64, refine_steps=0) -> int:
+        r = self.request(n, precision, refine_steps)
+        return int(lib().ff_server_footprint(ctypes.byref(r)))
+
+    def submit(self, n, precision=FF_FP64, refine_steps=0, field: Optional[Field] = None, f_host=None,
+               phi_host=None, want_norms=False) -> int:
+        """f from a device-side field descriptor or a host array (float64, C order)."""
+        import numpy as np
+
+        if f_host is not None:
+            f_host = np.ascontiguousarray(f_host, dtype=np.float64)
+        r = self.request(n, precision, refine_steps, field, f_host, phi_host, want_norms)
+        rid = ctypes.c_uint64()
+        check(lib().ff_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)), "submit")
+        self._keep[rid.value] = (f_host, phi_host)
+        return rid.value
+
+    def wait(self, rid: int, timeout: float = 120.0) -> dict:
+        out = Result()
+        st = lib().ff_server_wait(self._h, rid, timeout, ctypes.byref(out))
+        self._keep.pop(rid, None)
+        check(st, f"request {rid}")
+        return {"status": out.status, "latency_s": out.latency_s, "queue_s": out.queue_s,
+                "norms": tuple(out.norms), "points": out.points}
+
+    def stats(self) -> dict:
+        s = ServingStats()
+        check(lib().ff_server_stats(self._h, ctypes.byref(s)), "stats")
+        return {k: getattr(s, k) for k, _ in ServingStats._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().ff_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def field_default(n: Sequence[int]) -> Field:
     """The benchmark's f on the 1x1x1 grid (FishFry.hpp:108-139)."""
     f = Field()
@@ -396,7 +497,7 @@ def field_norms(field: Field, phi, stream=None) -> Tuple[float, float, float]:
 from .driver import FishFry  # noqa: E402  (needs the names above)
 
 __all__ = [
-    "ParisPencil", "FishFry", "Batch", "FF_BATCH_GRAPH", "FF_BATCH_PERSISTENT", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
+    "ParisPencil", "FishFry", "Batch", "Server", "FF_BATCH_GRAPH", "FF_BATCH_PERSISTENT", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
     "field_norms", "reference_bytes", "benchmark_grid", "FishFryError", "lib", "LIB_PATH",
     "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_BLOCKED_LAYOUT",
 ]

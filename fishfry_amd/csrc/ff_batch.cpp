@@ -243,8 +243,8 @@ ff_status build_chain(const BatchCore& b, int i, double* density, double* potent
   base.nz = nz;
   base.nh = nh;
   base.odd = odd;
-  base.scale = 1.0 / (double(nx) * double(ny) * double(nz));  // ParisPencil.cpp:467
-  base.ddi = dd_const(nx, it.lo[0], it.hi[0]);               // ParisPencil.cpp:68-70
+  base.scale = 1.0 / (double(nx) * double(ny) * double(nz));  // ParisPencil.cpp:408
+  base.ddi = dd_const(nx, it.lo[0], it.hi[0]);               // ParisPencil.cpp:9-11
   base.ddj = dd_const(ny, it.lo[1], it.hi[1]);
   base.ddk = dd_const(nz, it.lo[2], it.hi[2]);
   base.dep = -1;

@@ -92,7 +92,7 @@ __device__ void item_col(const BOp& o, long long tile, unsigned char* smem) {
   if (o.mode == COL_INV) X = lds_fft<F, +1>(A, B, T, o.pitch, o.line, tw);
   else X = lds_fft<F, -1>(A, B, T, o.pitch, o.line, tw);
   if (o.mode == COL_FILTER || o.mode == COL_SYMBOL) {
-    // spectral filter of ParisPencil.cpp:328-343 at (i, j, kh) = (r, outer, c0 + c)
+    // spectral filter of ParisPencil.cpp:269-284 at (i, j, kh) = (r, outer, c0 + c)
     const uint32_t ninv = qinv(n);
     const int jj = outer;
     const double j2 = sqr(double(jj < o.ny - jj ? jj : o.ny - jj) * o.ddj);

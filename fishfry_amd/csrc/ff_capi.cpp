@@ -87,8 +87,8 @@ ff_status ff_solve(const ff_plan* plan, size_t bytes, double* density, double* p
   if (st == FF_OK) st = hip_check(hipStreamSynchronize(s), "solve synchronize");
   const double t1 = now_seconds();
   if (st == FF_OK && stamps) {
-    // the reference's 14 stamps (ParisPencil.cpp:157,201,203,247,249,294,296,
-    // 370,372,416,418,461,463,483); its reorders and self all-to-alls are
+    // the reference's 14 stamps (ParisPencil.cpp:98,142,144,188,190,235,237,
+    // 311,313,357,359,402,404,424); its reorders and self all-to-alls are
     // fused away here, so they take zero time at the end of their pass
     static const char* names[FF_SOLVE_STAMPS] = {"Total",
                                                  "Reorder 3D block into sub-pencils",

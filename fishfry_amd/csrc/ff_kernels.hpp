@@ -2,14 +2,14 @@
 //
 // Data layout in HBM (all passes): the half spectrum is complex [i][j][kh],
 // kh fastest, row pitch nh = nz/2+1 -- the layout the reference has after its
-// D2Z (ParisPencil.cpp:224) -- so no pass ever transposes.  The z passes work
+// D2Z (ParisPencil.cpp:165) -- so no pass ever transposes.  The z passes work
 // on contiguous rows; the y and x passes work on strided columns, a workgroup
 // owning a tile of T neighbouring kh columns so every row of the tile is one
 // contiguous T*sizeof(complex) segment in HBM.
 //
-//   k_r2c   z forward, real rows -> half-spectrum rows       (ParisPencil.cpp:224)
-//   k_col   y forward / inverse, or x forward*filter*inverse (:271, :318-347, :393)
-//   k_c2r   z inverse + the 1/(nx*ny*nz) scale                (:439, :466-481)
+//   k_r2c   z forward, real rows -> half-spectrum rows       (ParisPencil.cpp:165)
+//   k_col   y forward / inverse, or x forward*filter*inverse (:212, :259-288, :334)
+//   k_c2r   z inverse + the 1/(nx*ny*nz) scale                (:380, :406-421)
 #pragma once
 
 #include "ff_fft.hpp"
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
     fft_line<E, +1, F>(v, a.lp, tw, lds, idx, r0);
   } else {
     fft_line<E, -1, F>(v, a.lp, tw, lds, idx, r0);
-    // spectral filter of ParisPencil.cpp:328-343 at (i, j, kh) = (r, outer, col)
+    // spectral filter of ParisPencil.cpp:269-284 at (i, j, kh) = (r, outer, col)
     const int jj = outer, kk = col;
     const double j2 = sqr(double(jj < a.ny - jj ? jj : a.ny - jj) * a.ddj);
     const double k2 = sqr(double(kk) * a.ddk);

@@ -1,0 +1,518 @@
+// Serving loop: a request queue in front of the solver with admission by
+// HBM footprint and continuous batching.
+//
+// One scheduler thread owns every HIP call.  Device memory for requests comes
+// from one resident arena allocated when the server starts (first-fit
+// sub-allocation on the host), so steady-state serving never calls hipMalloc
+// or hipFree.  Plans and twiddle tables are cached per grid shape and stay
+// resident.  Pending requests are grouped by size class; a group is
+// dispatched as soon as its buffers fit in the arena and a stream is free, so
+// requests that arrive while others run are admitted as memory frees up and
+// their kernels overlap with the groups already on the device.  Every
+// request's solve uses only its own buffers and the same kernels and
+// configurations whatever else is running, so its result is bit-identical
+// alone and in a mix.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "ff_solver.hpp"
+
+using namespace ff;
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// First-fit allocator over [0, size) with coalescing free list.
+class Arena {
+ public:
+  void reset(size_t size) {
+    size_ = size;
+    free_.clear();
+    free_[0] = size;
+    used_ = 0;
+    peak_ = 0;
+  }
+  bool alloc(size_t bytes, size_t& off) {
+    bytes = align_up(std::max<size_t>(bytes, 256), 256);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        off = it->first;
+        const size_t rest = it->second - bytes;
+        free_.erase(it);
+        if (rest) free_[off + bytes] = rest;
+        used_ += bytes;
+        peak_ = std::max(peak_, used_);
+        sizes_[off] = bytes;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off) {
+    auto s = sizes_.find(off);
+    if (s == sizes_.end()) return;
+    size_t bytes = s->second;
+    sizes_.erase(s);
+    used_ -= bytes;
+    auto next = free_.lower_bound(off);
+    if (next != free_.end() && next->first == off + bytes) {
+      bytes += next->second;
+      free_.erase(next);
+    }
+    auto it = free_.lower_bound(off);
+    if (it != free_.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == off) {
+        prev->second += bytes;
+        return;
+      }
+    }
+    free_[off] = bytes;
+  }
+  size_t size() const { return size_; }
+  size_t used() const { return used_; }
+  size_t peak() const { return peak_; }
+
+ private:
+  size_t size_ = 0, used_ = 0, peak_ = 0;
+  std::map<size_t, size_t> free_;   // offset -> length
+  std::map<size_t, size_t> sizes_;  // allocated offset -> length
+};
+
+struct Req {
+  uint64_t id = 0;
+  ff_request r{};
+  ff_field field{};
+  bool has_field = false;
+  double t_submit = 0, t_dispatch = 0, t_done = 0;
+  size_t off_d = 0, off_p = 0, off_s = 0;
+  bool admitted = false;  // off_d / off_p (and off_s if has_s) hold arena blocks
+  bool has_s = false;
+  size_t bytes = 0, scratch = 0;
+  ff_status status = FF_OK;
+  double norms[3] = {0, 0, 0};
+  bool done = false;
+  int cls = 0;
+};
+
+struct Group {
+  std::vector<Req*> reqs;
+  int stream = 0;
+  hipEvent_t ev = nullptr;
+};
+
+}  // namespace
+
+struct ff_server {
+  ff_server_config cfg{};
+  int device = 0;
+  void* arena_mem = nullptr;
+  Arena arena;
+  std::vector<hipStream_t> streams;
+  std::vector<int> stream_load;
+  std::vector<hipEvent_t> free_events;
+  // plans by (extents, precision, refinement steps, lo, hi)
+  std::map<std::tuple<int, int, int, int, int, double, double, double, double, double, double>, ff_plan*> plans;
+  std::mutex mu;
+  std::condition_variable cv_submit, cv_done;
+  std::deque<Req*> incoming;
+  std::vector<Req*> pending;
+  std::vector<Group> inflight;
+  std::map<uint64_t, Req*> all;
+  std::atomic<bool> stop{false};
+  std::thread worker;
+  uint64_t next_id = 1;
+  // statistics (under mu)
+  std::vector<double> latencies;
+  double first_submit = 0, last_done = 0, points = 0;
+  long long completed = 0, rejected = 0, dispatched_groups = 0;
+  std::string fatal;
+};
+
+namespace {
+
+int size_class(const ff_request& r) {
+  const double n = double(r.n[0]) * r.n[1] * r.n[2];
+  const int lg = (int)std::floor(std::log2(std::max(1.0, n)) / 3.0);  // edge octave
+  return (r.precision * 16 + std::min(r.refine_steps, 15)) * 64 + lg;
+}
+
+ff_plan* plan_for(ff_server* s, const Req& q, ff_status& st) {
+  // The domain of a field request is the field's own; a host-array request
+  // is on the benchmark's grid, [-pi, pi) per axis (FishFry.hpp:108-139).
+  double lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    const double d = q.has_field ? q.field.d[a] : 2.0 * M_PI / q.r.n[a];
+    lo[a] = q.has_field ? q.field.lo[a] : -M_PI;
+    hi[a] = q.has_field ? lo[a] + d * (q.r.n[a] - 1) : M_PI - d;
+  }
+  const auto key = std::make_tuple(q.r.n[0], q.r.n[1], q.r.n[2], q.r.precision, q.r.refine_steps, lo[0], lo[1],
+                                   lo[2], hi[0], hi[1], hi[2]);
+  auto it = s->plans.find(key);
+  if (it != s->plans.end()) return it->second;
+  ff_options o{q.r.precision, q.r.refine_steps, 0};
+  ff_plan* p = nullptr;
+  st = plan_create(q.r.n, lo, hi, &o, &p);
+  if (st != FF_OK) return nullptr;
+  s->plans[key] = p;
+  return p;
+}
+
+// Frees the request's memory, then publishes its result (after which a
+// waiter may delete it).  count: include it in the latency statistics.
+void finish(ff_server* s, Req* q, ff_status st, bool count) {
+  if (q->admitted) {
+    if (q->has_s) s->arena.release(q->off_s);
+    s->arena.release(q->off_d);
+    s->arena.release(q->off_p);
+    q->admitted = false;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  q->status = st;
+  q->t_done = now_s();
+  if (count) {
+    s->latencies.push_back(q->t_done - q->t_submit);
+    s->points += double(q->r.n[0]) * q->r.n[1] * q->r.n[2];
+    s->last_done = std::max(s->last_done, q->t_done);
+    s->completed++;
+  }
+  q->done = true;
+}
+
+// Admit and dispatch one size class group on stream k.
+void dispatch(ff_server* s, std::vector<Req*>& group, int k) {
+  hipStream_t hs = s->streams[k];
+  unsigned char* base = static_cast<unsigned char*>(s->arena_mem);
+  Group g;
+  g.stream = k;
+  for (Req* q : group) {
+    ff_status st = FF_OK;
+    ff_plan* cached = plan_for(s, *q, st);
+    if (!cached) {
+      finish(s, q, st, false);
+      continue;
+    }
+    ff_plan p = *cached;  // shares twiddles; the work area is the request's own
+    if (q->has_s) plan_bind_scratch(&p, base + q->off_s);
+    double* d = reinterpret_cast<double*>(base + q->off_d);
+    double* phi = reinterpret_cast<double*>(base + q->off_p);
+    if (q->has_field) {
+      st = field_init(&q->field, d, hs);
+    } else {
+      st = hip_check(hipMemcpyAsync(d, q->r.f_host, sizeof(double) * q->r.n[0] * q->r.n[1] * q->r.n[2],
+                                    hipMemcpyHostToDevice, hs),
+                     "request input copy");
+    }
+    if (st == FF_OK) st = solve_enqueue(&p, d, phi, hs, nullptr);
+    if (st == FF_OK && q->r.phi_host)
+      st = hip_check(hipMemcpyAsync(q->r.phi_host, phi, sizeof(double) * q->r.n[0] * q->r.n[1] * q->r.n[2],
+                                    hipMemcpyDeviceToHost, hs),
+                     "request output copy");
+    if (st != FF_OK) {
+      finish(s, q, st, false);
+      continue;
+    }
+    q->t_dispatch = now_s();
+    g.reqs.push_back(q);
+  }
+  if (g.reqs.empty()) return;
+  if (!s->free_events.empty()) {
+    g.ev = s->free_events.back();
+    s->free_events.pop_back();
+  } else if (hipEventCreateWithFlags(&g.ev, hipEventDisableTiming) != hipSuccess) {
+    (void)hipStreamSynchronize(hs);
+    for (Req* q : g.reqs) finish(s, q, FF_ERR_HIP, false);
+    return;
+  }
+  (void)hipEventRecord(g.ev, hs);
+  s->stream_load[k]++;
+  s->dispatched_groups++;
+  s->inflight.push_back(g);
+}
+
+void complete(ff_server* s, Group& g) {
+  for (Req* q : g.reqs) {
+    ff_status st = FF_OK;
+    if (q->r.want_norms && q->has_field) {
+      unsigned char* base = static_cast<unsigned char*>(s->arena_mem);
+      double e[3], z[3];
+      std::lock_guard<std::mutex> sg(scratch_mutex());
+      st = field_diff(&q->field, reinterpret_cast<double*>(base + q->off_p), e, s->streams[g.stream]);
+      if (st == FF_OK) st = field_diff(&q->field, nullptr, z, s->streams[g.stream]);
+      if (st == FF_OK) {
+        q->norms[0] = e[0] / z[0];
+        q->norms[1] = std::sqrt(e[1] / z[1]);
+        q->norms[2] = e[2] / z[2];
+      }
+    }
+    finish(s, q, st, true);
+  }
+  s->stream_load[g.stream]--;
+  s->free_events.push_back(g.ev);
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->cv_done.notify_all();
+}
+
+void worker_main(ff_server* s) {
+  (void)hipSetDevice(s->device);
+  const double window = s->cfg.batch_window_us * 1e-6;
+  while (true) {
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      if (s->incoming.empty() && s->pending.empty() && s->inflight.empty()) {
+        if (s->stop) break;
+        s->cv_submit.wait_for(lk, std::chrono::milliseconds(5));
+      }
+      while (!s->incoming.empty()) {
+        s->pending.push_back(s->incoming.front());
+        s->incoming.pop_front();
+      }
+    }
+    // completions
+    for (size_t i = 0; i < s->inflight.size();) {
+      if (hipEventQuery(s->inflight[i].ev) == hipSuccess) {
+        Group g = s->inflight[i];
+        s->inflight.erase(s->inflight.begin() + i);
+        complete(s, g);
+      } else {
+        ++i;
+      }
+    }
+    // admission and dispatch, oldest class first
+    const size_t before = s->pending.size();
+    const double t = now_s();
+    std::map<int, std::vector<Req*>> classes;
+    for (Req* q : s->pending) classes[q->cls].push_back(q);
+    std::vector<std::pair<double, int>> order;
+    for (auto& c : classes) order.push_back({c.second.front()->t_submit, c.first});
+    std::sort(order.begin(), order.end());
+    for (auto& oc : order) {
+      std::vector<Req*>& reqs = classes[oc.second];
+      const bool full = (int)reqs.size() >= s->cfg.max_batch;
+      const bool due = t - reqs.front()->t_submit >= window || s->inflight.empty();
+      if (!full && !due) continue;
+      int k = 0;
+      for (int j = 1; j < (int)s->streams.size(); ++j)
+        if (s->stream_load[j] < s->stream_load[k]) k = j;
+      if (s->stream_load[k] >= 2) break;  // every stream has two groups queued
+      std::vector<Req*> group;
+      for (Req* q : reqs) {
+        if ((int)group.size() >= s->cfg.max_batch) break;
+        size_t od, op, os = 0;
+        if (!s->arena.alloc(q->bytes, od)) continue;
+        if (!s->arena.alloc(q->bytes, op)) {
+          s->arena.release(od);
+          continue;
+        }
+        if (q->scratch && !s->arena.alloc(q->scratch, os)) {
+          s->arena.release(od);
+          s->arena.release(op);
+          continue;
+        }
+        q->off_d = od;
+        q->off_p = op;
+        q->off_s = os;
+        q->has_s = q->scratch != 0;
+        q->admitted = true;
+        group.push_back(q);
+      }
+      if (group.empty()) continue;  // waits for memory to free up
+      for (Req* q : group) s->pending.erase(std::find(s->pending.begin(), s->pending.end(), q));
+      dispatch(s, group, k);
+    }
+    // nothing could be admitted (memory or streams busy) or only in-flight
+    // work remains: poll again shortly instead of spinning
+    if (!s->inflight.empty() && (s->pending.empty() || s->pending.size() == before))
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->cv_done.notify_all();
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+ff_status ff_server_create(const ff_server_config* cfg, ff_server** out) {
+  clear_error();
+  if (!out) return set_error(FF_ERR_INVALID, "ff_server_create: null output");
+  *out = nullptr;
+  ff_server* s = new (std::nothrow) ff_server();
+  if (!s) return set_error(FF_ERR_NOMEM, "host allocation failed");
+  ff_server_config c{size_t(8) << 30, 4, 16, 200.0, 0};
+  if (cfg) c = *cfg;
+  if (c.streams < 1 || c.streams > 16 || c.max_batch < 1 || c.hbm_budget < (size_t(1) << 20) ||
+      c.batch_window_us < 0) {
+    delete s;
+    return set_error(FF_ERR_INVALID, "ff_server_create: streams 1..16, max_batch >= 1, budget >= 1 MiB");
+  }
+  s->cfg = c;
+  ff_status st = hip_check(hipGetDevice(&s->device), "hipGetDevice");
+  if (st == FF_OK) st = hip_check(hipMalloc(&s->arena_mem, c.hbm_budget), "serving arena hipMalloc");
+  for (int k = 0; k < c.streams && st == FF_OK; ++k) {
+    hipStream_t hs = nullptr;
+    st = hip_check(hipStreamCreateWithFlags(&hs, hipStreamNonBlocking), "serving stream");
+    if (st == FF_OK) {
+      s->streams.push_back(hs);
+      s->stream_load.push_back(0);
+    }
+  }
+  if (st != FF_OK) {
+    for (hipStream_t hs : s->streams) (void)hipStreamDestroy(hs);
+    if (s->arena_mem) (void)hipFree(s->arena_mem);
+    delete s;
+    return st;
+  }
+  s->arena.reset(c.hbm_budget);
+  s->worker = std::thread(worker_main, s);
+  *out = s;
+  return FF_OK;
+}
+
+size_t ff_server_footprint(const ff_request* r) {
+  if (!r) return 0;
+  for (int a = 0; a < 3; ++a)
+    if (r->n[a] < 2) return 0;
+  const int ones[3] = {1, 1, 1};
+  size_t b = 2 * align_up(reference_bytes(r->n, ones), 256);
+  if (r->precision == FF_FP32_REFINE) {
+    const size_t N = (size_t)r->n[0] * r->n[1] * r->n[2];
+    const size_t Nh = (size_t)r->n[0] * r->n[1] * (r->n[2] / 2 + 1);
+    b += align_up(Nh * 16 + align_up(N * 4, 256) + 256, 256);
+  }
+  return b;
+}
+
+ff_status ff_server_submit(ff_server* s, const ff_request* r, uint64_t* id) {
+  clear_error();
+  if (!s || !r || !id) return set_error(FF_ERR_INVALID, "ff_server_submit: null argument");
+  if ((r->field == nullptr) == (r->f_host == nullptr))
+    return set_error(FF_ERR_INVALID, "ff_server_submit: give exactly one of field and f_host");
+  if (r->precision != FF_FP64 && r->precision != FF_FP32_REFINE)
+    return set_error(FF_ERR_INVALID, "ff_server_submit: bad precision");
+  if (r->refine_steps < 0 || r->refine_steps > 8) return set_error(FF_ERR_INVALID, "ff_server_submit: bad steps");
+  for (int a = 0; a < 3; ++a)
+    if (r->n[a] < 2) return set_error(FF_ERR_INVALID, "ff_server_submit: extents must be >= 2");
+  if (r->field) {
+    ff_status st = field_check(r->field);
+    if (st != FF_OK) return st;
+    for (int a = 0; a < 3; ++a)
+      if (r->field->n[a] != r->n[a]) return set_error(FF_ERR_INVALID, "ff_server_submit: field extents differ");
+  }
+  const size_t need = ff_server_footprint(r);
+  if (need > s->arena.size()) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->rejected++;
+    return set_error(FF_ERR_REJECTED, "ff_server_submit: request larger than the serving HBM budget");
+  }
+  Req* q = new (std::nothrow) Req();
+  if (!q) return set_error(FF_ERR_NOMEM, "host allocation failed");
+  q->r = *r;
+  if (r->field) {
+    q->field = *r->field;
+    q->has_field = true;
+    q->r.field = &q->field;
+  }
+  const int ones[3] = {1, 1, 1};
+  q->bytes = reference_bytes(r->n, ones);
+  if (r->precision == FF_FP32_REFINE) {
+    const size_t N = (size_t)r->n[0] * r->n[1] * r->n[2];
+    const size_t Nh = (size_t)r->n[0] * r->n[1] * (r->n[2] / 2 + 1);
+    q->scratch = Nh * 16 + align_up(N * 4, 256) + 256;
+  }
+  q->cls = size_class(*r);
+  q->t_submit = now_s();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    q->id = s->next_id++;
+    if (!s->first_submit) s->first_submit = q->t_submit;
+    s->all[q->id] = q;
+    s->incoming.push_back(q);
+  }
+  s->cv_submit.notify_one();
+  *id = q->id;
+  return FF_OK;
+}
+
+ff_status ff_server_wait(ff_server* s, uint64_t id, double timeout_s, ff_result* out) {
+  clear_error();
+  if (!s || !out) return set_error(FF_ERR_INVALID, "ff_server_wait: null argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->all.find(id);
+  if (it == s->all.end()) return set_error(FF_ERR_NOT_FOUND, "ff_server_wait: unknown request id");
+  Req* q = it->second;
+  const auto deadline = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
+  while (!q->done) {
+    if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout && !q->done)
+      return set_error(FF_ERR_TIMEOUT, "ff_server_wait: timed out");
+  }
+  out->status = q->status;
+  out->latency_s = q->t_done - q->t_submit;
+  out->queue_s = q->t_dispatch - q->t_submit;
+  out->points = (long long)q->r.n[0] * q->r.n[1] * q->r.n[2];
+  for (int k = 0; k < 3; ++k) out->norms[k] = q->norms[k];
+  s->all.erase(it);
+  delete q;
+  const ff_status rs = (ff_status)out->status;
+  return rs == FF_OK ? FF_OK : set_error(rs, "request failed");
+}
+
+ff_status ff_server_stats(ff_server* s, ff_serving_stats* out) {
+  clear_error();
+  if (!s || !out) return set_error(FF_ERR_INVALID, "ff_server_stats: null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::vector<double> l = s->latencies;
+  std::sort(l.begin(), l.end());
+  auto pct = [&](double qv) {
+    if (l.empty()) return 0.0;
+    const double pos = qv * double(l.size() - 1);
+    const size_t a = (size_t)pos;
+    const size_t b = std::min(a + 1, l.size() - 1);
+    return l[a] + (pos - double(a)) * (l[b] - l[a]);
+  };
+  out->completed = s->completed;
+  out->rejected = s->rejected;
+  out->groups = s->dispatched_groups;
+  out->p50_s = pct(0.5);
+  out->p99_s = pct(0.99);
+  out->points_per_s = (s->last_done > s->first_submit) ? s->points / (s->last_done - s->first_submit) : 0.0;
+  out->arena_bytes = s->arena.size();
+  out->arena_peak_bytes = s->arena.peak();
+  return FF_OK;
+}
+
+ff_status ff_server_destroy(ff_server* s) {
+  if (!s) return FF_OK;
+  s->stop = true;
+  s->cv_submit.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  for (hipStream_t hs : s->streams) (void)hipStreamSynchronize(hs);
+  for (auto& kv : s->plans) plan_destroy(kv.second);
+  for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
+  for (hipStream_t hs : s->streams) (void)hipStreamDestroy(hs);
+  if (s->arena_mem) (void)hipFree(s->arena_mem);
+  for (auto& kv : s->all) delete kv.second;
+  delete s;
+  return FF_OK;
+}
+
+}  // extern "C"

@@ -1,10 +1,10 @@
 // Plan construction and the pass sequence of the periodic Poisson solve.
 //
-// At 1x1x1 tasks the reference's 25 steps (ParisPencil.cpp:154-484) are 6 FFT
+// At 1x1x1 tasks the reference's 25 steps (ParisPencil.cpp:95-425) are 6 FFT
 // executions, one filter, one scale and 17 reorders / self all-to-alls.  Here
 // they are five kernels, each reading and writing the whole array once:
-//   z forward (D2Z, :224), y forward (:271), x forward*filter*inverse
-//   (:318,:322-344,:347), y inverse (:393), z inverse (Z2D, :439) with the
+//   z forward (D2Z, :165), y forward (:212), x forward*filter*inverse
+//   (:259,:263-285,:288), y inverse (:334), z inverse (Z2D, :380) with the
 //   1/(nx*ny*nz) scale of :466-481 in its epilogue.
 #include <algorithm>
 #include <climits>
@@ -18,7 +18,7 @@
 namespace ff {
 
 size_t reference_bytes(const int n[3], const int m[3]) {
-  // ParisPencil.cpp:77 (nh), :83-85 (mp x mq split of mk), :110-134
+  // ParisPencil.cpp:18 (nh), :24-26 (mp x mq split of mk), :51-75
   const long nh = n[2] / 2 + 1;
   const long mi = m[0], mj = m[1], mk = m[2];
   long mq = std::lround(std::sqrt((double)mk));
@@ -41,7 +41,7 @@ size_t reference_bytes(const int n[3], const int m[3]) {
 }
 
 static double dd_const(int n, double lo, double hi) {
-  // ParisPencil.cpp:68-70, same operand order
+  // ParisPencil.cpp:9-11, same operand order
   return 2.0 * M_PI * double(n - 1) / (double(n) * (hi - lo));
 }
 
@@ -154,11 +154,7 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
       plan_destroy(p);
       return st;
     }
-    unsigned char* b = static_cast<unsigned char*>(p->scratch);
-    p->spec = b;
-    p->res = reinterpret_cast<float*>(b + spec);
-    p->dc = reinterpret_cast<double*>(b + spec + res);
-    p->dcf = p->dc + 1;
+    plan_bind_scratch(p, p->scratch);
   }
   if (o.precision == FF_FP64 && (o.flags & FF_PRESERVE_INPUT)) {
     st = hip_check(hipMalloc(&p->work, (size_t)p->Nh * 16), "work array hipMalloc");
@@ -169,6 +165,21 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   }
   *out = p;
   return FF_OK;
+}
+
+size_t plan_scratch_bytes(const ff_plan* p) {
+  if (p->opt.precision != FF_FP32_REFINE) return 0;
+  return (size_t)p->Nh * 16 + ((size_t)p->N * 4 + 255) / 256 * 256 + 256;
+}
+
+void plan_bind_scratch(ff_plan* p, void* base) {
+  unsigned char* b = static_cast<unsigned char*>(base);
+  const size_t spec = (size_t)p->Nh * 16;
+  const size_t res = ((size_t)p->N * 4 + 255) / 256 * 256;
+  p->spec = b;
+  p->res = reinterpret_cast<float*>(b + spec);
+  p->dc = reinterpret_cast<double*>(b + spec + res);
+  p->dcf = p->dc + 1;
 }
 
 void plan_destroy(ff_plan* p) {
@@ -195,7 +206,7 @@ static RowArgs row_args(const ff_plan* p, int prec, bool inverse) {
   a.nblocks = (int)((a.nrows + c.T - 1) / c.T);
   a.ps = c.ps;
   a.pitch = c.pitch;
-  a.scale = 1.0 / (double(p->n[0]) * double(p->n[1]) * double(p->n[2]));  // ParisPencil.cpp:467
+  a.scale = 1.0 / (double(p->n[0]) * double(p->n[1]) * double(p->n[2]));  // ParisPencil.cpp:408
   a.shift = 0.0;
   a.shiftp = nullptr;
   a.tw = p->tw_z[prec];
@@ -238,11 +249,11 @@ ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* sr
   a.ncols = p->nh;
   const long long Bnh = (long long)p->nh << p->lgB;
   const long long blk = (long long)p->n[1] * Bnh;  // one block of 2^lgB x-planes
-  if (axis == 1) {  // y: planes i, positions j   (ParisPencil.cpp:138 plan, batch nx*nh)
+  if (axis == 1) {  // y: planes i, positions j   (ParisPencil.cpp:79 plan, batch nx*nh)
     a.nouter = p->n[0];
     a.so = Stride2{blk, p->nh, p->lgB};
     a.sr = Stride2{0, Bnh, 30};
-  } else {  // x: planes j, positions i          (ParisPencil.cpp:137 plan, batch ny*nh)
+  } else {  // x: planes j, positions i          (ParisPencil.cpp:78 plan, batch ny*nh)
     a.nouter = p->n[1];
     a.so = Stride2{0, Bnh, 30};
     a.sr = Stride2{blk, p->nh, p->lgB};
@@ -300,7 +311,7 @@ ff_status solve_enqueue(const ff_plan* p, double* density, double* potential, hi
   ff_status st;
   // ping-pong density <-> potential (or potential <-> the plan's work array
   // with FF_PRESERVE_INPUT); the result lands in potential as in
-  // ParisPencil.cpp:162-163,478
+  // ParisPencil.cpp:103-104,419
   void* w = p->work ? p->work : (void*)density;
   if ((st = run_zfwd(p, 0, 0, density, potential, s))) return st;
   if (ev && (st = hip_check(hipEventRecord(ev[0], s), "event"))) return st;

@@ -51,6 +51,10 @@ ff_status solve_enqueue(const ff_plan* p, double* density, double* potential, hi
 ff_status plan_create(const int n[3], const double lo[3], const double hi[3], const ff_options* opts,
                       ff_plan** out);
 void plan_destroy(ff_plan* p);
+// FP32 refinement work area: its size, and pointing a plan (or a copy of one)
+// at a work area of that size.
+size_t plan_scratch_bytes(const ff_plan* p);
+void plan_bind_scratch(ff_plan* p, void* base);
 size_t reference_bytes(const int n[3], const int m[3]);
 
 }  // namespace ff

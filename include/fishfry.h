@@ -64,28 +64,28 @@ typedef struct ff_stamp {
   double seconds;
 } ff_stamp;
 
-#define FF_SOLVE_STAMPS 14 /* stamps written by ff_solve, ParisPencil.cpp:157-483 */
+#define FF_SOLVE_STAMPS 14 /* stamps written by ff_solve, ParisPencil.cpp:98-424 */
 
 typedef struct ff_plan ff_plan;
 
 /* ParisPencil::ParisPencil(n, lo, hi, m, id) -- ParisPencil.hpp:23,
- * ParisPencil.cpp:66-141.  n: global cells per axis; lo/hi: physical bounds
+ * ParisPencil.cpp:7-82.  n: global cells per axis; lo/hi: physical bounds
  * (hi is one cell below the periodic end); m: tasks per axis, must be {1,1,1}
  * (FF_ERR_UNSUPPORTED otherwise); id must be {0,0,0}.  opts may be NULL (FP64).
  * Builds radix plans and twiddle tables on the current HIP device. */
 ff_status ff_plan_create(const int n[3], const double lo[3], const double hi[3], const int m[3],
                          const int id[3], const ff_options* opts, ff_plan** out);
 
-/* ParisPencil::bytes() -- ParisPencil.hpp:30, formula ParisPencil.cpp:108-134.
+/* ParisPencil::bytes() -- ParisPencil.hpp:30, formula ParisPencil.cpp:49-75.
  * Bytes each of density and potential must hold.  0 for a NULL plan. */
 size_t ff_plan_bytes(const ff_plan* plan);
 
 /* ParisPencil::solve(bytes, density, potential, stamps) -- ParisPencil.hpp:44,
- * ParisPencil.cpp:154-484.  density: [nx][ny][nz] FP64 input, clobbered
+ * ParisPencil.cpp:95-425.  density: [nx][ny][nz] FP64 input, clobbered
  * (FP64 mode) or preserved (FP32 modes); potential: receives phi in the same
  * layout.  bytes < ff_plan_bytes() -> FF_ERR_BYTES.  Work is enqueued on
  * stream and the call returns after it completes (the reference ends with a
- * device synchronize, ParisPencil.cpp:482).  stamps may be NULL; otherwise
+ * device synchronize, ParisPencil.cpp:423).  stamps may be NULL; otherwise
  * it receives up to *n_stamps entries (FF_SOLVE_STAMPS with the reference's
  * phase names) and *n_stamps is set to the count written. */
 ff_status ff_solve(const ff_plan* plan, size_t bytes, double* density, double* potential, void* stream,
@@ -95,14 +95,14 @@ ff_status ff_solve(const ff_plan* plan, size_t bytes, double* density, double* p
  * stamps): for callers that overlap solves or capture them in a HIP graph. */
 ff_status ff_solve_async(const ff_plan* plan, size_t bytes, double* density, double* potential, void* stream);
 
-/* ParisPencil::~ParisPencil -- ParisPencil.cpp:143-152.  NULL is a no-op. */
+/* ParisPencil::~ParisPencil -- ParisPencil.cpp:84-93.  NULL is a no-op. */
 ff_status ff_plan_destroy(ff_plan* plan);
 
 /* Thread-local text of the last error ("" if none). */
 const char* ff_last_error(void);
 
 /* The reference's per-array byte formula for any task grid m
- * (ParisPencil.cpp:83-85,108-134), without creating a plan. */
+ * (ParisPencil.cpp:24-26,49-75), without creating a plan. */
 size_t ff_reference_bytes(const int n[3], const int m[3]);
 
 /* Human-readable JSON description of a plan (radix plans, tiles, bytes). */
@@ -177,6 +177,62 @@ ff_status ff_batch_autotune(ff_batch* b, double* const* density, double* const* 
                             int* tuned, double* ms_before, double* ms_after);
 ff_status ff_batch_describe(const ff_batch* b, char* buf, size_t len);
 ff_status ff_batch_destroy(ff_batch* b);
+
+/* ----------------------------------------------------------------------- */
+/* Serving: a request loop with admission by HBM footprint and continuous    */
+/* batching (no reference counterpart).                                      */
+
+typedef struct ff_server ff_server;
+
+typedef struct ff_server_config {
+  size_t hbm_budget;      /* device bytes reserved at start for request buffers      */
+  int streams;            /* device streams groups are spread over (1..16)            */
+  int max_batch;          /* requests dispatched together per size class              */
+  double batch_window_us; /* longest wait for same-class companions while busy        */
+  int flags;              /* reserved, 0                                              */
+} ff_server_config;
+
+struct ff_field;
+
+typedef struct ff_request {
+  int n[3];                     /* grid extents (benchmark grid: lo = -pi, 2pi period) */
+  int precision;                /* ff_precision                                        */
+  int refine_steps;             /* FF_FP32_REFINE only                                 */
+  const struct ff_field* field; /* f generated on the device from this descriptor, or  */
+  const double* f_host;         /* f copied from host memory [n0][n1][n2]              */
+  double* phi_host;             /* optional: receives phi [n0][n1][n2]                 */
+  int want_norms;               /* field requests: relative L1/L2/Linf vs the exact phi */
+} ff_request;
+
+typedef struct ff_result {
+  int status;        /* ff_status of the request                       */
+  double latency_s;  /* submit -> completion seen by the scheduler     */
+  double queue_s;    /* submit -> dispatch                             */
+  double norms[3];   /* want_norms: relative L1, L2, Linf              */
+  long long points;
+} ff_result;
+
+typedef struct ff_serving_stats {
+  long long completed, rejected, groups;
+  double p50_s, p99_s;      /* latency percentiles of completed requests  */
+  double points_per_s;      /* completed points / (last done - first submit) */
+  size_t arena_bytes, arena_peak_bytes;
+} ff_serving_stats;
+
+/* Starts the scheduler thread on the current device (cfg NULL: 8 GiB arena,
+ * 4 streams, groups of 16, 200 us window). */
+ff_status ff_server_create(const ff_server_config* cfg, ff_server** out);
+/* Device bytes a request occupies while admitted (its admission footprint). */
+size_t ff_server_footprint(const ff_request* r);
+/* Queues a request; FF_ERR_REJECTED if its footprint exceeds the budget.
+ * The field descriptor is copied; f_host / phi_host must stay valid until the
+ * request completes. */
+ff_status ff_server_submit(ff_server* s, const ff_request* r, uint64_t* id);
+/* Waits for a request (once per id) and returns its result. */
+ff_status ff_server_wait(ff_server* s, uint64_t id, double timeout_s, ff_result* out);
+ff_status ff_server_stats(ff_server* s, ff_serving_stats* out);
+/* Drains the queue, then stops the scheduler and frees everything. */
+ff_status ff_server_destroy(ff_server* s);
 
 /* ----------------------------------------------------------------------- */
 /* Driver kernels of FishFry (FishFry.hpp:14-101).                          */

@@ -83,7 +83,7 @@ def fft(x: np.ndarray, sign: int = -1, axis: int = -1) -> np.ndarray:
 
 
 def rfft(x: np.ndarray) -> np.ndarray:
-    """R2C along the last (contiguous) axis, ParisPencil.cpp:140,224."""
+    """R2C along the last (contiguous) axis, ParisPencil.cpp:81,165."""
     a = np.ascontiguousarray(x, dtype=np.float64)
     n = a.shape[-1]
     out = np.empty(a.shape[:-1] + (n // 2 + 1,), dtype=np.complex128)
@@ -93,7 +93,7 @@ def rfft(x: np.ndarray) -> np.ndarray:
 
 
 def irfft(x: np.ndarray, n: int) -> np.ndarray:
-    """C2R (unnormalized) along the last axis, ParisPencil.cpp:139,439."""
+    """C2R (unnormalized) along the last axis, ParisPencil.cpp:80,380."""
     a = np.ascontiguousarray(x, dtype=np.complex128)
     assert a.shape[-1] == n // 2 + 1
     out = np.empty(a.shape[:-1] + (n,), dtype=np.float64)
@@ -119,7 +119,7 @@ def grid(n):
 
 
 def solve(density: np.ndarray, lo=None, hi=None, nthreads: int = 0) -> np.ndarray:
-    """Full periodic Poisson solve, ParisPencil.cpp:154-484 at 1x1x1 tasks."""
+    """Full periodic Poisson solve, ParisPencil.cpp:95-425 at 1x1x1 tasks."""
     d = np.ascontiguousarray(density, dtype=np.float64)
     n = d.shape
     if lo is None or hi is None:
@@ -133,7 +133,7 @@ def solve(density: np.ndarray, lo=None, hi=None, nthreads: int = 0) -> np.ndarra
 
 
 def filter_half(spec: np.ndarray, nz: int, lo=None, hi=None) -> np.ndarray:
-    """Spectral filter of ParisPencil.cpp:322-344 applied to a copy of the
+    """Spectral filter of ParisPencil.cpp:263-285 applied to a copy of the
     half spectrum spec[nx][ny][nz/2+1]."""
     a = np.ascontiguousarray(spec, dtype=np.complex128).copy()
     n = (a.shape[0], a.shape[1], nz)

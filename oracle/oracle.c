@@ -8,8 +8,8 @@
  * 3, 5 specialised, any other prime factor by a direct DFT of that length),
  * with twiddles from ora_twiddle().  The transforms are unnormalized with
  * exponent -1 forward and +1 inverse, the cuFFT/hipFFT convention that
- * ParisPencil.cpp relies on (forward at :224,271,318, inverse at :347,393,439,
- * single 1/N scale at :467).
+ * ParisPencil.cpp relies on (forward at :165,212,259, inverse at :288,334,380,
+ * single 1/N scale at :408).
  */
 #define _GNU_SOURCE
 #include "oracle.h"
@@ -268,7 +268,7 @@ int ora_irfft(int n, const double *in, double *out, long howmany)
 
 double ora_dd(int n, double lo, double hi)
 {
-  /* ParisPencil.cpp:68-70, same operand order */
+  /* ParisPencil.cpp:9-11, same operand order */
   return 2.0 * M_PI * (double)(n - 1) / ((double)n * (hi - lo));
 }
 
@@ -276,7 +276,7 @@ static long lmax(long a, long b) { return a > b ? a : b; }
 
 size_t ora_bytes(const int n[3], const int m[3])
 {
-  /* ParisPencil.cpp:77,83-85,110-134 */
+  /* ParisPencil.cpp:18,24-26,51-75 */
   const int nh = n[2] / 2 + 1;
   const int mi = m[0], mj = m[1], mk = m[2];
   int mq = (int)lround(sqrt((double)mk));
@@ -315,7 +315,7 @@ void ora_filter(const int n[3], const double lo[3], const double hi[3], double *
   const double ddj = ora_dd(nj, lo[1], hi[1]);
   const double ddk = ora_dd(nk, lo[2], hi[2]);
   cplx *w = (cplx *)spec;
-  /* spectral filter  (ParisPencil.cpp:322-344), same expression order */
+  /* spectral filter  (ParisPencil.cpp:263-285), same expression order */
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < ni; i++) {
     for (int j = 0; j < nj; j++) {
@@ -354,25 +354,25 @@ int ora_solve(const int n[3], const double lo[3], const double hi[3],
   cplx *w = (cplx *)work;
   int err = 0;
 
-  /* D2Z along z: [i][j][k] -> [i][j][kh]  (ParisPencil.cpp:224) */
+  /* D2Z along z: [i][j][k] -> [i][j][kh]  (ParisPencil.cpp:165) */
   err |= ora_rfft(nk, density, work, (long)ni * nj);
-  /* Z2Z forward along y: column stride nh  (ParisPencil.cpp:271) */
+  /* Z2Z forward along y: column stride nh  (ParisPencil.cpp:212) */
   for (int i = 0; i < ni && !err; i++) err |= fft_lines(&pj_, w + (long)i * nj * nh, nh, nh, 1, -1);
-  /* Z2Z forward along x: column stride nj*nh  (ParisPencil.cpp:318) */
+  /* Z2Z forward along x: column stride nj*nh  (ParisPencil.cpp:259) */
   if (!err) err |= fft_lines(&pi_, w, (long)nj * nh, (long)nj * nh, 1, -1);
   if (err) { plan_free(&pi_); plan_free(&pj_); return -1; }
 
   ora_filter(n, lo, hi, work);
 
-  /* Z2Z inverse along x, then y  (ParisPencil.cpp:347,393) */
+  /* Z2Z inverse along x, then y  (ParisPencil.cpp:288,334) */
   err |= fft_lines(&pi_, w, (long)nj * nh, (long)nj * nh, 1, +1);
   for (int i = 0; i < ni && !err; i++) err |= fft_lines(&pj_, w + (long)i * nj * nh, nh, nh, 1, +1);
   plan_free(&pi_);
   plan_free(&pj_);
   if (err) return -1;
-  /* Z2D along z  (ParisPencil.cpp:439) */
+  /* Z2D along z  (ParisPencil.cpp:380) */
   if (ora_irfft(nk, work, potential, (long)ni * nj)) return -1;
-  /* normalize  (ParisPencil.cpp:467,478) */
+  /* normalize  (ParisPencil.cpp:408,419) */
   const double divN = 1.0 / ((double)ni * (double)nj * (double)nk);
   const long N = (long)ni * nj * nk;
 #pragma omp parallel for schedule(static)

@@ -31,33 +31,33 @@ extern "C" {
 /* Batched unnormalized 1D complex FFT, interleaved (re,im) doubles.
  * sign = -1 forward (exponent -1), +1 inverse; element (b,r) lives at
  * data[2*(b*dist + r*stride)].  Follows the cuFFT/hipFFT Z2Z convention used at
- * ParisPencil.cpp:137-138 (plans) and :271,318,347,393 (execs). */
+ * ParisPencil.cpp:78-79 (plans) and :212,259,288,334 (execs). */
 int ora_fft(int n, double *data, long howmany, long stride, long dist, int sign);
 
 /* Batched R2C along contiguous rows: in[b*n + k] -> out complex[b*(n/2+1) + kh].
- * ParisPencil.cpp:140 (D2Z plan), :224 (exec). */
+ * ParisPencil.cpp:81 (D2Z plan), :165 (exec). */
 int ora_rfft(int n, const double *in, double *out, long howmany);
 
 /* Batched C2R (unnormalized) along contiguous rows: complex[b*(n/2+1)+kh] ->
  * out[b*n + k].  The imaginary parts of the DC term and (even n) the Nyquist
  * term are ignored, i.e. the input is read as a Hermitian half spectrum.
- * ParisPencil.cpp:139 (Z2D plan), :439 (exec). */
+ * ParisPencil.cpp:80 (Z2D plan), :380 (exec). */
 int ora_irfft(int n, const double *in, double *out, long howmany);
 
-/* Filter constants dd = 2*pi*(n-1)/(n*(hi-lo)), ParisPencil.cpp:68-70. */
+/* Filter constants dd = 2*pi*(n-1)/(n*(hi-lo)), ParisPencil.cpp:9-11. */
 double ora_dd(int n, double lo, double hi);
 
 /* Bytes the reference asks callers to allocate per work array,
- * ParisPencil.cpp:108-134 (general m[3]). */
+ * ParisPencil.cpp:49-75 (general m[3]). */
 size_t ora_bytes(const int n[3], const int m[3]);
 
 /* The spectral filter alone, in place on a half spectrum [ni][nj][nk/2+1]
- * (complex interleaved): ParisPencil.cpp:322-344 at 1x1x1 tasks. */
+ * (complex interleaved): ParisPencil.cpp:263-285 at 1x1x1 tasks. */
 void ora_filter(const int n[3], const double lo[3], const double hi[3], double *spec);
 
-/* Full periodic Poisson solve at 1x1x1 tasks, ParisPencil.cpp:154-484:
- * D2Z(z), Z2Z fwd (y), Z2Z fwd (x), spectral filter (:322-344), Z2Z inv (x),
- * Z2Z inv (y), Z2D (z), scale by 1/(nx*ny*nz) (:466-481).
+/* Full periodic Poisson solve at 1x1x1 tasks, ParisPencil.cpp:95-425:
+ * D2Z(z), Z2Z fwd (y), Z2Z fwd (x), spectral filter (:263-285), Z2Z inv (x),
+ * Z2Z inv (y), Z2D (z), scale by 1/(nx*ny*nz) (:406-421).
  * density [nx][ny][nz] (z fastest) is read, potential receives phi in the
  * same layout.  work must hold 2*nx*ny*(nz/2+1) doubles.  nthreads <= 0 uses
  * the OpenMP default. */

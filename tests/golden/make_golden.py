@@ -1,15 +1,15 @@
 """Write tests/golden/golden.npz: known-answer vectors for the FFT pieces and
-the full periodic Poisson solve of fishfry (ParisPencil.cpp:154-484 at 1x1x1
+the full periodic Poisson solve of fishfry (ParisPencil.cpp:95-425 at 1x1x1
 tasks).
 
 The reference ships no tests or fixtures and its FFT lives in cuFFT/hipFFT
 (not vendored, not linkable here), so the expected outputs come from an
 independent implementation: scipy.fft (pocketfft) evaluated in long double
 (80-bit) on float64 inputs, then rounded to float64.  The solve follows the
-reference's sequence: D2Z along z (ParisPencil.cpp:224), forward C2C along y
-(:271) and x (:318), the spectral filter (:322-344) with the reference's dd
-(:68-70), inverse C2C along x (:347) and y (:393), C2R along z (:439), scale by
-1/(nx*ny*nz) (:467-478).
+reference's sequence: D2Z along z (ParisPencil.cpp:165), forward C2C along y
+(:212) and x (:259), the spectral filter (:263-285) with the reference's dd
+(:9-11), inverse C2C along x (:288) and y (:334), C2R along z (:380), scale by
+1/(nx*ny*nz) (:406-421).
 
 Run from the repo root:  python tests/golden/make_golden.py
 """

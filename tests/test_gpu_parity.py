@@ -236,7 +236,7 @@ def test_stamps_and_contract():
     t = [s[1] for s in stamps]
     assert all(t[i] <= t[i + 1] for i in range(len(t) - 1))
     assert t[-1] > t[0]
-    # bytes check (ParisPencil.cpp:160)
+    # bytes check (ParisPencil.cpp:101)
     with pytest.raises(ff.FishFryError) as e:
         plan.solve(nb - 8, a, b)
     assert e.value.status == ff.FF_ERR_BYTES

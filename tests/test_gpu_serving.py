@@ -1,0 +1,152 @@
+"""GPU tests of the serving loop: results equal alone and in a mix (bitwise),
+host-array requests match the CPU oracle, admission by HBM footprint under a
+small budget (rejection and memory reuse), and the serving workload -- Poisson
+arrivals, log-uniform edges in [32, 512] rounded to the nearest 2^a 3^b 5^c,
+half FP64 and half FP32 + 2 refinement steps, f = the benchmark field plus 16
+random modes -- with every result checked against its exact potential."""
+import math
+import time
+
+import numpy as np
+import pytest
+
+import fishfry_amd as ff
+import oracle
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def smooth_edge(e):
+    """Nearest integer >= 2 whose only prime factors are 2, 3 and 5."""
+    def ok(m):
+        for p in (2, 3, 5):
+            while m % p == 0:
+                m //= p
+        return m == 1
+    for d in range(0, 64):
+        for c in (e - d, e + d):
+            if c >= 2 and ok(c):
+                return c
+    return e
+
+
+def random_field(n, rng, nmodes=16, kmax=12):
+    f = ff.field_default(n)
+    modes, seen = [], set()
+    while len(modes) < nmodes:
+        k = tuple(int(v) for v in rng.integers(-kmax, kmax + 1, size=3))
+        if k == (0, 0, 0) or not (1 <= math.sqrt(sum(v * v for v in k)) <= kmax):
+            continue
+        if k in seen or tuple(-v for v in k) in seen:
+            continue
+        seen.add(k)
+        modes.append((k, float(rng.standard_normal()), float(rng.standard_normal())))
+    return ff.field_set_modes(f, modes)
+
+
+def test_alone_equals_in_mix_bitwise():
+    rng = np.random.default_rng(11)
+    srv = ff.Server(hbm_budget=8 << 30, streams=4, max_batch=8)
+    targets = [((48, 40, 36), ff.FF_FP64, 0), ((40, 32, 50), ff.FF_FP32_REFINE, 2)]
+    fields = [random_field(n, rng) for n, _, _ in targets]
+    alone = []
+    for (n, prec, steps), fld in zip(targets, fields):
+        phi = np.empty(n)
+        rid = srv.submit(n, prec, steps, field=fld, phi_host=phi)
+        srv.wait(rid)
+        alone.append(phi)
+    # the same requests among 30 others of mixed shapes and precisions
+    ids, outs = [], {}
+    for k in range(32):
+        if k in (7, 19):
+            j = 0 if k == 7 else 1
+            n, prec, steps = targets[j]
+            phi = np.empty(n)
+            ids.append(("t%d" % j, srv.submit(n, prec, steps, field=fields[j], phi_host=phi)))
+            outs["t%d" % j] = phi
+        else:
+            n = tuple(smooth_edge(int(v)) for v in rng.integers(16, 96, size=3))
+            prec = ff.FF_FP64 if k % 2 else ff.FF_FP32_REFINE
+            ids.append(("o", srv.submit(n, prec, 2, field=random_field(n, rng))))
+    for tag, rid in ids:
+        srv.wait(rid)
+    st = srv.stats()
+    say("mix stats", st)
+    assert np.array_equal(outs["t0"], alone[0])
+    assert np.array_equal(outs["t1"], alone[1])
+    assert st["completed"] == 34 and st["groups"] >= 2
+    srv.close()
+
+
+def test_host_requests_match_oracle():
+    rng = np.random.default_rng(12)
+    srv = ff.Server(hbm_budget=2 << 30)
+    shapes = [(32, 32, 32), (24, 20, 18), (30, 25, 27), (64, 16, 40)]
+    fs = [rng.standard_normal(n) for n in shapes]
+    phis = [np.empty(n) for n in shapes]
+    ids = [srv.submit(n, f_host=f, phi_host=p) for n, f, p in zip(shapes, fs, phis)]
+    for rid in ids:
+        r = srv.wait(rid)
+        assert r["status"] == 0 and r["latency_s"] >= r["queue_s"] >= 0
+    for f, p in zip(fs, phis):
+        want = oracle.solve(f)
+        assert np.linalg.norm(p - want) / np.linalg.norm(want) < 1e-13
+    srv.close()
+
+
+def test_admission_by_footprint():
+    rng = np.random.default_rng(13)
+    n = (64, 64, 64)
+    probe = ff.Server(hbm_budget=1 << 20)
+    fp = probe.footprint(n)
+    with pytest.raises(ff.FishFryError) as e:
+        probe.submit(n, field=ff.field_default(n))
+    assert e.value.status == ff.FF_ERR_REJECTED
+    assert probe.stats()["rejected"] == 1
+    probe.close()
+    # room for two such requests at a time: twenty still complete, memory reused
+    srv = ff.Server(hbm_budget=2 * fp + (1 << 16), streams=2, max_batch=4)
+    ids = [srv.submit(n, field=random_field(n, rng), want_norms=True) for _ in range(20)]
+    worst = max(srv.wait(rid)["norms"][2] for rid in ids)
+    st = srv.stats()
+    say("admission", fp, st, worst)
+    assert st["completed"] == 20 and st["arena_peak_bytes"] <= st["arena_bytes"]
+    assert worst < 1e-13
+    srv.close()
+
+
+def test_serving_workload():
+    rng = np.random.default_rng(20261202)
+    nreq, rate = 48, 400.0  # requests, arrivals per second
+    srv = ff.Server(hbm_budget=24 << 30, streams=4, max_batch=8, batch_window_us=300.0)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    reqs = []
+    for k in range(nreq):
+        if rng.random() < 0.5:
+            e = smooth_edge(int(round(math.exp(rng.uniform(math.log(32), math.log(512))))))
+            n = (e, e, e)
+        else:
+            n = tuple(smooth_edge(int(round(math.exp(rng.uniform(math.log(32), math.log(512)))))) for _ in range(3))
+        prec = ff.FF_FP64 if k % 2 == 0 else ff.FF_FP32_REFINE
+        reqs.append((n, prec))
+    t0 = time.perf_counter()
+    ids = []
+    for (n, prec), ta in zip(reqs, arrivals):
+        dt = ta - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        ids.append((n, prec, srv.submit(n, prec, 2, field=random_field(n, rng), want_norms=True)))
+    worst = {ff.FF_FP64: 0.0, ff.FF_FP32_REFINE: 0.0}
+    for n, prec, rid in ids:
+        r = srv.wait(rid, timeout=300.0)
+        worst[prec] = max(worst[prec], r["norms"][2])
+    st = srv.stats()
+    say("serving", st, "worst Linf", worst)
+    assert st["completed"] == nreq
+    assert worst[ff.FF_FP64] < 1e-13 and worst[ff.FF_FP32_REFINE] < 1e-13
+    srv.close()

@@ -136,7 +136,7 @@ def test_mean_removal_linearity_shift():
     g = rng.standard_normal(shape)
     pf = oracle.solve(f)
     pg = oracle.solve(g)
-    # the mean mode is zeroed (ParisPencil.cpp:341): adding a constant changes nothing
+    # the mean mode is zeroed (ParisPencil.cpp:275,282): adding a constant changes nothing
     assert rel(oracle.solve(f + 3.25), pf) < 1e-14
     assert abs(pf.mean()) < 1e-15 * np.abs(pf).max()
     # linearity
