@@ -1,0 +1,70 @@
+"""The precision workload at full size: f = the benchmark's analytic field plus
+16 random Fourier modes (1 <= |k| <= 12, +-k deduplicated, N(0,1) amplitudes,
+seeded), solved in FP64 and with FP32 passes + FP64 residual and 0..3
+refinement steps at 1000^3 (radix 2*5) and 1024^3.  Errors are measured
+against the exact potential (the field descriptor's closed form) and against
+the FP64 solve, whose parity with the CPU oracle the smaller tests establish.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import fishfry_amd as ff
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def modes(rng, count=16, kmax=12):
+    out, seen = [], set()
+    while len(out) < count:
+        k = tuple(int(v) for v in rng.integers(-kmax, kmax + 1, size=3))
+        if k == (0, 0, 0) or not (1 <= math.sqrt(sum(v * v for v in k)) <= kmax):
+            continue
+        if k in seen or tuple(-v for v in k) in seen:
+            continue
+        seen.add(k)
+        out.append((k, float(rng.standard_normal()), float(rng.standard_normal())))
+    return out
+
+
+@pytest.mark.parametrize("n", [(1000, 1000, 1000), (1024, 1024, 1024)])
+def test_precision_workload(n):
+    say("precision", n)
+    rng = np.random.default_rng(20261202)
+    field = ff.field_set_modes(ff.field_default(n), modes(rng))
+    lo, hi, _ = ff.benchmark_grid(n)
+    N = n[0] * n[1] * n[2]
+    p64 = ff.ParisPencil(n, lo, hi)
+    nb = p64.bytes()
+    a = torch.empty(nb // 8, dtype=torch.float64, device="cuda")
+    ref = torch.empty(nb // 8, dtype=torch.float64, device="cuda")
+    ff.field_init(field, a)
+    p64.solve(nb, a, ref)
+    l64 = ff.field_norms(field, ref)
+    say("  fp64 norms vs exact", l64)
+    assert l64[2] < 4e-15
+    p64.close()
+    phi = torch.empty(nb // 8, dtype=torch.float64, device="cuda")
+    refn = torch.linalg.vector_norm(ref[:N]).item()
+    errs = {}
+    for steps in (0, 1, 2, 3):
+        plan = ff.ParisPencil(n, lo, hi, precision=ff.FF_FP32_REFINE, refine_steps=steps)
+        ff.field_init(field, a)
+        plan.solve(nb, a, phi)
+        vs64 = torch.linalg.vector_norm(phi[:N] - ref[:N]).item() / refn
+        lin = ff.field_norms(field, phi)[2]
+        errs[steps] = (vs64, lin)
+        say(f"  fp32 + {steps} steps: rel L2 vs FP64 {vs64:.3e}, Linf vs exact {lin:.3e} ({lin / l64[2]:.2f}x FP64)")
+        plan.close()
+    assert errs[0][0] < 1e-6
+    assert errs[1][0] < 1e-10
+    assert errs[2][0] < 1e-14 and errs[2][1] < 2.0 * l64[2]
+    assert errs[3][0] < 1e-14 and errs[3][1] < 2.0 * l64[2]
+    del a, ref, phi
+    torch.cuda.empty_cache()
