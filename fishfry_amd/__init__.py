@@ -39,6 +39,7 @@ FF_FP64 = 0
 FF_FP32_REFINE = 1
 FF_PRESERVE_INPUT = 1
 FF_BLOCKED_LAYOUT = 2
+FF_TUNE_SPLIT_LDS = 1
 FF_SOLVE_STAMPS = 14
 FF_MAX_MODES = 32
 
@@ -65,7 +66,7 @@ class Stamp(ctypes.Structure):
 class Tune(ctypes.Structure):
     """ff_tune: per-pass override of elements per thread (E) and lines per workgroup (T)."""
     _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_int), ("precision", ctypes.c_int), ("E", ctypes.c_int),
-                ("T", ctypes.c_int)]
+                ("T", ctypes.c_int), ("flags", ctypes.c_int)]
 
 
 class BatchItem(ctypes.Structure):
@@ -311,9 +312,10 @@ class ParisPencil:
         self.close()
 
 
-def tune_set(kind: int, n: int, precision: int, E: int, T: int) -> None:
-    """Override E / T for passes of kind (0 z rows, 1 y, 2 x) and length n."""
-    t = Tune(kind, n, precision, E, T)
+def tune_set(kind: int, n: int, precision: int, E: int, T: int, flags: int = 0) -> None:
+    """Override E / T (and FF_TUNE_SPLIT_LDS for column passes) for passes of
+    kind (0 z forward rows, 1 y, 2 x, 3 z inverse rows) and length n."""
+    t = Tune(kind, n, precision, E, T, flags)
     check(lib().ff_tune_set(ctypes.byref(t)), "tune_set")
 
 
@@ -499,5 +501,5 @@ from .driver import FishFry  # noqa: E402  (needs the names above)
 __all__ = [
     "ParisPencil", "FishFry", "Batch", "Server", "FF_BATCH_GRAPH", "FF_BATCH_PERSISTENT", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
     "field_norms", "reference_bytes", "benchmark_grid", "FishFryError", "lib", "LIB_PATH",
-    "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_BLOCKED_LAYOUT",
+    "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_BLOCKED_LAYOUT", "FF_TUNE_SPLIT_LDS",
 ]

@@ -630,9 +630,12 @@ ff_status ff_batch_autotune(ff_batch* b, double* const* density, double* const* 
     ff_plan* p = b->plans[kv.second];
     const int eb = prec ? 8 : 16;
     double best = 1e30;
-    int bE = 0, bT = 0;
+    int bE = 0, bT = 0, bF = 0;
+    const int nflags = (kind == 1 || kind == 2) ? 2 : 1;  // column passes: also the split LDS exchange
     for (int E : kEValues) {
-      for (int T = 1; T <= 64 && st == FF_OK; T *= 2) {
+      for (int cand = 0; cand < 7 * nflags && st == FF_OK; ++cand) {
+        const int T = 1 << (cand % 7);
+        const int fl = cand / 7;
         // candidate configuration on a scratch copy of the plan's pass config
         ff_plan trial = *p;
         bool ok;
@@ -642,7 +645,7 @@ ff_status ff_batch_autotune(ff_batch* b, double* const* density, double* const* 
           if (ok) (kind == 0 ? trial.zr : trial.zi)[prec] = c;
         } else {
           ColCfg c;
-          ok = col_cfg_with(n, p->nh, eb, c, E, T);
+          ok = col_cfg_with(n, p->nh, eb, c, E, T, fl ? FF_TUNE_SPLIT_LDS : 0);
           if (ok) (kind == 1 ? trial.cy : trial.cx)[prec] = c;
         }
         if (!ok) continue;
@@ -665,11 +668,12 @@ ff_status ff_batch_autotune(ff_batch* b, double* const* density, double* const* 
           best = ms;
           bE = E;
           bT = T;
+          bF = fl ? FF_TUNE_SPLIT_LDS : 0;
         }
       }
     }
     if (st == FF_OK && bE) {
-      const ff_tune t{kind, n, prec, bE, bT};
+      const ff_tune t{kind, n, prec, bE, bT, bF};
       st = tune_set(&t);
       count++;
     }

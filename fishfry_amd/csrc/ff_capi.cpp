@@ -190,7 +190,7 @@ ff_status ff_tune_query(int kind, int n, int precision, int ncols, ff_tune* out)
   if (!out || kind < 0 || kind > 3 || n < 1 || (precision != 0 && precision != 1) || ncols < 1)
     return set_error(FF_ERR_INVALID, "ff_tune_query: bad arguments");
   const int eb = precision ? 8 : 16;
-  int E = 0, T = 0;
+  int E = 0, T = 0, flags = 0;
   if (kind == 0 || kind == 3) {
     RowCfg c;
     if (!make_row_cfg(2 * n, eb, c, kind, precision))
@@ -203,8 +203,9 @@ ff_status ff_tune_query(int kind, int n, int precision, int ncols, ff_tune* out)
       return set_error(FF_ERR_UNSUPPORTED, "ff_tune_query: no plan for that length");
     E = c.line.E;
     T = c.T;
+    flags = c.split ? FF_TUNE_SPLIT_LDS : 0;
   }
-  *out = ff_tune{kind, n, precision, E, T};
+  *out = ff_tune{kind, n, precision, E, T, flags};
   return FF_OK;
 }
 

@@ -287,39 +287,80 @@ __device__ __forceinline__ void put_natural(const cx<F> (&v)[E], cx<F>* lds, con
   for (int i = 0; i < E; ++i) lds[idx(r0 + i * L)] = v[i];
 }
 
+// Split exchange: one component (PART 0 real, 1 imaginary) at a time through
+// an LDS array of F, half the LDS of a complex exchange.  After the real
+// gather v[].x is in the new order while v[].y still holds the old order,
+// which the imaginary scatter then reads.
+template <int E, int R, int PART, typename F, typename Idx>
+__device__ __forceinline__ void scatter_part(const cx<F> (&v)[E], F* lds, const Idx& idx, int r0, int L, int ns,
+                                             uint32_t nsinv) {
+  constexpr int Q = E / R;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = r0 + q * L;
+    const int k = ns > 1 ? fast_mod(j, ns, nsinv) : 0;
+    const int base = (j - k) * R + k;
+#pragma unroll
+    for (int m = 0; m < R; ++m) lds[idx(base + m * ns)] = PART ? v[q + m * Q].y : v[q + m * Q].x;
+  }
+}
+
+template <int E, int PART, typename F, typename Idx>
+__device__ __forceinline__ void gather_part(cx<F> (&v)[E], const F* lds, const Idx& idx, int r0, int L) {
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    if constexpr (PART) v[i].y = lds[idx(r0 + i * L)];
+    else v[i].x = lds[idx(r0 + i * L)];
+  }
+}
+
 // One pass with compile-time radix R: butterflies, then (unless last) the LDS
 // exchange to natural order for the next pass.
-template <int E, int R, int DIR, typename F, typename Idx>
+template <int E, int R, int DIR, typename F, bool SPLIT, typename Idx>
 __device__ __forceinline__ void run_pass(cx<F> (&v)[E], const LinePlan& lp, int p, const cx<F>* __restrict__ tw,
                                          cx<F>* lds, const Idx& idx, int r0, bool last) {
   if constexpr (radix_ok<E, R>()) {
     pass_regs<E, R, DIR, F>(v, r0, lp.L, lp.ns[p], lp.nsinv[p], lp.twstep[p], tw);
     if (!last) {
-      __syncthreads();
-      scatter<E, R, F>(v, lds, idx, r0, lp.L, lp.ns[p], lp.nsinv[p]);
-      __syncthreads();
-      gather<E, F>(v, lds, idx, r0, lp.L);
+      if constexpr (!SPLIT) {
+        __syncthreads();
+        scatter<E, R, F>(v, lds, idx, r0, lp.L, lp.ns[p], lp.nsinv[p]);
+        __syncthreads();
+        gather<E, F>(v, lds, idx, r0, lp.L);
+      } else {
+        F* l = reinterpret_cast<F*>(lds);
+        __syncthreads();
+        scatter_part<E, R, 0, F>(v, l, idx, r0, lp.L, lp.ns[p], lp.nsinv[p]);
+        __syncthreads();
+        gather_part<E, 0, F>(v, l, idx, r0, lp.L);
+        __syncthreads();
+        scatter_part<E, R, 1, F>(v, l, idx, r0, lp.L, lp.ns[p], lp.nsinv[p]);
+        __syncthreads();
+        gather_part<E, 1, F>(v, l, idx, r0, lp.L);
+      }
     }
   }
 }
 
 // Full line transform.  All threads of the workgroup must call it (it contains
 // workgroup barriers); threads of padding lines simply transform garbage.
-template <int E, int DIR, typename F, typename Idx>
+// SPLIT: exchange real and imaginary parts separately (LDS holds one
+// component per slot).
+template <int E, int DIR, typename F, typename Idx, bool SPLIT = false>
 __device__ __forceinline__ void fft_line(cx<F> (&v)[E], const LinePlan& lp, const cx<F>* __restrict__ tw,
                                          cx<F>* lds, const Idx& idx, int r0) {
   for (int p = 0; p < lp.np; ++p) {
     const bool last = (p + 1 == lp.np);
     switch (lp.radix[p]) {
-      case 2: run_pass<E, 2, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
-      case 3: run_pass<E, 3, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
-      case 4: run_pass<E, 4, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
-      case 5: run_pass<E, 5, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
-      case 6: run_pass<E, 6, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
-      case 8: run_pass<E, 8, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
-      case 10: run_pass<E, 10, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
-      case 12: run_pass<E, 12, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
-      case 16: run_pass<E, 16, DIR, F>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 2: run_pass<E, 2, DIR, F, SPLIT>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 3: run_pass<E, 3, DIR, F, SPLIT>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 4: run_pass<E, 4, DIR, F, SPLIT>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 5: run_pass<E, 5, DIR, F, SPLIT>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 6: run_pass<E, 6, DIR, F, SPLIT>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 8: run_pass<E, 8, DIR, F, SPLIT>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 10: run_pass<E, 10, DIR, F, SPLIT>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 12: run_pass<E, 12, DIR, F, SPLIT>(v, lp, p, tw, lds, idx, r0, last); break;
+      case 16: run_pass<E, 16, DIR, F, SPLIT>(v, lp, p, tw, lds, idx, r0, last); break;
       default: break;
     }
   }

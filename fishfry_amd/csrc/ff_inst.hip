@@ -18,26 +18,29 @@ static hipError_t launch(const void* fn, const Launch& l, const void* args, size
   return hipLaunchKernel(fn, l.grid, l.block, argv, l.lds, l.stream);
 }
 
-template <>
-hipError_t PassLaunchers<FF_E>::col(int f32, int mode, const Launch& l, const ColArgs& a) {
-  constexpr int E = FF_E;
-  const void* fn = nullptr;
+template <int E, bool S>
+static const void* col_fn(int f32, int mode) {
   if (f32) {
     switch (mode) {
-      case COL_FWD: fn = (const void*)&k_col<float, E, COL_FWD>; break;
-      case COL_INV: fn = (const void*)&k_col<float, E, COL_INV>; break;
-      case COL_FILTER: fn = (const void*)&k_col<float, E, COL_FILTER>; break;
-      default: return hipErrorInvalidValue;
-    }
-  } else {
-    switch (mode) {
-      case COL_FWD: fn = (const void*)&k_col<double, E, COL_FWD>; break;
-      case COL_INV: fn = (const void*)&k_col<double, E, COL_INV>; break;
-      case COL_FILTER: fn = (const void*)&k_col<double, E, COL_FILTER>; break;
-      case COL_SYMBOL: fn = (const void*)&k_col<double, E, COL_SYMBOL>; break;
-      default: return hipErrorInvalidValue;
+      case COL_FWD: return (const void*)&k_col<float, E, COL_FWD, S>;
+      case COL_INV: return (const void*)&k_col<float, E, COL_INV, S>;
+      case COL_FILTER: return (const void*)&k_col<float, E, COL_FILTER, S>;
+      default: return nullptr;
     }
   }
+  switch (mode) {
+    case COL_FWD: return (const void*)&k_col<double, E, COL_FWD, S>;
+    case COL_INV: return (const void*)&k_col<double, E, COL_INV, S>;
+    case COL_FILTER: return (const void*)&k_col<double, E, COL_FILTER, S>;
+    case COL_SYMBOL: return (const void*)&k_col<double, E, COL_SYMBOL, S>;
+    default: return nullptr;
+  }
+}
+
+template <>
+hipError_t PassLaunchers<FF_E>::col(int f32, int mode, int split, const Launch& l, const ColArgs& a) {
+  const void* fn = split ? col_fn<FF_E, true>(f32, mode) : col_fn<FF_E, false>(f32, mode);
+  if (!fn) return hipErrorInvalidValue;
   return launch(fn, l, &a, sizeof(a));
 }
 

@@ -45,6 +45,7 @@ void twiddle_ld(long long t, long long n, long double& c, long double& s);
 struct ColCfg {
   LineCfg line;
   int T = 1, lgT = 0, ps = 30;
+  int split = 0;  // FF_TUNE_SPLIT_LDS: one component per LDS slot
   size_t lds = 0;
   int threads = 0;
 };
@@ -62,13 +63,13 @@ struct RowCfg {
 // overrides (ff_tune_set) take precedence over the built-in heuristic.
 bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int kind, int prec);
 bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int kind, int prec);
-bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T);
+bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, int flags = 0);
 bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T);
-bool tune_lookup(int kind, int n, int prec, int& E, int& T);
+bool tune_lookup(int kind, int n, int prec, int& E, int& T, int& flags);
 ff_status tune_set(const ff_tune* t);
 void tune_reset();
 
-hipError_t launch_col(int E, int f32, int mode, const Launch& l, const ColArgs& a);
+hipError_t launch_col(int E, int f32, int mode, int split, const Launch& l, const ColArgs& a);
 hipError_t launch_r2c(int E, int f32, int in_f32, int odd, const Launch& l, const RowArgs& a);
 hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, const Launch& l, const RowArgs& a);
 

@@ -55,6 +55,7 @@ struct ColArgs {
   int nx, ny;         // extents for the spectral symbol (x pass)
   double ddi, ddj, ddk;
   double* dc_out;  // FILTER / SYMBOL: receives the forward (0,0,0) term (sum of the input)
+  long long rstep; // sr(p + L) - sr(p) for every p when that is constant, else 0
   const void* tw;
   LinePlan lp;
 };
@@ -63,7 +64,8 @@ struct ColArgs {
 //       3 forward * symbol * inverse (applies L_h for a residual).
 enum { COL_FWD = 0, COL_INV = 1, COL_FILTER = 2, COL_SYMBOL = 3 };
 
-template <typename F, int E, int MODE>
+// SPLIT: exchange real and imaginary parts separately through half the LDS.
+template <typename F, int E, int MODE, bool SPLIT>
 __global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
@@ -76,7 +78,9 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
   const int tile = b - outer * a.tiles;
   const int col = tile * a.T + c;
   const bool valid = col < a.ncols;
-  const long long base = a.so(outer) + col;
+  // Threads past the last column load the last column (in bounds, never
+  // stored), so the loads need no predicate.
+  const long long base = a.so(outer) + (valid ? col : a.ncols - 1);
   // src may equal dst (in-place pass): every thread stores exactly the
   // elements it loaded, after all of its loads.
   const cx<F>* src = reinterpret_cast<const cx<F>*>(a.src) + base;
@@ -84,18 +88,21 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
   const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
 
   cx<F> v[E];
+  if (a.rstep) {
+    const cx<F>* s = src + a.sr(r0);
 #pragma unroll
-  for (int i = 0; i < E; ++i) {
-    if (valid) v[i] = src[a.sr(r0 + i * L)];
-    else v[i] = cx<F>{F(0), F(0)};
+    for (int i = 0; i < E; ++i) v[i] = s[(long long)i * a.rstep];
+  } else {
+#pragma unroll
+    for (int i = 0; i < E; ++i) v[i] = src[a.sr(r0 + i * L)];
   }
   const ColIdx idx{a.T, c, a.ps};
   if constexpr (MODE == COL_FWD) {
-    fft_line<E, -1, F>(v, a.lp, tw, lds, idx, r0);
+    fft_line<E, -1, F, ColIdx, SPLIT>(v, a.lp, tw, lds, idx, r0);
   } else if constexpr (MODE == COL_INV) {
-    fft_line<E, +1, F>(v, a.lp, tw, lds, idx, r0);
+    fft_line<E, +1, F, ColIdx, SPLIT>(v, a.lp, tw, lds, idx, r0);
   } else {
-    fft_line<E, -1, F>(v, a.lp, tw, lds, idx, r0);
+    fft_line<E, -1, F, ColIdx, SPLIT>(v, a.lp, tw, lds, idx, r0);
     // spectral filter of ParisPencil.cpp:269-284 at (i, j, kh) = (r, outer, col)
     const int jj = outer, kk = col;
     const double j2 = sqr(double(jj < a.ny - jj ? jj : a.ny - jj) * a.ddj);
@@ -112,11 +119,17 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
       const F df = F(d);
       v[i] = cx<F>{df * v[i].x, df * v[i].y};
     }
-    fft_line<E, +1, F>(v, a.lp, tw, lds, idx, r0);
+    fft_line<E, +1, F, ColIdx, SPLIT>(v, a.lp, tw, lds, idx, r0);
   }
   if (valid) {
+    if (a.rstep) {
+      cx<F>* d = dst + a.sr(r0);
 #pragma unroll
-    for (int i = 0; i < E; ++i) dst[a.sr(r0 + i * L)] = v[i];
+      for (int i = 0; i < E; ++i) d[(long long)i * a.rstep] = v[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < E; ++i) dst[a.sr(r0 + i * L)] = v[i];
+    }
   }
 }
 
@@ -163,25 +176,22 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
   const int r0 = tid - line * L;
   const long long row = (long long)xcd_remap(blockIdx.x, a.nblocks) * a.T + line;
   const bool valid = row < a.nrows;
+  const long long rowc = valid ? row : a.nrows - 1;  // loads stay in bounds; stores are predicated
   const int M = a.lp.n;
   const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
   const RowIdx idx{line * a.pitch, a.ps};
   cx<F> v[E];
   if constexpr (!ODD) {
-    const IN* __restrict__ src = reinterpret_cast<const IN*>(a.src) + row * a.nz;
+    const IN* __restrict__ src = reinterpret_cast<const IN*>(a.src) + rowc * a.nz;
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      if (valid) {
-        const int k = r0 + i * L;
-        if constexpr (sizeof(IN) == 8) {
-          const cx<double> p = reinterpret_cast<const cx<double>*>(src)[k];
-          v[i] = cx<F>{F(p.x), F(p.y)};
-        } else {
-          const cx<float> p = reinterpret_cast<const cx<float>*>(src)[k];
-          v[i] = cx<F>{F(p.x), F(p.y)};
-        }
+      const int k = r0 + i * L;
+      if constexpr (sizeof(IN) == 8) {
+        const cx<double> p = reinterpret_cast<const cx<double>*>(src)[k];
+        v[i] = cx<F>{F(p.x), F(p.y)};
       } else {
-        v[i] = cx<F>{F(0), F(0)};
+        const cx<float> p = reinterpret_cast<const cx<float>*>(src)[k];
+        v[i] = cx<F>{F(p.x), F(p.y)};
       }
     }
     fft_line<E, -1, F>(v, a.lp, tw, lds, idx, r0);
@@ -207,9 +217,9 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
       }
     }
   } else {
-    const IN* __restrict__ src = reinterpret_cast<const IN*>(a.src) + row * a.nz;
+    const IN* __restrict__ src = reinterpret_cast<const IN*>(a.src) + rowc * a.nz;
 #pragma unroll
-    for (int i = 0; i < E; ++i) v[i] = cx<F>{valid ? F(src[r0 + i * L]) : F(0), F(0)};
+    for (int i = 0; i < E; ++i) v[i] = cx<F>{F(src[r0 + i * L]), F(0)};
     fft_line<E, -1, F>(v, a.lp, tw, lds, idx, r0);
     cx<F>* __restrict__ dst = reinterpret_cast<cx<F>*>(a.dst) + (valid ? spec_row(a, row) : 0);
     if (valid) {
@@ -270,15 +280,15 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
   const int M = a.lp.n;
   const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
   const RowIdx idx{line * a.pitch, a.ps};
-  const cx<F>* __restrict__ src = reinterpret_cast<const cx<F>*>(a.src) + (valid ? spec_row(a, row) : 0);
-  const int nload = ODD ? a.nh : M + 1;
+  // rows past the end load the last row (in bounds); their stores are predicated
+  const cx<F>* __restrict__ src = reinterpret_cast<const cx<F>*>(a.src) + spec_row(a, valid ? row : a.nrows - 1);
   // stage the half spectrum in LDS
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const int k = r0 + i * L;
-    if (k < nload) lds[idx(k)] = valid ? src[k] : cx<F>{F(0), F(0)};
+    if (!ODD || k < a.nh) lds[idx(k)] = src[k];
   }
-  if (!ODD && r0 == 0) lds[idx(M)] = valid ? src[M] : cx<F>{F(0), F(0)};
+  if (!ODD && r0 == 0) lds[idx(M)] = src[M];
   __syncthreads();
   cx<F> v[E];
   if constexpr (!ODD) {
@@ -334,7 +344,7 @@ struct Launch {
 
 template <int E>
 struct PassLaunchers {
-  static hipError_t col(int f32, int mode, const Launch& l, const ColArgs& a);
+  static hipError_t col(int f32, int mode, int split, const Launch& l, const ColArgs& a);
   static hipError_t r2c(int f32, int in_f32, int odd, const Launch& l, const RowArgs& a);
   static hipError_t c2r(int f32, int out_kind, int odd, int epi, const Launch& l, const RowArgs& a);
 };

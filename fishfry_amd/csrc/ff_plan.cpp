@@ -190,21 +190,21 @@ static std::mutex& tune_mutex() {
   static std::mutex m;
   return m;
 }
-static std::map<std::tuple<int, int, int>, std::pair<int, int>>& tune_table() {
-  static std::map<std::tuple<int, int, int>, std::pair<int, int>> t;
+// (kind, n, precision) -> (E, T, flags)
+static std::map<std::tuple<int, int, int>, std::tuple<int, int, int>>& tune_table() {
+  static std::map<std::tuple<int, int, int>, std::tuple<int, int, int>> t;
   return t;
 }
 
-bool tune_lookup(int kind, int n, int prec, int& E, int& T) {
+bool tune_lookup(int kind, int n, int prec, int& E, int& T, int& flags) {
   std::lock_guard<std::mutex> g(tune_mutex());
   auto it = tune_table().find({kind, n, prec});
   if (it == tune_table().end()) return false;
-  E = it->second.first;
-  T = it->second.second;
+  std::tie(E, T, flags) = it->second;
   return true;
 }
 
-bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T) {
+bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, int flags) {
   if (!plan_line(n, out.line, E)) return false;
   const int L = out.line.lp.L;
   if (T <= 0) {
@@ -220,8 +220,9 @@ bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T) {
   while ((1 << out.lgT) < T) out.lgT++;
   out.ps = 4;
   out.threads = T * L;
+  out.split = (flags & FF_TUNE_SPLIT_LDS) ? 1 : 0;
   const size_t elems = (size_t)n * T + ((size_t)(n - 1) >> out.ps) + 1;
-  out.lds = elems * (size_t)elem_bytes;
+  out.lds = elems * (size_t)(out.split ? elem_bytes / 2 : elem_bytes);
   return out.lds <= 160 * 1024;
 }
 
@@ -241,8 +242,8 @@ bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T) {
 }
 
 bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int kind, int prec) {
-  int E = 0, T = 0;
-  if (tune_lookup(kind, n, prec, E, T) && col_cfg_with(n, ncols, elem_bytes, out, E, T)) return true;
+  int E = 0, T = 0, flags = 0;
+  if (tune_lookup(kind, n, prec, E, T, flags) && col_cfg_with(n, ncols, elem_bytes, out, E, T, flags)) return true;
   return col_cfg_with(n, ncols, elem_bytes, out, 0, 0);
 }
 
@@ -259,8 +260,8 @@ static bool builtin_default(int kind, int n, int prec, int& E, int& T) {
 
 bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int kind, int prec) {
   const int M = (nz % 2 == 0) ? nz / 2 : nz;
-  int E = 0, T = 0;
-  if (tune_lookup(kind, M, prec, E, T) && row_cfg_with(nz, elem_bytes, out, E, T)) return true;
+  int E = 0, T = 0, flags = 0;
+  if (tune_lookup(kind, M, prec, E, T, flags) && row_cfg_with(nz, elem_bytes, out, E, T)) return true;
   if (builtin_default(kind, M, prec, E, T) && row_cfg_with(nz, elem_bytes, out, E, T)) return true;
   return row_cfg_with(nz, elem_bytes, out, 0, 0);
 }
@@ -275,17 +276,18 @@ ff_status tune_set(const ff_tune* t) {
     return FF_OK;
   }
   const int eb = t->precision ? 8 : 16;
+  if (t->flags & ~FF_TUNE_SPLIT_LDS) return set_error(FF_ERR_INVALID, "ff_tune_set: unknown flags");
   bool ok;
   if (t->kind == 0 || t->kind == 3) {
     RowCfg c;
-    ok = row_cfg_with(2 * t->n, eb, c, t->E, t->T);
+    ok = t->flags == 0 && row_cfg_with(2 * t->n, eb, c, t->E, t->T);
   } else {
     ColCfg c;
-    ok = col_cfg_with(t->n, 1 << 20, eb, c, t->E, t->T);
+    ok = col_cfg_with(t->n, 1 << 20, eb, c, t->E, t->T, t->flags);
   }
-  if (!ok) return set_error(FF_ERR_INVALID, "ff_tune_set: no valid configuration for that E/T");
+  if (!ok) return set_error(FF_ERR_INVALID, "ff_tune_set: no valid configuration for that E/T/flags");
   std::lock_guard<std::mutex> g(tune_mutex());
-  tune_table()[{t->kind, t->n, t->precision}] = {t->E, t->T};
+  tune_table()[{t->kind, t->n, t->precision}] = std::make_tuple(t->E, t->T, t->flags);
   return FF_OK;
 }
 
@@ -298,7 +300,7 @@ void tune_reset() {
 
 #define FF_DECL(E)                                                                                         \
   template <>                                                                                              \
-  hipError_t PassLaunchers<E>::col(int, int, const Launch&, const ColArgs&);                               \
+  hipError_t PassLaunchers<E>::col(int, int, int, const Launch&, const ColArgs&);                               \
   template <>                                                                                              \
   hipError_t PassLaunchers<E>::r2c(int, int, int, const Launch&, const RowArgs&);                          \
   template <>                                                                                              \
@@ -338,8 +340,8 @@ FF_DECL(30)
     default: return hipErrorInvalidValue;    \
   }
 
-hipError_t launch_col(int E, int f32, int mode, const Launch& l, const ColArgs& a) {
-  FF_SWITCH_E(E, col(f32, mode, l, a))
+hipError_t launch_col(int E, int f32, int mode, int split, const Launch& l, const ColArgs& a) {
+  FF_SWITCH_E(E, col(f32, mode, split, l, a))
 }
 hipError_t launch_r2c(int E, int f32, int in_f32, int odd, const Launch& l, const RowArgs& a) {
   FF_SWITCH_E(E, r2c(f32, in_f32, odd, l, a))

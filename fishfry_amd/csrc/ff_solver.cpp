@@ -271,10 +271,16 @@ ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* sr
   a.ddj = p->ddj;
   a.ddk = p->ddk;
   a.dc_out = dc_out;
+  // positions r0 + i*L (i < E) are a constant stride apart when the two-level
+  // stride is one level (lg >= 30) or L is a multiple of the block 2^lg
+  const int L = c.line.lp.L;
+  if (a.sr.lg >= 30) a.rstep = (long long)L * a.sr.lo;
+  else if (L % (1 << a.sr.lg) == 0) a.rstep = (long long)(L >> a.sr.lg) * a.sr.hi;
+  else a.rstep = 0;
   a.tw = axis == 1 ? p->tw_y[prec] : p->tw_x[prec];
   a.lp = c.line.lp;
   const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
-  return hip_check(launch_col(c.line.E, prec, mode, l, a), "column pass launch");
+  return hip_check(launch_col(c.line.E, prec, mode, c.split, l, a), "column pass launch");
 }
 
 static ff_status solve_fp32_refine(const ff_plan* p, const double* f, double* phi, hipStream_t s) {

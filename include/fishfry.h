@@ -128,7 +128,12 @@ typedef struct ff_tune {
   int precision;
   int E;
   int T;
+  int flags; /* FF_TUNE_SPLIT_LDS (column passes) */
 } ff_tune;
+
+/* ff_tune.flags: column passes exchange real and imaginary parts separately
+ * through half the LDS (more workgroups per CU, twice the barriers). */
+#define FF_TUNE_SPLIT_LDS 1
 
 ff_status ff_tune_set(const ff_tune* t);
 void ff_tune_reset(void);

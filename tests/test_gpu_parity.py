@@ -140,6 +140,41 @@ def test_layouts_agree_bitwise():
     assert rel(outs[0], oracle.solve(f)) < TOL
 
 
+@pytest.mark.parametrize("n,E", [((64, 48, 40), 0), ((30, 25, 27), 0), ((96, 1000, 64), 10), ((1024, 16, 8), 16),
+                                 ((768, 12, 6), 12)])
+def test_split_exchange_bitwise(n, E):
+    """The column passes' split real/imaginary LDS exchange moves the same
+    values as the complex exchange.  The two are separate kernels, so the
+    compiler may contract multiply-adds differently: results agree to
+    round-off, and both match the oracle."""
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal(n)
+    try:
+        for prec, steps in ((ff.FF_FP64, 0), (ff.FF_FP32_REFINE, 1)):
+            outs = []
+            pi = 0 if prec == ff.FF_FP64 else 1
+            for split in (0, ff.FF_TUNE_SPLIT_LDS):
+                ff.tune_reset()
+                for kind, length in ((1, n[1]), (2, n[0])):
+                    for p in sorted({0, pi}):  # FP32 refinement also runs FP64 residual passes
+                        cur = ff.tune_query(kind, length, p, n[2] // 2 + 1)
+                        try:
+                            ff.tune_set(kind, length, p, E or cur.E, 0 if E else cur.T, split)
+                        except ff.FishFryError:  # E does not divide this length
+                            ff.tune_set(kind, length, p, cur.E, cur.T, split)
+                say("split", n, E, prec, split)
+                outs.append(gpu_solve(f, precision=prec, refine_steps=steps))
+            d = rel(outs[1], outs[0])
+            say("  split vs complex exchange: rel L2", d)
+            # FP32 passes + 1 refinement are accurate to ~1e-11; FP64 to round-off
+            assert d < (1e-14 if prec == ff.FF_FP64 else 1e-10)
+            want = oracle.solve(f)
+            tol = TOL if prec == ff.FF_FP64 else 1e-9  # FP32 + 1 step: ~1e-10 at 768-long lines
+            assert rel(outs[0], want) < tol and rel(outs[1], want) < tol
+    finally:
+        ff.tune_reset()
+
+
 SOLVE_SHAPES = [(8, 8, 8), (12, 10, 9), (16, 24, 20), (32, 32, 32), (64, 64, 64), (6, 9, 15), (5, 6, 4),
                 (9, 8, 12), (48, 40, 36), (128, 64, 96), (2, 2, 2), (2, 3, 4), (2, 8, 16), (16, 2, 9),
                 (30, 25, 27), (100, 96, 80), (64, 64, 2), (4, 4, 2), (3, 3, 3), (250, 2, 6), (96, 160, 48),
