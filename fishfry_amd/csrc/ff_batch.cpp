@@ -631,7 +631,7 @@ ff_status ff_batch_autotune(ff_batch* b, double* const* density, double* const* 
     const int eb = prec ? 8 : 16;
     double best = 1e30;
     int bE = 0, bT = 0, bF = 0;
-    const int nflags = (kind == 1 || kind == 2) ? 2 : 1;  // column passes: also the split LDS exchange
+    const int nflags = (kind == 1 || kind == 2) ? 4 : 1;  // column passes: split exchange, two groups
     for (int E : kEValues) {
       for (int cand = 0; cand < 7 * nflags && st == FF_OK; ++cand) {
         const int T = 1 << (cand % 7);
@@ -645,7 +645,7 @@ ff_status ff_batch_autotune(ff_batch* b, double* const* density, double* const* 
           if (ok) (kind == 0 ? trial.zr : trial.zi)[prec] = c;
         } else {
           ColCfg c;
-          ok = col_cfg_with(n, p->nh, eb, c, E, T, fl ? FF_TUNE_SPLIT_LDS : 0);
+          ok = col_cfg_with(n, p->nh, eb, c, E, T, fl);
           if (ok) (kind == 1 ? trial.cy : trial.cx)[prec] = c;
         }
         if (!ok) continue;
@@ -668,7 +668,7 @@ ff_status ff_batch_autotune(ff_batch* b, double* const* density, double* const* 
           best = ms;
           bE = E;
           bT = T;
-          bF = fl ? FF_TUNE_SPLIT_LDS : 0;
+          bF = fl;
         }
       }
     }

@@ -137,7 +137,7 @@ ff_status ff_plan_describe(const ff_plan* p, char* buf, size_t len) {
   snprintf(tmp, sizeof tmp, "\"n\":[%d,%d,%d],\"nh\":%d,\"bytes\":%zu,\"precision\":%d,\"refine_steps\":%d,", p->n[0],
            p->n[1], p->n[2], p->nh, p->bytes, p->opt.precision, p->opt.refine_steps);
   s += tmp;
-  snprintf(tmp, sizeof tmp, "\"dd\":[%.17g,%.17g,%.17g],\"block\":%d,", p->ddi, p->ddj, p->ddk, 1 << p->lgB);
+  snprintf(tmp, sizeof tmp, "\"dd\":[%.17g,%.17g,%.17g],\"kt\":%d,", p->ddi, p->ddj, p->ddk, p->lgKT ? 1 << p->lgKT : 0);
   s += tmp;
   auto line = [&](const char* key, const LineCfg& c, int T, size_t lds, int threads) {
     std::string r = "\"";
@@ -168,12 +168,14 @@ ff_status ff_solve_pass(const ff_plan* p, int pass, const double* src, double* d
   if (!p || !src || !dst) return set_error(FF_ERR_INVALID, "ff_solve_pass: null argument");
   if (p->opt.precision != FF_FP64) return set_error(FF_ERR_INVALID, "ff_solve_pass: FP64 plans only");
   hipStream_t s = (hipStream_t)stream;
+  int li = LAY_PLAIN, lo = LAY_PLAIN;
+  if (pass >= 0 && pass <= 4) pass_layouts(p, pass, li, lo);
   switch (pass) {
     case 0: return run_zfwd(p, 0, 0, src, dst, s);
-    case 1: return run_col(p, 0, 1, COL_FWD, src, dst, s);
-    case 2: return run_col(p, 0, 0, COL_FILTER, src, dst, s);
-    case 3: return run_col(p, 0, 1, COL_INV, src, dst, s);
-    case 4: return run_zinv(p, 0, 0, EPI_STORE, src, dst, nullptr, nullptr, s);
+    case 1: return run_col(p, 0, 1, COL_FWD, src, dst, s, nullptr, li, lo);
+    case 2: return run_col(p, 0, 0, COL_FILTER, src, dst, s, nullptr, li, lo);
+    case 3: return run_col(p, 0, 1, COL_INV, src, dst, s, nullptr, li, lo);
+    case 4: return run_zinv(p, 0, 0, EPI_STORE, src, dst, nullptr, nullptr, s, li);
     default: return set_error(FF_ERR_INVALID, "ff_solve_pass: pass must be 0..4");
   }
 }
@@ -203,7 +205,7 @@ ff_status ff_tune_query(int kind, int n, int precision, int ncols, ff_tune* out)
       return set_error(FF_ERR_UNSUPPORTED, "ff_tune_query: no plan for that length");
     E = c.line.E;
     T = c.T;
-    flags = c.split ? FF_TUNE_SPLIT_LDS : 0;
+    flags = (c.split ? FF_TUNE_SPLIT_LDS : 0) | (c.groups == 2 ? FF_TUNE_TWO_GROUPS : 0);
   }
   *out = ff_tune{kind, n, precision, E, T, flags};
   return FF_OK;

@@ -18,28 +18,37 @@ static hipError_t launch(const void* fn, const Launch& l, const void* args, size
   return hipLaunchKernel(fn, l.grid, l.block, argv, l.lds, l.stream);
 }
 
-template <int E, bool S>
+template <int E, bool S, int G>
 static const void* col_fn(int f32, int mode) {
   if (f32) {
     switch (mode) {
-      case COL_FWD: return (const void*)&k_col<float, E, COL_FWD, S>;
-      case COL_INV: return (const void*)&k_col<float, E, COL_INV, S>;
-      case COL_FILTER: return (const void*)&k_col<float, E, COL_FILTER, S>;
+      case COL_FWD: return (const void*)&k_col<float, E, COL_FWD, S, G>;
+      case COL_INV: return (const void*)&k_col<float, E, COL_INV, S, G>;
+      case COL_FILTER: return (const void*)&k_col<float, E, COL_FILTER, S, G>;
       default: return nullptr;
     }
   }
   switch (mode) {
-    case COL_FWD: return (const void*)&k_col<double, E, COL_FWD, S>;
-    case COL_INV: return (const void*)&k_col<double, E, COL_INV, S>;
-    case COL_FILTER: return (const void*)&k_col<double, E, COL_FILTER, S>;
-    case COL_SYMBOL: return (const void*)&k_col<double, E, COL_SYMBOL, S>;
+    case COL_FWD: return (const void*)&k_col<double, E, COL_FWD, S, G>;
+    case COL_INV: return (const void*)&k_col<double, E, COL_INV, S, G>;
+    case COL_FILTER: return (const void*)&k_col<double, E, COL_FILTER, S, G>;
+    case COL_SYMBOL: return (const void*)&k_col<double, E, COL_SYMBOL, S, G>;
     default: return nullptr;
   }
 }
 
+// Two column groups per workgroup are compiled for the elements-per-thread
+// values that long column transforms use.
+constexpr bool kTwoGroups = FF_E == 8 || FF_E == 16;
+
 template <>
-hipError_t PassLaunchers<FF_E>::col(int f32, int mode, int split, const Launch& l, const ColArgs& a) {
-  const void* fn = split ? col_fn<FF_E, true>(f32, mode) : col_fn<FF_E, false>(f32, mode);
+hipError_t PassLaunchers<FF_E>::col(int f32, int mode, int split, int groups, const Launch& l, const ColArgs& a) {
+  const void* fn = nullptr;
+  if (groups == 1) {
+    fn = split ? col_fn<FF_E, true, 1>(f32, mode) : col_fn<FF_E, false, 1>(f32, mode);
+  } else if (groups == 2) {
+    if constexpr (kTwoGroups) fn = split ? col_fn<FF_E, true, 2>(f32, mode) : col_fn<FF_E, false, 2>(f32, mode);
+  }
   if (!fn) return hipErrorInvalidValue;
   return launch(fn, l, &a, sizeof(a));
 }

@@ -45,7 +45,8 @@ void twiddle_ld(long long t, long long n, long double& c, long double& s);
 struct ColCfg {
   LineCfg line;
   int T = 1, lgT = 0, ps = 30;
-  int split = 0;  // FF_TUNE_SPLIT_LDS: one component per LDS slot
+  int split = 0;   // FF_TUNE_SPLIT_LDS: one component per LDS slot
+  int groups = 1;  // FF_TUNE_TWO_GROUPS: column groups of T per workgroup
   size_t lds = 0;
   int threads = 0;
 };
@@ -69,7 +70,7 @@ bool tune_lookup(int kind, int n, int prec, int& E, int& T, int& flags);
 ff_status tune_set(const ff_tune* t);
 void tune_reset();
 
-hipError_t launch_col(int E, int f32, int mode, int split, const Launch& l, const ColArgs& a);
+hipError_t launch_col(int E, int f32, int mode, int split, int groups, const Launch& l, const ColArgs& a);
 hipError_t launch_r2c(int E, int f32, int in_f32, int odd, const Launch& l, const RowArgs& a);
 hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, const Launch& l, const RowArgs& a);
 

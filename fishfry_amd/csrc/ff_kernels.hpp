@@ -26,36 +26,51 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 
 __device__ __forceinline__ double sqr(double x) { return x * x; }
 
-// Spectrum layout: element (i, j, kh) of the half spectrum lives at complex
-// offset ((i >> lgB)*ny + j)*(B*nh) + (i & (B-1))*nh + kh, i.e. x is split
-// into blocks of B planes with y between the blocks.  B = 1 is the
-// reference's [i][j][kh]; B > 1 keeps both column passes within a few dozen
-// pages per tile instead of one page per row for x.
-// Two-level strides address it: position p along an axis maps to
-// (p >> lg)*hi + (p & ((1 << lg) - 1))*lo.
-struct Stride2 {
-  long long hi, lo;
-  int lg;
-  __device__ __forceinline__ long long operator()(int p) const {
-    return (long long)(p >> lg) * hi + (long long)(p & ((1 << lg) - 1)) * lo;
-  }
+// Half-spectrum layouts.  Element (i, j, kh) lives at complex offset
+//   plain      [i][j][kh]:      (i*ny + j)*nh + kh     (the reference's, after its D2Z)
+//   y-blocked  [kb][i][j][kt]:  kb*KS + (i*ny + j)*W + kt
+//   x-blocked  [kb][j][i][kt]:  kb*KS + (j*nx + i)*W + kt
+// with kh = kb*KT + kt, KS = nx*ny*KT and W = KT except for the last block,
+// whose width is nh - (nb-1)*KT (so every layout holds exactly nx*ny*nh).
+// The blocked layouts let a column pass write each tile of KT columns as one
+// contiguous region; the strided side of a pass is then always a read.
+//
+// A SpecSide addresses one layout as seen by a column pass with planes
+// `outer` and transform positions r:
+//   offset(outer, r, col) = kb*KS + (outer*fo + r*fr)*W + kt,
+//   kb = col >> lgKT, kt = col & (KT-1).
+// The plain layout is the single-block case (KT >= nh, W = nh).
+struct SpecSide {
+  long long KS;
+  long long fo, fr;
+  int lgKT;
+  int nb, Wlast;
 };
+
+// Offset of (outer, r0, col) and the offset step between positions r and r+L.
+__device__ __forceinline__ void side_addr(const SpecSide& s, int outer, int col, int r0, int L, long long& base,
+                                          long long& step) {
+  const int kb = col >> s.lgKT;
+  const int kt = col & ((1 << s.lgKT) - 1);
+  const long long W = kb == s.nb - 1 ? (long long)s.Wlast : (1LL << s.lgKT);
+  base = (long long)kb * s.KS + ((long long)outer * s.fo + (long long)r0 * s.fr) * W + kt;
+  step = (long long)L * s.fr * W;
+}
 
 struct ColArgs {
   const void* src;
   void* dst;
-  Stride2 so;         // plane index -> offset
-  Stride2 sr;         // transform position -> offset
+  SpecSide in, out;   // layouts read and written
   int ncols;          // columns per plane (nh)
   int nouter;         // planes
-  int T, lgT;         // columns per workgroup (power of two)
+  int outer_fast;     // workgroup order: 1 = consecutive workgroups take consecutive planes
+  int T, lgT;         // columns per group (power of two); a tile is G groups
   int tiles;          // tiles per plane
   int nblocks;        // nouter * tiles
   int ps;             // LDS pad shift
   int nx, ny;         // extents for the spectral symbol (x pass)
   double ddi, ddj, ddk;
   double* dc_out;  // FILTER / SYMBOL: receives the forward (0,0,0) term (sum of the input)
-  long long rstep; // sr(p + L) - sr(p) for every p when that is constant, else 0
   const void* tw;
   LinePlan lp;
 };
@@ -64,39 +79,12 @@ struct ColArgs {
 //       3 forward * symbol * inverse (applies L_h for a residual).
 enum { COL_FWD = 0, COL_INV = 1, COL_FILTER = 2, COL_SYMBOL = 3 };
 
-// SPLIT: exchange real and imaginary parts separately through half the LDS.
+// One column line held in registers through the transform, filter or symbol
+// and the inverse transform of MODE.
 template <typename F, int E, int MODE, bool SPLIT>
-__global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
-  const int tid = threadIdx.x;
-  const int c = tid & (a.T - 1);
-  const int r0 = tid >> a.lgT;
+__device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const cx<F>* __restrict__ tw, cx<F>* lds,
+                                         const ColIdx& idx, int r0, int outer, int col, bool valid) {
   const int L = a.lp.L;
-  const int b = xcd_remap(blockIdx.x, a.nblocks);
-  const int outer = b / a.tiles;
-  const int tile = b - outer * a.tiles;
-  const int col = tile * a.T + c;
-  const bool valid = col < a.ncols;
-  // Threads past the last column load the last column (in bounds, never
-  // stored), so the loads need no predicate.
-  const long long base = a.so(outer) + (valid ? col : a.ncols - 1);
-  // src may equal dst (in-place pass): every thread stores exactly the
-  // elements it loaded, after all of its loads.
-  const cx<F>* src = reinterpret_cast<const cx<F>*>(a.src) + base;
-  cx<F>* dst = reinterpret_cast<cx<F>*>(a.dst) + base;
-  const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
-
-  cx<F> v[E];
-  if (a.rstep) {
-    const cx<F>* s = src + a.sr(r0);
-#pragma unroll
-    for (int i = 0; i < E; ++i) v[i] = s[(long long)i * a.rstep];
-  } else {
-#pragma unroll
-    for (int i = 0; i < E; ++i) v[i] = src[a.sr(r0 + i * L)];
-  }
-  const ColIdx idx{a.T, c, a.ps};
   if constexpr (MODE == COL_FWD) {
     fft_line<E, -1, F, ColIdx, SPLIT>(v, a.lp, tw, lds, idx, r0);
   } else if constexpr (MODE == COL_INV) {
@@ -121,14 +109,61 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
     }
     fft_line<E, +1, F, ColIdx, SPLIT>(v, a.lp, tw, lds, idx, r0);
   }
-  if (valid) {
-    if (a.rstep) {
-      cx<F>* d = dst + a.sr(r0);
+}
+
+// A workgroup owns a tile of G*T neighbouring columns of one plane: thread
+// (c, r0) holds positions r0 + i*L of columns c + g*T (g < G).  The G groups
+// are transformed one after the other through the same LDS (T columns of
+// it), but all loads are issued first and all stores last, so each row of the
+// tile -- G*T*sizeof(complex) contiguous bytes -- is read and written whole.
+// SPLIT: exchange real and imaginary parts separately through half the LDS.
+template <typename F, int E, int MODE, bool SPLIT, int G>
+__global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int c = tid & (a.T - 1);
+  const int r0 = tid >> a.lgT;
+  const int L = a.lp.L;
+  const int b = xcd_remap(blockIdx.x, a.nblocks);
+  int outer, tile;
+  if (a.outer_fast) {
+    tile = b / a.nouter;
+    outer = b - tile * a.nouter;
+  } else {
+    outer = b / a.tiles;
+    tile = b - outer * a.tiles;
+  }
+  const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
+  // Threads past the last column load the last column (in bounds, never
+  // stored), so the loads need no predicate.  src may equal dst only when
+  // both sides use the same layout (in-place pass): every thread stores
+  // exactly the elements it loaded, after all of its loads.
+  cx<F> v[G][E];
 #pragma unroll
-      for (int i = 0; i < E; ++i) d[(long long)i * a.rstep] = v[i];
-    } else {
+  for (int g = 0; g < G; ++g) {
+    const int col = (tile * G + g) * a.T + c;
+    long long ib, is;
+    side_addr(a.in, outer, col < a.ncols ? col : a.ncols - 1, r0, L, ib, is);
+    const cx<F>* src = reinterpret_cast<const cx<F>*>(a.src) + ib;
 #pragma unroll
-      for (int i = 0; i < E; ++i) dst[a.sr(r0 + i * L)] = v[i];
+    for (int i = 0; i < E; ++i) v[g][i] = src[(long long)i * is];
+  }
+  const ColIdx idx{a.T, c, a.ps};
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int col = (tile * G + g) * a.T + c;
+    col_line<F, E, MODE, SPLIT>(v[g], a, tw, lds, idx, r0, outer, col, col < a.ncols);
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int col = (tile * G + g) * a.T + c;
+    if (col < a.ncols) {
+      long long ob, os;
+      side_addr(a.out, outer, col, r0, L, ob, os);
+      cx<F>* dst = reinterpret_cast<cx<F>*>(a.dst) + ob;
+#pragma unroll
+      for (int i = 0; i < E; ++i) dst[(long long)i * os] = v[g][i];
     }
   }
 }
@@ -139,7 +174,7 @@ struct RowArgs {
   const void* aux;  // f for the residual epilogue
   long long nrows;
   int nz, nh;
-  int ny, lgB;    // spectrum layout (see Stride2)
+  SpecSide spec;  // k_c2r: layout of the half spectrum read (lgKT >= 30: plain rows)
   int T;          // rows per workgroup
   uint32_t Linv;  // ceil(2^32 / L)
   int nblocks;
@@ -155,13 +190,8 @@ struct RowArgs {
 template <typename T>
 __device__ __forceinline__ double to_d(T x) { return double(x); }
 
-// Complex offset of spectrum row (i, j), row = i*ny + j.
-__device__ __forceinline__ long long spec_row(const RowArgs& a, long long row) {
-  const int i = (int)(row / a.ny);
-  const int j = (int)(row - (long long)i * a.ny);
-  const int B = 1 << a.lgB;
-  return ((long long)(i >> a.lgB) * a.ny + j) * ((long long)B * a.nh) + (long long)(i & (B - 1)) * a.nh;
-}
+// Complex offset of plain spectrum row (i, j), row = i*ny + j.
+__device__ __forceinline__ long long spec_row(const RowArgs& a, long long row) { return row * a.nh; }
 
 // z forward: real rows (IN precision) -> half-spectrum rows (F precision).
 // Even nz: pack (x[2r], x[2r+1]) as one complex, an nz/2-point transform, then
@@ -280,15 +310,42 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
   const int M = a.lp.n;
   const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
   const RowIdx idx{line * a.pitch, a.ps};
-  // rows past the end load the last row (in bounds); their stores are predicated
-  const cx<F>* __restrict__ src = reinterpret_cast<const cx<F>*>(a.src) + spec_row(a, valid ? row : a.nrows - 1);
-  // stage the half spectrum in LDS
+  // stage the half spectrum of the workgroup's rows in LDS; rows past the end
+  // load the last row (in bounds), their stores are predicated
+  if (a.spec.lgKT >= 30) {
+    const cx<F>* __restrict__ src = reinterpret_cast<const cx<F>*>(a.src) + spec_row(a, valid ? row : a.nrows - 1);
 #pragma unroll
-  for (int i = 0; i < E; ++i) {
-    const int k = r0 + i * L;
-    if (!ODD || k < a.nh) lds[idx(k)] = src[k];
+    for (int i = 0; i < E; ++i) {
+      const int k = r0 + i * L;
+      if (!ODD || k < a.nh) lds[idx(k)] = src[k];
+    }
+    if (!ODD && r0 == 0) lds[idx(M)] = src[M];
+  } else {
+    // y-blocked spectrum: the workgroup's T rows are contiguous inside each
+    // kh block (T*KT elements), so all threads copy element e of the tile in
+    // (block, row, kt) order
+    const cx<F>* __restrict__ src = reinterpret_cast<const cx<F>*>(a.src);
+    const long long row0 = (long long)xcd_remap(blockIdx.x, a.nblocks) * a.T;
+    const int KT = 1 << a.spec.lgKT;
+    const int TK = a.T * KT;
+    const int total = a.T * a.nh;
+    const int nthreads = a.T * L;
+    for (int e = tid; e < total; e += nthreads) {
+      int kb = e / TK;
+      int W = KT;
+      if (kb >= a.spec.nb - 1) {
+        kb = a.spec.nb - 1;
+        W = a.spec.Wlast;
+      }
+      const int rem = e - kb * TK;
+      const int rr = rem / W;
+      const int kt = rem - rr * W;
+      long long r = row0 + rr;
+      if (r >= a.nrows) r = a.nrows - 1;
+      lds[rr * a.pitch + ((kb * KT + kt) + ((kb * KT + kt) >> a.ps))] =
+          src[(long long)kb * a.spec.KS + r * W + kt];
+    }
   }
-  if (!ODD && r0 == 0) lds[idx(M)] = src[M];
   __syncthreads();
   cx<F> v[E];
   if constexpr (!ODD) {
@@ -344,7 +401,7 @@ struct Launch {
 
 template <int E>
 struct PassLaunchers {
-  static hipError_t col(int f32, int mode, int split, const Launch& l, const ColArgs& a);
+  static hipError_t col(int f32, int mode, int split, int groups, const Launch& l, const ColArgs& a);
   static hipError_t r2c(int f32, int in_f32, int odd, const Launch& l, const RowArgs& a);
   static hipError_t c2r(int f32, int out_kind, int odd, int epi, const Launch& l, const RowArgs& a);
 };

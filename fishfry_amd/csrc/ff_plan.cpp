@@ -221,6 +221,8 @@ bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, i
   out.ps = 4;
   out.threads = T * L;
   out.split = (flags & FF_TUNE_SPLIT_LDS) ? 1 : 0;
+  out.groups = (flags & FF_TUNE_TWO_GROUPS) ? 2 : 1;
+  if (out.groups == 2 && out.line.E != 8 && out.line.E != 16) return false;
   const size_t elems = (size_t)n * T + ((size_t)(n - 1) >> out.ps) + 1;
   out.lds = elems * (size_t)(out.split ? elem_bytes / 2 : elem_bytes);
   return out.lds <= 160 * 1024;
@@ -230,7 +232,9 @@ bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T) {
   const int M = (nz % 2 == 0) ? nz / 2 : nz;
   if (!plan_line(M, out.line, E)) return false;
   const int L = out.line.lp.L;
-  if (T <= 0) T = std::max(1, 256 / L);
+  // default: about 128 threads per workgroup (measured fastest for both z
+  // passes at 1024^3 on MI355X: more workgroups per CU beat wider ones)
+  if (T <= 0) T = std::max(1, 128 / L);
   if (T < 1 || T * L > max_threads_for(out.line.E)) return false;
   out.T = T;
   out.ps = 4;
@@ -247,22 +251,10 @@ bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int kind, int p
   return col_cfg_with(n, ncols, elem_bytes, out, 0, 0);
 }
 
-// Built-in defaults measured on MI355X at 1024^3 (FP64): the z inverse is
-// fastest with 4 elements per thread, the z forward with 8.
-static bool builtin_default(int kind, int n, int prec, int& E, int& T) {
-  if (prec == 0 && kind == 3 && n == 512) {
-    E = 4;
-    T = 4;
-    return true;
-  }
-  return false;
-}
-
 bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int kind, int prec) {
   const int M = (nz % 2 == 0) ? nz / 2 : nz;
   int E = 0, T = 0, flags = 0;
   if (tune_lookup(kind, M, prec, E, T, flags) && row_cfg_with(nz, elem_bytes, out, E, T)) return true;
-  if (builtin_default(kind, M, prec, E, T) && row_cfg_with(nz, elem_bytes, out, E, T)) return true;
   return row_cfg_with(nz, elem_bytes, out, 0, 0);
 }
 
@@ -276,7 +268,7 @@ ff_status tune_set(const ff_tune* t) {
     return FF_OK;
   }
   const int eb = t->precision ? 8 : 16;
-  if (t->flags & ~FF_TUNE_SPLIT_LDS) return set_error(FF_ERR_INVALID, "ff_tune_set: unknown flags");
+  if (t->flags & ~(FF_TUNE_SPLIT_LDS | FF_TUNE_TWO_GROUPS)) return set_error(FF_ERR_INVALID, "ff_tune_set: unknown flags");
   bool ok;
   if (t->kind == 0 || t->kind == 3) {
     RowCfg c;
@@ -300,7 +292,7 @@ void tune_reset() {
 
 #define FF_DECL(E)                                                                                         \
   template <>                                                                                              \
-  hipError_t PassLaunchers<E>::col(int, int, int, const Launch&, const ColArgs&);                               \
+  hipError_t PassLaunchers<E>::col(int, int, int, int, const Launch&, const ColArgs&);                               \
   template <>                                                                                              \
   hipError_t PassLaunchers<E>::r2c(int, int, int, const Launch&, const RowArgs&);                          \
   template <>                                                                                              \
@@ -340,8 +332,8 @@ FF_DECL(30)
     default: return hipErrorInvalidValue;    \
   }
 
-hipError_t launch_col(int E, int f32, int mode, int split, const Launch& l, const ColArgs& a) {
-  FF_SWITCH_E(E, col(f32, mode, split, l, a))
+hipError_t launch_col(int E, int f32, int mode, int split, int groups, const Launch& l, const ColArgs& a) {
+  FF_SWITCH_E(E, col(f32, mode, split, groups, l, a))
 }
 hipError_t launch_r2c(int E, int f32, int in_f32, int odd, const Launch& l, const RowArgs& a) {
   FF_SWITCH_E(E, r2c(f32, in_f32, odd, l, a))

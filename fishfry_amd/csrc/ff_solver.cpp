@@ -93,12 +93,8 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   p->ddk = dd_const(n[2], lo[2], hi[2]);
   const int ones[3] = {1, 1, 1};
   p->bytes = reference_bytes(n, ones);
-  // Block x when an x-plane of the half spectrum spans more than a 2 MiB
-  // page: B = the largest power of two <= 16 dividing nx.
-  p->lgB = 0;
-  if ((o.flags & FF_BLOCKED_LAYOUT) && (long long)n[1] * p->nh * 16 > (2LL << 20)) {
-    while (p->lgB < 4 && n[0] % (2 << p->lgB) == 0) p->lgB++;
-  }
+  // kh blocks of 8 complex doubles (one 128-byte line per block row)
+  p->lgKT = (o.flags & FF_BLOCKED_LAYOUT) ? 3 : 0;
 
   ff_status st = hip_check(hipGetDevice(&p->device), "hipGetDevice");
   if (st != FF_OK) {
@@ -192,6 +188,45 @@ void plan_destroy(ff_plan* p) {
 
 // ---- passes -------------------------------------------------------------------
 
+// A layout as seen by a column pass along axis (1 = y: planes i, positions j;
+// 0 = x: planes j, positions i), or by the row passes (axis 1).
+static SpecSide spec_side(const ff_plan* p, int axis, int lay) {
+  SpecSide s;
+  memset(&s, 0, sizeof s);
+  const long long nx = p->n[0], ny = p->n[1];
+  if (lay == LAY_PLAIN) {
+    s.lgKT = 30;  // one block of width nh
+    s.nb = 1;
+    s.Wlast = p->nh;
+  } else {
+    const int KT = 1 << p->lgKT;
+    s.lgKT = p->lgKT;
+    s.nb = (p->nh + KT - 1) / KT;
+    s.Wlast = p->nh - (s.nb - 1) * KT;
+    s.KS = nx * ny * KT;
+  }
+  if (lay == LAY_X) {  // [kb][j][i][kt]
+    s.fo = axis == 1 ? 1 : nx;
+    s.fr = axis == 1 ? nx : 1;
+  } else {             // [i][j] inside each block
+    s.fo = axis == 1 ? ny : 1;
+    s.fr = axis == 1 ? 1 : ny;
+  }
+  return s;
+}
+
+void pass_layouts(const ff_plan* p, int pass, int& lay_in, int& lay_out) {
+  lay_in = lay_out = LAY_PLAIN;
+  if (!p->lgKT) return;
+  switch (pass) {
+    case 1: lay_out = LAY_Y; break;                  // y forward: plain -> y-blocked
+    case 2: lay_in = LAY_Y; lay_out = LAY_X; break;  // x: y-blocked -> x-blocked
+    case 3: lay_in = LAY_X; lay_out = LAY_Y; break;  // y inverse: x-blocked -> y-blocked
+    case 4: lay_in = LAY_Y; break;                   // z inverse reads y-blocked
+    default: break;
+  }
+}
+
 static RowArgs row_args(const ff_plan* p, int prec, bool inverse) {
   const RowCfg& c = inverse ? p->zi[prec] : p->zr[prec];
   RowArgs a;
@@ -199,8 +234,7 @@ static RowArgs row_args(const ff_plan* p, int prec, bool inverse) {
   a.nrows = (long long)p->n[0] * p->n[1];
   a.nz = p->n[2];
   a.nh = p->nh;
-  a.ny = p->n[1];
-  a.lgB = p->lgB;
+  a.spec = spec_side(p, 1, LAY_PLAIN);
   a.T = c.T;
   a.Linv = c.Linv;
   a.nblocks = (int)((a.nrows + c.T - 1) / c.T);
@@ -225,8 +259,9 @@ ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void
 }
 
 ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void* src, void* dst, const double* f,
-                   const double* shiftp, hipStream_t s) {
+                   const double* shiftp, hipStream_t s, int lay_in) {
   RowArgs a = row_args(p, prec, true);
+  a.spec = spec_side(p, 1, lay_in);
   a.src = src;
   a.dst = dst;
   a.aux = f;
@@ -240,27 +275,23 @@ ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void
 }
 
 ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* src, void* dst, hipStream_t s,
-                  double* dc_out) {
+                  double* dc_out, int lay_in, int lay_out) {
   const ColCfg& c = axis == 1 ? p->cy[prec] : p->cx[prec];
+  if (src == dst && lay_in != lay_out) return set_error(FF_ERR_INVALID, "in-place column pass must keep its layout");
   ColArgs a;
   memset(&a, 0, sizeof a);
   a.src = src;
   a.dst = dst;
   a.ncols = p->nh;
-  const long long Bnh = (long long)p->nh << p->lgB;
-  const long long blk = (long long)p->n[1] * Bnh;  // one block of 2^lgB x-planes
-  if (axis == 1) {  // y: planes i, positions j   (ParisPencil.cpp:79 plan, batch nx*nh)
-    a.nouter = p->n[0];
-    a.so = Stride2{blk, p->nh, p->lgB};
-    a.sr = Stride2{0, Bnh, 30};
-  } else {  // x: planes j, positions i          (ParisPencil.cpp:78 plan, batch ny*nh)
-    a.nouter = p->n[1];
-    a.so = Stride2{0, Bnh, 30};
-    a.sr = Stride2{blk, p->nh, p->lgB};
-  }
+  // y: planes i, positions j (ParisPencil.cpp:79 plan, batch nx*nh);
+  // x: planes j, positions i (ParisPencil.cpp:78 plan, batch ny*nh)
+  a.nouter = axis == 1 ? p->n[0] : p->n[1];
+  a.in = spec_side(p, axis, lay_in);
+  a.out = spec_side(p, axis, lay_out);
+  a.outer_fast = 0;
   a.T = c.T;
   a.lgT = c.lgT;
-  a.tiles = (a.ncols + c.T - 1) / c.T;
+  a.tiles = (a.ncols + c.groups * c.T - 1) / (c.groups * c.T);
   const long long nb = (long long)a.nouter * a.tiles;
   if (nb > INT_MAX) return set_error(FF_ERR_INVALID, "grid too large");
   a.nblocks = (int)nb;
@@ -271,16 +302,10 @@ ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* sr
   a.ddj = p->ddj;
   a.ddk = p->ddk;
   a.dc_out = dc_out;
-  // positions r0 + i*L (i < E) are a constant stride apart when the two-level
-  // stride is one level (lg >= 30) or L is a multiple of the block 2^lg
-  const int L = c.line.lp.L;
-  if (a.sr.lg >= 30) a.rstep = (long long)L * a.sr.lo;
-  else if (L % (1 << a.sr.lg) == 0) a.rstep = (long long)(L >> a.sr.lg) * a.sr.hi;
-  else a.rstep = 0;
   a.tw = axis == 1 ? p->tw_y[prec] : p->tw_x[prec];
   a.lp = c.line.lp;
   const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
-  return hip_check(launch_col(c.line.E, prec, mode, c.split, l, a), "column pass launch");
+  return hip_check(launch_col(c.line.E, prec, mode, c.split, c.groups, l, a), "column pass launch");
 }
 
 static ff_status solve_fp32_refine(const ff_plan* p, const double* f, double* phi, hipStream_t s) {
@@ -319,15 +344,17 @@ ff_status solve_enqueue(const ff_plan* p, double* density, double* potential, hi
   // with FF_PRESERVE_INPUT); the result lands in potential as in
   // ParisPencil.cpp:103-104,419
   void* w = p->work ? p->work : (void*)density;
+  int li[5], lo[5];
+  for (int k = 0; k < 5; ++k) pass_layouts(p, k, li[k], lo[k]);
   if ((st = run_zfwd(p, 0, 0, density, potential, s))) return st;
   if (ev && (st = hip_check(hipEventRecord(ev[0], s), "event"))) return st;
-  if ((st = run_col(p, 0, 1, COL_FWD, potential, w, s))) return st;
+  if ((st = run_col(p, 0, 1, COL_FWD, potential, w, s, nullptr, li[1], lo[1]))) return st;
   if (ev && (st = hip_check(hipEventRecord(ev[1], s), "event"))) return st;
-  if ((st = run_col(p, 0, 0, COL_FILTER, w, potential, s))) return st;
+  if ((st = run_col(p, 0, 0, COL_FILTER, w, potential, s, nullptr, li[2], lo[2]))) return st;
   if (ev && (st = hip_check(hipEventRecord(ev[2], s), "event"))) return st;
-  if ((st = run_col(p, 0, 1, COL_INV, potential, w, s))) return st;
+  if ((st = run_col(p, 0, 1, COL_INV, potential, w, s, nullptr, li[3], lo[3]))) return st;
   if (ev && (st = hip_check(hipEventRecord(ev[3], s), "event"))) return st;
-  if ((st = run_zinv(p, 0, 0, EPI_STORE, w, potential, nullptr, nullptr, s))) return st;
+  if ((st = run_zinv(p, 0, 0, EPI_STORE, w, potential, nullptr, nullptr, s, li[4]))) return st;
   if (ev && (st = hip_check(hipEventRecord(ev[4], s), "event"))) return st;
   return FF_OK;
 }

@@ -7,7 +7,7 @@ struct ff_plan {
   int n[3];
   double lo[3], hi[3];
   int nh;
-  int lgB;       // spectrum layout: x split in blocks of 2^lgB planes
+  int lgKT;      // FP64 spectrum between the y passes: 0 plain, else kh blocks of 2^lgKT
   long long N;   // nx*ny*nz
   long long Nh;  // nx*ny*nh
   double ddi, ddj, ddk;
@@ -39,10 +39,16 @@ namespace ff {
 
 // Pass kernels over caller-provided buffers.  prec: 0 FP64, 1 FP32.
 ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void* dst, hipStream_t s);
+// Half-spectrum layouts (see SpecSide): plain [i][j][kh], y-blocked
+// [kb][i][j][kt], x-blocked [kb][j][i][kt].
+enum { LAY_PLAIN = 0, LAY_Y = 1, LAY_X = 2 };
 ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* src, void* dst, hipStream_t s,
-                  double* dc_out = nullptr);
+                  double* dc_out = nullptr, int lay_in = LAY_PLAIN, int lay_out = LAY_PLAIN);
 ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void* src, void* dst, const double* f,
-                   const double* shiftp, hipStream_t s);
+                   const double* shiftp, hipStream_t s, int lay_in = LAY_PLAIN);
+// Layouts the FP64 solve's column passes read and write: y forward, x, y inverse
+// (and the z inverse reads what the y inverse wrote).
+void pass_layouts(const ff_plan* p, int pass, int& lay_in, int& lay_out);
 
 // The whole solve; ev (optional) receives one record after each of the five
 // FP64 passes.

@@ -47,9 +47,12 @@ typedef enum ff_precision {
 #define FF_PRESERVE_INPUT 1 /* FP64: the plan owns a third work array (ff_plan_bytes()
                                more device memory) and solve leaves density intact;
                                FP32 modes always preserve density                   */
-#define FF_BLOCKED_LAYOUT 2 /* store the intermediate half spectrum with x in blocks of
-                               up to 16 planes once an x-plane exceeds 2 MiB (default:
-                               the reference's [i][j][kh]; see ff_plan_describe "block") */
+#define FF_BLOCKED_LAYOUT 2 /* FP64: between the y passes store the half spectrum in
+                               kh blocks of KT = 8 (y-blocked [kb][i][j][kt] after the y
+                               forward and y inverse, x-blocked [kb][j][i][kt] after the
+                               x pass) so every column pass writes contiguous tiles;
+                               default: the reference's [i][j][kh] throughout
+                               (ff_plan_describe "kt": 0 or KT)                       */
 
 typedef struct ff_options {
   int precision;    /* ff_precision                                             */
@@ -111,9 +114,13 @@ ff_status ff_plan_describe(const ff_plan* plan, char* buf, size_t len);
 /* One pass of the solve, for per-pass parity checks and timing:
  * 0 z forward (real src -> complex dst), 1 y forward, 2 x forward*filter*
  * inverse, 3 y inverse, 4 z inverse + scale (complex src -> real dst).
- * Real arrays are [i][j][k]; the half spectrum is the plan's layout: element
- * (i, j, kh) at ((i/B)*ny + j)*B*nh + (i%B)*nh + kh, B from ff_plan_describe
- * (B = 1, the reference's D2Z output layout, unless FF_BLOCKED_LAYOUT). */
+ * Real arrays are [i][j][k].  The half spectrum is [i][j][kh] (the reference's
+ * D2Z output layout); with FF_BLOCKED_LAYOUT pass 1 writes y-blocked, pass 2
+ * reads y-blocked and writes x-blocked, pass 3 reads x-blocked and writes
+ * y-blocked, pass 4 reads y-blocked.  Blocked layouts, with kh = kb*KT + kt,
+ * nb = ceil(nh/KT), W = KT (nh - (nb-1)*KT for the last block):
+ *   y-blocked (i, j, kh) at kb*nx*ny*KT + (i*ny + j)*W + kt
+ *   x-blocked (i, j, kh) at kb*nx*ny*KT + (j*nx + i)*W + kt            */
 
 /* Tuning overrides consulted when a plan is created (process-wide, thread
  * safe).  kind: 0 z forward rows, 1 y columns, 2 x columns, 3 z inverse
@@ -128,12 +135,17 @@ typedef struct ff_tune {
   int precision;
   int E;
   int T;
-  int flags; /* FF_TUNE_SPLIT_LDS (column passes) */
+  int flags; /* FF_TUNE_SPLIT_LDS | FF_TUNE_TWO_GROUPS (column passes) */
 } ff_tune;
 
-/* ff_tune.flags: column passes exchange real and imaginary parts separately
- * through half the LDS (more workgroups per CU, twice the barriers). */
+/* ff_tune.flags (column passes):
+ * FF_TUNE_SPLIT_LDS   exchange real and imaginary parts separately through
+ *                     half the LDS (more workgroups per CU, twice the barriers)
+ * FF_TUNE_TWO_GROUPS  a workgroup transforms two groups of T columns one after
+ *                     the other, so it reads and writes 2*T-column rows whole
+ *                     (E = 8 or 16)                                          */
 #define FF_TUNE_SPLIT_LDS 1
+#define FF_TUNE_TWO_GROUPS 2
 
 ff_status ff_tune_set(const ff_tune* t);
 void ff_tune_reset(void);

@@ -51,21 +51,47 @@ def get_real(buf, n):
     return buf[:N].cpu().numpy().reshape(n)
 
 
-def block_of(plan):
-    return json.loads(plan.describe())["block"]
+def kt_of(plan):
+    return json.loads(plan.describe())["kt"]
 
 
-def get_half(buf, n, B=1):
-    """Half spectrum [i][j][kh] from the plan's layout ((i/B)*ny + j)*B*nh + (i%B)*nh + kh."""
+PLAIN, YB, XB = 0, 1, 2
+
+
+def to_layout(spec, lay, KT):
+    """Flat complex array of spec[i][j][kh] in a spectrum layout: plain
+    [i][j][kh], y-blocked [kb][i][j][kt] or x-blocked [kb][j][i][kt]."""
+    if lay == PLAIN:
+        return np.ascontiguousarray(spec).ravel()
+    parts = []
+    for k0 in range(0, spec.shape[2], KT):
+        blk = spec[:, :, k0:k0 + KT]
+        parts.append(np.ascontiguousarray(blk if lay == YB else blk.transpose(1, 0, 2)).ravel())
+    return np.concatenate(parts)
+
+
+def from_layout(flat, n, lay, KT):
     nh = n[2] // 2 + 1
-    M = n[0] * n[1] * nh
-    a = buf[: 2 * M].cpu().numpy().view(np.complex128)
-    return a.reshape(n[0] // B, n[1], B, nh).transpose(0, 2, 1, 3).reshape(n[0], n[1], nh)
+    if lay == PLAIN:
+        return flat.reshape(n[0], n[1], nh)
+    out = np.empty((n[0], n[1], nh), dtype=flat.dtype)
+    off = 0
+    for k0 in range(0, nh, KT):
+        W = min(KT, nh - k0)
+        size = n[0] * n[1] * W
+        blk = flat[off:off + size]
+        out[:, :, k0:k0 + W] = blk.reshape(n[0], n[1], W) if lay == YB else blk.reshape(n[1], n[0], W).transpose(1, 0, 2)
+        off += size
+    return out
 
 
-def put_half(buf, spec, B=1):
-    n0, n1, nh = spec.shape
-    put(buf, np.ascontiguousarray(spec.reshape(n0 // B, B, n1, nh).transpose(0, 2, 1, 3)))
+def get_half(buf, n, lay=PLAIN, KT=0):
+    M = n[0] * n[1] * (n[2] // 2 + 1)
+    return from_layout(buf[: 2 * M].cpu().numpy().view(np.complex128), n, lay, KT)
+
+
+def put_half(buf, spec, lay=PLAIN, KT=0):
+    put(buf, to_layout(spec, lay, KT))
 
 
 def gpu_solve(f, **kw):
@@ -84,12 +110,17 @@ PASS_SHAPES = [(16, 24, 20), (32, 32, 32), (12, 10, 9), (64, 48, 40), (5, 6, 4),
                (30, 25, 27), (32, 1024, 512), (24, 600, 1000)]
 
 
+@pytest.mark.parametrize("flags", [0, ff.FF_BLOCKED_LAYOUT])
 @pytest.mark.parametrize("n", PASS_SHAPES)
-def test_each_pass_matches_oracle(n):
+def test_each_pass_matches_oracle(n, flags):
     rng = np.random.default_rng(sum(n))
-    plan, nb, a, b = make(n)
-    B = block_of(plan)
-    say("pass parity", n, "block", B)
+    plan, nb, a, b = make(n, flags=flags)
+    KT = kt_of(plan)
+    assert KT == (8 if flags else 0)
+    # layouts read / written by passes 1..4 (plain throughout unless blocked)
+    lay = {1: (PLAIN, YB), 2: (YB, XB), 3: (XB, YB), 4: (YB, PLAIN)} if KT else \
+        {p: (PLAIN, PLAIN) for p in (1, 2, 3, 4)}
+    say("pass parity", n, "kt", KT)
     nh = n[2] // 2 + 1
     lo, hi, _ = oracle.grid(n)
     # pass 0: z forward (D2Z)
@@ -97,23 +128,26 @@ def test_each_pass_matches_oracle(n):
     put(a, f)
     plan.solve_pass(0, a, b)
     torch.cuda.synchronize()
-    assert rel(get_half(b, n, B), oracle.rfft(f)) < TOL
+    assert rel(get_half(b, n), oracle.rfft(f)) < TOL
     spec = rng.standard_normal((n[0], n[1], nh)) + 1j * rng.standard_normal((n[0], n[1], nh))
     # pass 1: y forward
-    put_half(a, spec, B)
+    put_half(a, spec, lay[1][0], KT)
     plan.solve_pass(1, a, b)
     torch.cuda.synchronize()
-    assert rel(get_half(b, n, B), oracle.fft(spec, -1, axis=1)) < TOL
+    assert rel(get_half(b, n, lay[1][1], KT), oracle.fft(spec, -1, axis=1)) < TOL
     # pass 2: x forward * filter * inverse
+    put_half(a, spec, lay[2][0], KT)
     plan.solve_pass(2, a, b)
     torch.cuda.synchronize()
     want = oracle.fft(oracle.filter_half(oracle.fft(spec, -1, axis=0), n[2], lo, hi), +1, axis=0)
-    assert rel(get_half(b, n, B), want) < TOL
+    assert rel(get_half(b, n, lay[2][1], KT), want) < TOL
     # pass 3: y inverse
+    put_half(a, spec, lay[3][0], KT)
     plan.solve_pass(3, a, b)
     torch.cuda.synchronize()
-    assert rel(get_half(b, n, B), oracle.fft(spec, +1, axis=1)) < TOL
+    assert rel(get_half(b, n, lay[3][1], KT), oracle.fft(spec, +1, axis=1)) < TOL
     # pass 4: z inverse (Z2D) and the 1/N scale; Im of DC / Nyquist ignored
+    put_half(a, spec, lay[4][0], KT)
     plan.solve_pass(4, a, b)
     torch.cuda.synchronize()
     want = oracle.irfft(spec, n[2]) * (1.0 / (float(n[0]) * float(n[1]) * float(n[2])))
@@ -121,51 +155,59 @@ def test_each_pass_matches_oracle(n):
     plan.close()
 
 
-def test_layouts_agree_bitwise():
-    """The blocked spectrum layout changes addresses only, not arithmetic."""
-    n = (32, 1024, 512)
+@pytest.mark.parametrize("n", [(32, 1024, 512), (48, 40, 36), (30, 25, 27), (64, 64, 2), (96, 160, 48)])
+def test_layouts_agree_bitwise(n):
+    """The blocked spectrum layouts change addresses only, not arithmetic."""
     rng = np.random.default_rng(4)
     f = rng.standard_normal(n)
     outs = []
-    for flags in (0, ff.FF_BLOCKED_LAYOUT):
+    for flags in (0, ff.FF_BLOCKED_LAYOUT, ff.FF_BLOCKED_LAYOUT | ff.FF_PRESERVE_INPUT):
         plan, nb, a, b = make(n, flags=flags)
-        say("layout", flags, "block", block_of(plan))
-        assert block_of(plan) == (16 if flags else 1)
+        say("layout", n, flags, "kt", kt_of(plan))
         put(a, f)
         plan.solve(nb, a, b)
         torch.cuda.synchronize()
         outs.append(get_real(b, n))
         plan.close()
-    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
     assert rel(outs[0], oracle.solve(f)) < TOL
 
 
+@pytest.mark.parametrize("variant", [ff.FF_TUNE_SPLIT_LDS, ff.FF_TUNE_TWO_GROUPS,
+                                     ff.FF_TUNE_SPLIT_LDS | ff.FF_TUNE_TWO_GROUPS])
 @pytest.mark.parametrize("n,E", [((64, 48, 40), 0), ((30, 25, 27), 0), ((96, 1000, 64), 10), ((1024, 16, 8), 16),
-                                 ((768, 12, 6), 12)])
-def test_split_exchange_bitwise(n, E):
-    """The column passes' split real/imaginary LDS exchange moves the same
-    values as the complex exchange.  The two are separate kernels, so the
-    compiler may contract multiply-adds differently: results agree to
-    round-off, and both match the oracle."""
+                                 ((768, 12, 6), 12), ((128, 64, 96), 8), ((256, 1024, 20), 16)])
+def test_column_variants_agree(n, E, variant):
+    """Column-pass variants -- the split real/imaginary LDS exchange, two
+    column groups per workgroup -- move the same values as the default
+    kernel.  They are separate kernels, so the compiler may contract
+    multiply-adds differently: results agree to round-off, and all match the
+    oracle.  Two groups apply where E is 8 or 16."""
     rng = np.random.default_rng(5)
     f = rng.standard_normal(n)
     try:
         for prec, steps in ((ff.FF_FP64, 0), (ff.FF_FP32_REFINE, 1)):
             outs = []
             pi = 0 if prec == ff.FF_FP64 else 1
-            for split in (0, ff.FF_TUNE_SPLIT_LDS):
+            for flags in (0, variant):
                 ff.tune_reset()
                 for kind, length in ((1, n[1]), (2, n[0])):
                     for p in sorted({0, pi}):  # FP32 refinement also runs FP64 residual passes
                         cur = ff.tune_query(kind, length, p, n[2] // 2 + 1)
-                        try:
-                            ff.tune_set(kind, length, p, E or cur.E, 0 if E else cur.T, split)
-                        except ff.FishFryError:  # E does not divide this length
-                            ff.tune_set(kind, length, p, cur.E, cur.T, split)
-                say("split", n, E, prec, split)
+                        for fl in (flags, flags & ff.FF_TUNE_SPLIT_LDS, 0):  # drop what this E cannot do
+                            try:
+                                ff.tune_set(kind, length, p, E or cur.E, 0 if E else cur.T, fl)
+                                break
+                            except ff.FishFryError:  # E does not divide this length
+                                try:
+                                    ff.tune_set(kind, length, p, cur.E, cur.T, fl)
+                                    break
+                                except ff.FishFryError:
+                                    continue
+                say("variant", n, E, prec, flags)
                 outs.append(gpu_solve(f, precision=prec, refine_steps=steps))
             d = rel(outs[1], outs[0])
-            say("  split vs complex exchange: rel L2", d)
+            say("  variant vs default kernel: rel L2", d)
             # FP32 passes + 1 refinement are accurate to ~1e-11; FP64 to round-off
             assert d < (1e-14 if prec == ff.FF_FP64 else 1e-10)
             want = oracle.solve(f)
