@@ -18,6 +18,10 @@ Prints one JSON line on stdout (progress goes to stderr):
                analytic f plus 16 random modes: points/s and Linf beside the
                FP64 solve of the same f
   cpu_baseline the CPU oracle's FP64 solve on the host cores (bounded sample)
+  extras       the other configurations, measured beside the headline: FP64
+               solves at 512^3, 768^3 and 1000^3; the 64-grid batching
+               workload (FP64 and FP32 + 2 steps); the Poisson serving
+               workload (p50/p99, points/s)
 """
 import argparse
 import json
@@ -106,6 +110,161 @@ def cpu_baseline(size, budget_s=25.0):
     }
 
 
+def smooth_edge(e):
+    """Nearest integer >= 2 whose only prime factors are 2, 3 and 5 (the
+    transform lengths the passes support)."""
+    def ok(m):
+        for p in (2, 3, 5):
+            while m % p == 0:
+                m //= p
+        return m == 1
+    for d in range(64):
+        for c in (e - d, e + d):
+            if c >= 2 and ok(c):
+                return c
+    return e
+
+
+def bench_sizes(ff, torch, sizes, K):
+    """FP64 solves of the analytic f at other sizes of the plain-method
+    configuration: ms per solve, points/s and Linf."""
+    out = []
+    for n in sizes:
+        lo, hi, _ = ff.benchmark_grid(n)
+        plan = ff.ParisPencil(n, lo, hi, flags=ff.FF_PRESERVE_INPUT)
+        nb = plan.bytes()
+        f = torch.empty(nb // 8, dtype=torch.float64, device="cuda")
+        phi = torch.empty(nb // 8, dtype=torch.float64, device="cuda")
+        field = ff.field_default(n)
+        ff.field_init(field, f)
+        plan.solve_async(nb, f, phi)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            plan.solve_async(nb, f, phi)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / K
+        linf = ff.field_norms(field, phi)[2]
+        N = n[0] * n[1] * n[2]
+        out.append({"n": list(n), "ms": round(ms, 4), "points_per_s": N / (ms * 1e-3), "linf": linf,
+                    "hbm_frac": round(sum(b for _, b in pass_bytes(n)) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
+        log(f"FP64 {n}: {ms:.3f} ms, {N / (ms * 1e-3):.4e} points/s, Linf {linf:.3e}")
+        plan.close()
+        del f, phi
+        torch.cuda.empty_cache()
+    return out
+
+
+def bench_batching(ff, torch, K):
+    """The batching configuration: 64 grids drawn (seeded) from {32^3, 48^3,
+    64^3, 96^3, 128^3, 160^3, 128x64x96}, f_b(x) = s_b f(x + delta_b), one
+    HIP-graph batch per precision, autotuned; aggregate points/s and the worst
+    Linf against phi_b = s_b phi(x + delta_b)."""
+    import numpy as np
+
+    sizes = [(32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 128, 128), (160, 160, 160),
+             (128, 64, 96)]
+    rng = np.random.default_rng(20261202)
+    shapes = [sizes[i] for i in rng.integers(0, len(sizes), 64)]
+    fields = []
+    for n in shapes:
+        fd = ff.field_default(n)
+        for a in range(3):
+            fd.shift[a] = float(rng.uniform(0, 2 * math.pi))
+        fd.scale = float(rng.uniform(0.5, 2.0))
+        fields.append(fd)
+    points = sum(n[0] * n[1] * n[2] for n in shapes)
+    out = []
+    stream = torch.cuda.Stream()  # graph capture needs a stream of its own
+    for precision, steps in ((ff.FF_FP64, 0), (ff.FF_FP32_REFINE, 2)):
+        items = [{"n": n, "precision": precision, "refine_steps": steps} for n in shapes]
+        b = ff.Batch(items, ff.FF_BATCH_GRAPH)
+        d = [torch.zeros(b.bytes(i) // 8, dtype=torch.float64, device="cuda") for i in range(64)]
+        p = [torch.zeros(b.bytes(i) // 8, dtype=torch.float64, device="cuda") for i in range(64)]
+        torch.cuda.synchronize()
+        tuned, before, after = b.autotune(d, p, reps=2, stream=stream)
+        stream.synchronize()
+        for k in range(64):
+            ff.field_init(fields[k], d[k])
+        torch.cuda.synchronize()
+        b.solve(d, p, stream)
+        b.wait(stream)
+        stream.synchronize()
+        worst = max(ff.field_norms(fields[k], p[k])[2] for k in range(64))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            b.solve(d, p, stream)
+        b.wait(stream)
+        stream.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / K
+        out.append({"precision": "fp64" if precision == ff.FF_FP64 else "fp32+2", "grids": 64, "points": points,
+                    "ms_per_batch": round(ms, 4), "points_per_s": points / (ms * 1e-3), "worst_linf": worst,
+                    "autotuned_passes": tuned, "ms_before_tuning": round(before, 4), "ms_after_tuning": round(after, 4)})
+        log(f"batch {out[-1]['precision']}: {ms:.3f} ms per 64-grid batch, {points / (ms * 1e-3):.4e} points/s, "
+            f"worst Linf {worst:.3e}, {tuned} passes tuned ({before:.3f} -> {after:.3f} ms)")
+        b.close()
+        del d, p
+        torch.cuda.empty_cache()
+    return out
+
+
+def bench_serving(ff, nreq=96, seed=20261202):
+    """The serving configuration: Poisson arrivals, edges log-uniform in
+    [32, 512] rounded to 2^a 3^b 5^c (half cubic), half FP64 and half FP32 + 2
+    refinement steps, f = the analytic field plus 16 random modes; the arrival
+    rate is 75 % of the throughput a closed-loop burst of the same mix
+    reaches.  Reports p50/p99 latency, points/s and the worst Linf."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+
+    def draw():
+        if rng.random() < 0.5:
+            e = smooth_edge(int(round(math.exp(rng.uniform(math.log(32), math.log(512))))))
+            return (e, e, e)
+        return tuple(smooth_edge(int(round(math.exp(rng.uniform(math.log(32), math.log(512)))))) for _ in range(3))
+
+    def request(k):
+        n = draw()
+        prec = ff.FF_FP64 if k % 2 == 0 else ff.FF_FP32_REFINE
+        modes = random_modes(int(rng.integers(1 << 30)))
+        return n, prec, ff.field_set_modes(ff.field_default(n), modes)
+
+    srv = ff.Server(hbm_budget=48 << 30, streams=4, max_batch=8, batch_window_us=300.0)
+    # closed-loop burst: capacity in requests/s for this mix
+    burst = [request(k) for k in range(32)]
+    t0 = time.perf_counter()
+    ids = [srv.submit(n, prec, 2, field=fld) for n, prec, fld in burst]
+    for rid in ids:
+        srv.wait(rid, timeout=300.0)
+    cap = len(burst) / (time.perf_counter() - t0)
+    srv.close()
+    rate = 0.75 * cap
+    srv = ff.Server(hbm_budget=48 << 30, streams=4, max_batch=8, batch_window_us=300.0)
+    reqs = [request(k) for k in range(nreq)]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    t0 = time.perf_counter()
+    ids = []
+    for (n, prec, fld), ta in zip(reqs, arrivals):
+        dt = ta - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        ids.append((prec, srv.submit(n, prec, 2, field=fld, want_norms=True)))
+    worst = {ff.FF_FP64: 0.0, ff.FF_FP32_REFINE: 0.0}
+    for prec, rid in ids:
+        worst[prec] = max(worst[prec], srv.wait(rid, timeout=300.0)["norms"][2])
+    st = srv.stats()
+    srv.close()
+    res = {"requests": nreq, "arrival_rate_per_s": round(rate, 2), "burst_capacity_per_s": round(cap, 2),
+           "p50_ms": round(1e3 * st["p50_s"], 4), "p99_ms": round(1e3 * st["p99_s"], 4),
+           "points_per_s": st["points_per_s"], "groups": st["groups"], "arena_peak_bytes": st["arena_peak_bytes"],
+           "worst_linf_fp64": worst[ff.FF_FP64], "worst_linf_fp32_refine2": worst[ff.FF_FP32_REFINE]}
+    log(f"serving: {nreq} requests at {rate:.1f}/s (75% of {cap:.1f}/s): p50 {res['p50_ms']:.2f} ms, "
+        f"p99 {res['p99_ms']:.2f} ms, {st['points_per_s']:.4e} points/s, worst Linf {worst}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +274,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32+refinement Pareto")
     ap.add_argument("--cpu-size", type=int, default=512)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the other plain-method sizes, the batching and the serving workloads")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -237,6 +398,14 @@ def main():
         del f, phi
         torch.cuda.empty_cache()
 
+    extras = None
+    if not args.no_extras:
+        extras = {
+            "sizes_fp64": bench_sizes(ff, torch, [(512, 512, 512), (768, 768, 768), (1000, 1000, 1000)], 5),
+            "batching": bench_batching(ff, torch, 5),
+            "serving": bench_serving(ff),
+        }
+
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline (oracle) at {args.cpu_size}^3 ...")
@@ -267,6 +436,7 @@ def main():
         "passes": passes,
         "fp32_refine": fp32,
         "cpu_baseline": cpu,
+        "extras": extras,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -208,8 +208,11 @@ bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, i
   if (!plan_line(n, out.line, E)) return false;
   const int L = out.line.lp.L;
   if (T <= 0) {
-    // default: about 256 threads per workgroup, no wider than the columns
+    // default: about 256 threads per workgroup, widened until a tile row is
+    // at least 64 contiguous bytes (FP32 columns) while the workgroup fits;
+    // no wider than the columns
     T = floor_pow2(std::max(1, 256 / L));
+    while (T * elem_bytes < 64 && 2 * T * L <= max_threads_for(out.line.E)) T *= 2;
     int cap = 1;
     while (cap < ncols) cap *= 2;
     if (T > cap) T = cap;
