@@ -39,6 +39,7 @@ FF_FP64 = 0
 FF_FP32_REFINE = 1
 FF_PRESERVE_INPUT = 1
 FF_BLOCKED_LAYOUT = 2
+FF_SLAB_PIPELINE = 4
 FF_TUNE_SPLIT_LDS = 1
 FF_TUNE_TWO_GROUPS = 2
 FF_SOLVE_STAMPS = 14
@@ -57,7 +58,8 @@ class FishFryError(RuntimeError):
 
 
 class Options(ctypes.Structure):
-    _fields_ = [("precision", ctypes.c_int), ("refine_steps", ctypes.c_int), ("flags", ctypes.c_int)]
+    _fields_ = [("precision", ctypes.c_int), ("refine_steps", ctypes.c_int), ("flags", ctypes.c_int),
+                ("slab_planes", ctypes.c_int)]
 
 
 class Stamp(ctypes.Structure):
@@ -254,12 +256,12 @@ class ParisPencil:
 
     def __init__(self, n: Sequence[int], lo: Sequence[float], hi: Sequence[float],
                  m: Sequence[int] = (1, 1, 1), id: Sequence[int] = (0, 0, 0),
-                 precision: int = FF_FP64, refine_steps: int = 0, flags: int = 0):
+                 precision: int = FF_FP64, refine_steps: int = 0, flags: int = 0, slab_planes: int = 0):
         self.n = tuple(int(v) for v in n)
         self.precision = precision
         self.refine_steps = refine_steps
         h = ctypes.c_void_p()
-        opts = Options(precision, refine_steps, flags)
+        opts = Options(precision, refine_steps, flags, slab_planes)
         check(lib().ff_plan_create(_i3(n), _d3(lo), _d3(hi), _i3(m), _i3(id), ctypes.byref(opts), ctypes.byref(h)),
               "ParisPencil")
         self._h = h
@@ -502,5 +504,5 @@ from .driver import FishFry  # noqa: E402  (needs the names above)
 __all__ = [
     "ParisPencil", "FishFry", "Batch", "Server", "FF_BATCH_GRAPH", "FF_BATCH_PERSISTENT", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
     "field_norms", "reference_bytes", "benchmark_grid", "FishFryError", "lib", "LIB_PATH",
-    "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_BLOCKED_LAYOUT", "FF_TUNE_SPLIT_LDS", "FF_TUNE_TWO_GROUPS",
+    "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_BLOCKED_LAYOUT", "FF_SLAB_PIPELINE", "FF_TUNE_SPLIT_LDS", "FF_TUNE_TWO_GROUPS",
 ]

@@ -248,9 +248,35 @@ bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T) {
   return out.lds <= 160 * 1024;
 }
 
+// FP64 configurations measured fastest on MI355X for lengths where the
+// fewest-passes heuristic picks a slower elements-per-thread value (the
+// 768^3 and 1000^3 grids); user overrides (ff_tune_set) take precedence.
+struct BuiltinCfg {
+  int kind, n, E, T;
+};
+static const BuiltinCfg kBuiltin[] = {
+    {0, 384, 12, 4},  // z forward, nz = 768
+    {3, 384, 12, 4},  // z inverse, nz = 768
+    {1, 768, 24, 4},  // y, ny = 768
+    {2, 768, 12, 4},  // x, nx = 768
+    {1, 1000, 20, 4}, // y, ny = 1000
+};
+
+static bool builtin_lookup(int kind, int n, int prec, int& E, int& T) {
+  if (prec != 0) return false;
+  for (const BuiltinCfg& b : kBuiltin)
+    if (b.kind == kind && b.n == n) {
+      E = b.E;
+      T = b.T;
+      return true;
+    }
+  return false;
+}
+
 bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int kind, int prec) {
   int E = 0, T = 0, flags = 0;
   if (tune_lookup(kind, n, prec, E, T, flags) && col_cfg_with(n, ncols, elem_bytes, out, E, T, flags)) return true;
+  if (builtin_lookup(kind, n, prec, E, T) && col_cfg_with(n, ncols, elem_bytes, out, E, T)) return true;
   return col_cfg_with(n, ncols, elem_bytes, out, 0, 0);
 }
 
@@ -258,6 +284,7 @@ bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int kind, int prec) {
   const int M = (nz % 2 == 0) ? nz / 2 : nz;
   int E = 0, T = 0, flags = 0;
   if (tune_lookup(kind, M, prec, E, T, flags) && row_cfg_with(nz, elem_bytes, out, E, T)) return true;
+  if (builtin_lookup(kind, M, prec, E, T) && row_cfg_with(nz, elem_bytes, out, E, T)) return true;
   return row_cfg_with(nz, elem_bytes, out, 0, 0);
 }
 

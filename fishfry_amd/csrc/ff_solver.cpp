@@ -76,7 +76,7 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   if (o.precision != FF_FP64 && o.precision != FF_FP32_REFINE)
     return set_error(FF_ERR_INVALID, "options.precision must be FF_FP64 or FF_FP32_REFINE");
   if (o.refine_steps < 0 || o.refine_steps > 8) return set_error(FF_ERR_INVALID, "refine_steps must be 0..8");
-  if (o.flags & ~(FF_PRESERVE_INPUT | FF_BLOCKED_LAYOUT))
+  if (o.flags & ~(FF_PRESERVE_INPUT | FF_BLOCKED_LAYOUT | FF_SLAB_PIPELINE))
     return set_error(FF_ERR_INVALID, "unknown bits in options.flags");
 
   ff_plan* p = new (std::nothrow) ff_plan();
@@ -95,6 +95,12 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   p->bytes = reference_bytes(n, ones);
   // kh blocks of 8 complex doubles (one 128-byte line per block row)
   p->lgKT = (o.flags & FF_BLOCKED_LAYOUT) ? 3 : 0;
+  p->slab = 0;
+  if ((o.flags & FF_SLAB_PIPELINE) && !p->lgKT) {
+    const long long plane = (long long)n[1] * p->nh * 16;  // bytes of one x-plane of the spectrum
+    p->slab = o.slab_planes > 0 ? o.slab_planes : (int)std::max(1LL, (64LL << 20) / plane);
+    if (p->slab >= n[0]) p->slab = 0;  // one slab: plain passes
+  }
 
   ff_status st = hip_check(hipGetDevice(&p->device), "hipGetDevice");
   if (st != FF_OK) {
@@ -249,22 +255,34 @@ static RowArgs row_args(const ff_plan* p, int prec, bool inverse) {
   return a;
 }
 
-ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void* dst, hipStream_t s) {
+ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void* dst, hipStream_t s,
+                   long long row0, long long nrows) {
   RowArgs a = row_args(p, prec, false);
-  a.src = src;
-  a.dst = dst;
   const RowCfg& c = p->zr[prec];
+  if (nrows >= 0) {
+    a.nrows = nrows;
+    a.nblocks = (int)((nrows + c.T - 1) / c.T);
+    if (!nrows) return FF_OK;
+  }
+  a.src = static_cast<const unsigned char*>(src) + row0 * p->n[2] * (in_f32 ? 4 : 8);
+  a.dst = static_cast<unsigned char*>(dst) + row0 * p->nh * (prec ? 8 : 16);
   const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
   return hip_check(launch_r2c(c.line.E, prec, in_f32, p->n[2] % 2, l, a), "z forward launch");
 }
 
 ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void* src, void* dst, const double* f,
-                   const double* shiftp, hipStream_t s, int lay_in) {
+                   const double* shiftp, hipStream_t s, int lay_in, long long row0, long long nrows) {
   RowArgs a = row_args(p, prec, true);
   a.spec = spec_side(p, 1, lay_in);
-  a.src = src;
-  a.dst = dst;
-  a.aux = f;
+  if (nrows >= 0) {
+    if (lay_in != LAY_PLAIN) return set_error(FF_ERR_INVALID, "row ranges need the plain spectrum layout");
+    a.nrows = nrows;
+    a.nblocks = (int)((nrows + p->zi[prec].T - 1) / p->zi[prec].T);
+    if (!nrows) return FF_OK;
+  }
+  a.src = static_cast<const unsigned char*>(src) + row0 * p->nh * (prec ? 8 : 16);
+  a.dst = static_cast<unsigned char*>(dst) + row0 * p->n[2] * (out_kind ? 4 : 8);
+  a.aux = f ? f + row0 * p->n[2] : nullptr;
   if (shiftp) {
     a.shiftp = shiftp;
     a.shift = a.scale;  // shift = (sum of phi) / N
@@ -275,7 +293,7 @@ ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void
 }
 
 ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* src, void* dst, hipStream_t s,
-                  double* dc_out, int lay_in, int lay_out) {
+                  double* dc_out, int lay_in, int lay_out, int outer0, int nouter) {
   const ColCfg& c = axis == 1 ? p->cy[prec] : p->cx[prec];
   if (src == dst && lay_in != lay_out) return set_error(FF_ERR_INVALID, "in-place column pass must keep its layout");
   ColArgs a;
@@ -288,6 +306,16 @@ ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* sr
   a.nouter = axis == 1 ? p->n[0] : p->n[1];
   a.in = spec_side(p, axis, lay_in);
   a.out = spec_side(p, axis, lay_out);
+  if (nouter >= 0) {
+    // planes [outer0, outer0 + nouter) of a y pass over the plain layout
+    if (axis != 1 || lay_in != LAY_PLAIN || lay_out != LAY_PLAIN)
+      return set_error(FF_ERR_INVALID, "plane ranges need a y pass over the plain spectrum layout");
+    if (!nouter) return FF_OK;
+    const long long off = (long long)outer0 * p->n[1] * p->nh * (prec ? 8 : 16);
+    a.src = static_cast<const unsigned char*>(src) + off;
+    a.dst = static_cast<unsigned char*>(dst) + off;
+    a.nouter = nouter;
+  }
   a.outer_fast = 0;
   a.T = c.T;
   a.lgT = c.lgT;
@@ -346,6 +374,36 @@ ff_status solve_enqueue(const ff_plan* p, double* density, double* potential, hi
   void* w = p->work ? p->work : (void*)density;
   int li[5], lo[5];
   for (int k = 0; k < 5; ++k) pass_layouts(p, k, li[k], lo[k]);
+  if (p->slab) {
+    // Slab pipeline: each pair of passes runs slab by slab, so the second
+    // pass reads the slab the first just wrote while it is cache-resident.
+    // The spectrum of a plane (ny*nh complex) is larger than its real data
+    // (ny*nz doubles), so when w aliases density the forward pair walks the
+    // slabs from the top (a slab's y output never reaches real planes not yet
+    // read) and the inverse pair from the bottom (a slab's real output never
+    // reaches spectrum planes not yet read).
+    const int nx = p->n[0], S = p->slab;
+    const long long ny = p->n[1];
+    const int nsl = (nx + S - 1) / S;
+    for (int k = nsl - 1; k >= 0; --k) {
+      const int i0 = k * S, ni = std::min(S, nx - i0);
+      if ((st = run_zfwd(p, 0, 0, density, potential, s, i0 * ny, ni * ny))) return st;
+      if ((st = run_col(p, 0, 1, COL_FWD, potential, w, s, nullptr, LAY_PLAIN, LAY_PLAIN, i0, ni))) return st;
+    }
+    if (ev && (st = hip_check(hipEventRecord(ev[0], s), "event"))) return st;
+    if (ev && (st = hip_check(hipEventRecord(ev[1], s), "event"))) return st;
+    if ((st = run_col(p, 0, 0, COL_FILTER, w, potential, s))) return st;
+    if (ev && (st = hip_check(hipEventRecord(ev[2], s), "event"))) return st;
+    for (int k = 0; k < nsl; ++k) {
+      const int i0 = k * S, ni = std::min(S, nx - i0);
+      if ((st = run_col(p, 0, 1, COL_INV, potential, w, s, nullptr, LAY_PLAIN, LAY_PLAIN, i0, ni))) return st;
+      if ((st = run_zinv(p, 0, 0, EPI_STORE, w, potential, nullptr, nullptr, s, LAY_PLAIN, i0 * ny, ni * ny)))
+        return st;
+    }
+    if (ev && (st = hip_check(hipEventRecord(ev[3], s), "event"))) return st;
+    if (ev && (st = hip_check(hipEventRecord(ev[4], s), "event"))) return st;
+    return FF_OK;
+  }
   if ((st = run_zfwd(p, 0, 0, density, potential, s))) return st;
   if (ev && (st = hip_check(hipEventRecord(ev[0], s), "event"))) return st;
   if ((st = run_col(p, 0, 1, COL_FWD, potential, w, s, nullptr, li[1], lo[1]))) return st;

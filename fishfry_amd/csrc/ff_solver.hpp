@@ -8,6 +8,7 @@ struct ff_plan {
   double lo[3], hi[3];
   int nh;
   int lgKT;      // FP64 spectrum between the y passes: 0 plain, else kh blocks of 2^lgKT
+  int slab;      // FF_SLAB_PIPELINE: x-planes per slab (0: whole passes)
   long long N;   // nx*ny*nz
   long long Nh;  // nx*ny*nh
   double ddi, ddj, ddk;
@@ -38,14 +39,20 @@ struct ff_plan {
 namespace ff {
 
 // Pass kernels over caller-provided buffers.  prec: 0 FP64, 1 FP32.
-ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void* dst, hipStream_t s);
+// Row passes over rows [row0, row0 + nrows) of i*ny + j (nrows < 0: all rows),
+// column passes over planes [outer0, outer0 + nouter) (nouter < 0: all);
+// ranges apply to the plain spectrum layout.
+ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void* dst, hipStream_t s,
+                   long long row0 = 0, long long nrows = -1);
 // Half-spectrum layouts (see SpecSide): plain [i][j][kh], y-blocked
 // [kb][i][j][kt], x-blocked [kb][j][i][kt].
 enum { LAY_PLAIN = 0, LAY_Y = 1, LAY_X = 2 };
 ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* src, void* dst, hipStream_t s,
-                  double* dc_out = nullptr, int lay_in = LAY_PLAIN, int lay_out = LAY_PLAIN);
+                  double* dc_out = nullptr, int lay_in = LAY_PLAIN, int lay_out = LAY_PLAIN, int outer0 = 0,
+                  int nouter = -1);
 ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void* src, void* dst, const double* f,
-                   const double* shiftp, hipStream_t s, int lay_in = LAY_PLAIN);
+                   const double* shiftp, hipStream_t s, int lay_in = LAY_PLAIN, long long row0 = 0,
+                   long long nrows = -1);
 // Layouts the FP64 solve's column passes read and write: y forward, x, y inverse
 // (and the z inverse reads what the y inverse wrote).
 void pass_layouts(const ff_plan* p, int pass, int& lay_in, int& lay_out);

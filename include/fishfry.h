@@ -54,10 +54,18 @@ typedef enum ff_precision {
                                default: the reference's [i][j][kh] throughout
                                (ff_plan_describe "kt": 0 or KT)                       */
 
+#define FF_SLAB_PIPELINE 4  /* FP64: run the z forward / y forward pair, and later the
+                               y inverse / z inverse pair, one slab of x-planes at a time
+                               so the second pass of each pair reads what the first
+                               just wrote while it is still in the 256 MB Infinity
+                               Cache (plain spectrum layout only)                      */
+
 typedef struct ff_options {
   int precision;    /* ff_precision                                             */
   int refine_steps; /* FF_FP32_REFINE only: 0..8                                */
-  int flags;        /* 0 or FF_PRESERVE_INPUT | FF_BLOCKED_LAYOUT               */
+  int flags;        /* 0 or FF_PRESERVE_INPUT | FF_BLOCKED_LAYOUT | FF_SLAB_PIPELINE */
+  int slab_planes;  /* FF_SLAB_PIPELINE: x-planes per slab (0: about 64 MB of
+                       half spectrum per slab)                                  */
 } ff_options;
 
 /* One phase time stamp: the reference's TimeStamp (TimeStamp.hpp:7), a
