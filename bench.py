@@ -110,21 +110,6 @@ def cpu_baseline(size, budget_s=25.0):
     }
 
 
-def smooth_edge(e):
-    """Nearest integer >= 2 whose only prime factors are 2, 3 and 5 (the
-    transform lengths the passes support)."""
-    def ok(m):
-        for p in (2, 3, 5):
-            while m % p == 0:
-                m //= p
-        return m == 1
-    for d in range(64):
-        for c in (e - d, e + d):
-            if c >= 2 and ok(c):
-                return c
-    return e
-
-
 def bench_sizes(ff, torch, sizes, K):
     """FP64 solves of the analytic f at other sizes of the plain-method
     configuration: ms per solve, points/s and Linf."""
@@ -211,7 +196,7 @@ def bench_batching(ff, torch, K):
 
 def bench_serving(ff, nreq=96, seed=20261202):
     """The serving configuration: Poisson arrivals, edges log-uniform in
-    [32, 512] rounded to 2^a 3^b 5^c (half cubic), half FP64 and half FP32 + 2
+    [32, 512] (any integer; half cubic), half FP64 and half FP32 + 2
     refinement steps, f = the analytic field plus 16 random modes; the arrival
     rate is 75 % of the throughput a closed-loop burst of the same mix
     reaches.  Reports p50/p99 latency, points/s and the worst Linf."""
@@ -221,9 +206,9 @@ def bench_serving(ff, nreq=96, seed=20261202):
 
     def draw():
         if rng.random() < 0.5:
-            e = smooth_edge(int(round(math.exp(rng.uniform(math.log(32), math.log(512))))))
+            e = int(round(math.exp(rng.uniform(math.log(32), math.log(512)))))
             return (e, e, e)
-        return tuple(smooth_edge(int(round(math.exp(rng.uniform(math.log(32), math.log(512)))))) for _ in range(3))
+        return tuple(int(round(math.exp(rng.uniform(math.log(32), math.log(512))))) for _ in range(3))
 
     def request(k):
         n = draw()

@@ -30,14 +30,24 @@ std::mutex& scratch_mutex();
 constexpr int kEValues[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 15, 16, 20, 24, 30};
 
 struct LineCfg {
-  int n = 0;
+  int n = 0;      // transform length
   int E = 0;
-  LinePlan lp{};
+  LinePlan lp{};  // engine transform: length n, or for Bluestein the 2^a 3^b 5^c length m >= 2n - 1
+  bool bluestein() const { return lp.n != n; }
 };
 
-// Chooses E and a radix sequence for a length-n transform; false when n has a
-// prime factor above 5 or needs more than FF_MAX_THREADS threads per line.
+// Chooses E and a radix sequence for a length-n transform.  A length with a
+// prime factor above 5 is transformed by Bluestein's algorithm over the
+// smallest 2^a 3^b 5^c length m >= 2n - 1 that has a plan (forceE then
+// applies to m).  False when no plan fits FF_MAX_THREADS threads per line.
 bool plan_line(int n, LineCfg& out, int forceE = 0);
+
+// Bluestein tables of a length-n transform computed over length m, in
+// precision prec (0 FP64, 1 FP32), appended to buf at 256-byte aligned
+// offsets: the chirp w[t] = exp(-pi i t^2/n) (t < n) and, for each direction,
+// the length-m spectrum of the convolution kernel divided by m (forward:
+// b[t] = conj(w[|t|]); inverse: b[t] = w[|t|]; |t| < n, indices mod m).
+void push_bluestein(std::vector<unsigned char>& buf, int n, int m, int prec, size_t& chirp, size_t& bsf, size_t& bsi);
 
 // exp(-2 pi i t / n) in long double with exact integer argument reduction.
 void twiddle_ld(long long t, long long n, long double& c, long double& s);

@@ -73,11 +73,58 @@ struct ColArgs {
   double* dc_out;  // FILTER / SYMBOL: receives the forward (0,0,0) term (sum of the input)
   const void* tw;
   LinePlan lp;
+  int nline;                     // transform length (lp.n: engine length; larger for Bluestein)
+  const void *chirp, *bsf, *bsi;  // Bluestein tables (see line_dft)
 };
 
 // MODE: 0 forward, 1 inverse, 2 forward * filter * inverse (the solve),
 //       3 forward * symbol * inverse (applies L_h for a residual).
 enum { COL_FWD = 0, COL_INV = 1, COL_FILTER = 2, COL_SYMBOL = 3 };
+
+// Unnormalized DFT of direction DIR (-1 forward, +1 inverse) of a line of
+// length nline held at positions r0 + i*L.  When nline equals the engine
+// length lp.n this is the Stockham transform itself.  Otherwise (a length
+// with a prime factor above 5) it is Bluestein's algorithm over the engine
+// length m = lp.n >= 2*nline - 1: with w[t] = exp(-pi i t^2/n),
+//   X[k] = w[k]^s * sum_j (x[j] w[j]^s) conj(w[k-j])^s,   s = 1 (forward) or conj (inverse),
+// the sum being a length-m circular convolution done as forward transform,
+// product with the kernel spectrum (bsf / bsi, already divided by m) and
+// inverse transform.  Positions >= nline are zero on entry to the
+// convolution and undefined on exit.
+template <int E, int DIR, typename F, typename Idx, bool SPLIT = false>
+__device__ __forceinline__ void line_dft(cx<F> (&v)[E], const LinePlan& lp, int nline, const cx<F>* __restrict__ tw,
+                                         const void* chirp_p, const void* bsf_p, const void* bsi_p, cx<F>* lds,
+                                         const Idx& idx, int r0) {
+  if (nline == lp.n) {
+    fft_line<E, DIR, F, Idx, SPLIT>(v, lp, tw, lds, idx, r0);
+    return;
+  }
+  const cx<F>* __restrict__ chirp = reinterpret_cast<const cx<F>*>(chirp_p);
+  const cx<F>* __restrict__ bs = reinterpret_cast<const cx<F>*>(DIR < 0 ? bsf_p : bsi_p);
+  const int L = lp.L;
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int r = r0 + i * L;
+    if (r < nline) {
+      const cx<F> w = chirp[r];
+      v[i] = twmul<DIR>(v[i], w);
+    } else {
+      v[i] = cx<F>{F(0), F(0)};
+    }
+  }
+  fft_line<E, -1, F, Idx, SPLIT>(v, lp, tw, lds, idx, r0);
+#pragma unroll
+  for (int i = 0; i < E; ++i) v[i] = twmul<-1>(v[i], bs[r0 + i * L]);
+  fft_line<E, +1, F, Idx, SPLIT>(v, lp, tw, lds, idx, r0);
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int r = r0 + i * L;
+    if (r < nline) {
+      const cx<F> w = chirp[r];
+      v[i] = twmul<DIR>(v[i], w);
+    }
+  }
+}
 
 // One column line held in registers through the transform, filter or symbol
 // and the inverse transform of MODE.
@@ -86,11 +133,11 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
                                          const ColIdx& idx, int r0, int outer, int col, bool valid) {
   const int L = a.lp.L;
   if constexpr (MODE == COL_FWD) {
-    fft_line<E, -1, F, ColIdx, SPLIT>(v, a.lp, tw, lds, idx, r0);
+    line_dft<E, -1, F, ColIdx, SPLIT>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   } else if constexpr (MODE == COL_INV) {
-    fft_line<E, +1, F, ColIdx, SPLIT>(v, a.lp, tw, lds, idx, r0);
+    line_dft<E, +1, F, ColIdx, SPLIT>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   } else {
-    fft_line<E, -1, F, ColIdx, SPLIT>(v, a.lp, tw, lds, idx, r0);
+    line_dft<E, -1, F, ColIdx, SPLIT>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
     // spectral filter of ParisPencil.cpp:269-284 at (i, j, kh) = (r, outer, col)
     const int jj = outer, kk = col;
     const double j2 = sqr(double(jj < a.ny - jj ? jj : a.ny - jj) * a.ddj);
@@ -107,7 +154,7 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
       const F df = F(d);
       v[i] = cx<F>{df * v[i].x, df * v[i].y};
     }
-    fft_line<E, +1, F, ColIdx, SPLIT>(v, a.lp, tw, lds, idx, r0);
+    line_dft<E, +1, F, ColIdx, SPLIT>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   }
 }
 
@@ -146,8 +193,13 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
     long long ib, is;
     side_addr(a.in, outer, col < a.ncols ? col : a.ncols - 1, r0, L, ib, is);
     const cx<F>* src = reinterpret_cast<const cx<F>*>(a.src) + ib;
+    if (a.nline == a.lp.n) {
 #pragma unroll
-    for (int i = 0; i < E; ++i) v[g][i] = src[(long long)i * is];
+      for (int i = 0; i < E; ++i) v[g][i] = src[(long long)i * is];
+    } else {  // Bluestein: the engine line is longer than the data line
+#pragma unroll
+      for (int i = 0; i < E; ++i) v[g][i] = r0 + i * L < a.nline ? src[(long long)i * is] : cx<F>{F(0), F(0)};
+    }
   }
   const ColIdx idx{a.T, c, a.ps};
 #pragma unroll
@@ -163,7 +215,8 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
       side_addr(a.out, outer, col, r0, L, ob, os);
       cx<F>* dst = reinterpret_cast<cx<F>*>(a.dst) + ob;
 #pragma unroll
-      for (int i = 0; i < E; ++i) dst[(long long)i * os] = v[g][i];
+      for (int i = 0; i < E; ++i)
+        if (a.nline == a.lp.n || r0 + i * L < a.nline) dst[(long long)i * os] = v[g][i];
     }
   }
 }
@@ -182,9 +235,11 @@ struct RowArgs {
   double scale;   // 1/(nx*ny*nz)
   double shift;   // subtracted by the accumulate epilogue (times *shiftp if set)
   const double* shiftp;
-  const void* tw;      // W_n of the transform length
+  const void* tw;      // W_n of the engine transform length
   const void* twpost;  // exp(-2 pi i k / nz), k < nz/2 (even nz)
   LinePlan lp;
+  int nline;                     // transform length M (lp.n: engine length; larger for Bluestein)
+  const void *chirp, *bsf, *bsi;  // Bluestein tables (see line_dft)
 };
 
 template <typename T>
@@ -207,7 +262,8 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
   const long long row = (long long)xcd_remap(blockIdx.x, a.nblocks) * a.T + line;
   const bool valid = row < a.nrows;
   const long long rowc = valid ? row : a.nrows - 1;  // loads stay in bounds; stores are predicated
-  const int M = a.lp.n;
+  const int M = a.nline;                  // transform length (the engine's lp.n is longer for Bluestein)
+  const bool blue = a.nline != a.lp.n;    // positions >= M exist only for Bluestein
   const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
   const RowIdx idx{line * a.pitch, a.ps};
   cx<F> v[E];
@@ -216,15 +272,19 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int k = r0 + i * L;
-      if constexpr (sizeof(IN) == 8) {
-        const cx<double> p = reinterpret_cast<const cx<double>*>(src)[k];
-        v[i] = cx<F>{F(p.x), F(p.y)};
+      if (!blue || k < M) {
+        if constexpr (sizeof(IN) == 8) {
+          const cx<double> p = reinterpret_cast<const cx<double>*>(src)[k];
+          v[i] = cx<F>{F(p.x), F(p.y)};
+        } else {
+          const cx<float> p = reinterpret_cast<const cx<float>*>(src)[k];
+          v[i] = cx<F>{F(p.x), F(p.y)};
+        }
       } else {
-        const cx<float> p = reinterpret_cast<const cx<float>*>(src)[k];
-        v[i] = cx<F>{F(p.x), F(p.y)};
+        v[i] = cx<F>{F(0), F(0)};
       }
     }
-    fft_line<E, -1, F>(v, a.lp, tw, lds, idx, r0);
+    line_dft<E, -1, F>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
     __syncthreads();
     put_natural<E, F>(v, lds, idx, r0, L);
     __syncthreads();
@@ -234,6 +294,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const int k = r0 + i * L;
+        if (blue && k >= M) continue;
         const cx<F> zk = lds[idx(k)];
         const cx<F> zm = conj(lds[idx(k ? M - k : 0)]);
         // even spectrum (zk + zm)/2, odd spectrum (zk - zm)/(2i)
@@ -249,8 +310,11 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
   } else {
     const IN* __restrict__ src = reinterpret_cast<const IN*>(a.src) + rowc * a.nz;
 #pragma unroll
-    for (int i = 0; i < E; ++i) v[i] = cx<F>{F(src[r0 + i * L]), F(0)};
-    fft_line<E, -1, F>(v, a.lp, tw, lds, idx, r0);
+    for (int i = 0; i < E; ++i) {
+      const int k = r0 + i * L;
+      v[i] = cx<F>{(!blue || k < M) ? F(src[k]) : F(0), F(0)};
+    }
+    line_dft<E, -1, F>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
     cx<F>* __restrict__ dst = reinterpret_cast<cx<F>*>(a.dst) + (valid ? spec_row(a, row) : 0);
     if (valid) {
 #pragma unroll
@@ -307,7 +371,8 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
   const int r0 = tid - line * L;
   const long long row = (long long)xcd_remap(blockIdx.x, a.nblocks) * a.T + line;
   const bool valid = row < a.nrows;
-  const int M = a.lp.n;
+  const int M = a.nline;                // transform length (the engine's lp.n is longer for Bluestein)
+  const bool blue = a.nline != a.lp.n;  // positions >= M exist only for Bluestein
   const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
   const RowIdx idx{line * a.pitch, a.ps};
   // stage the half spectrum of the workgroup's rows in LDS; rows past the end
@@ -317,7 +382,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int k = r0 + i * L;
-      if (!ODD || k < a.nh) lds[idx(k)] = src[k];
+      if (k < a.nh) lds[idx(k)] = src[k];
     }
     if (!ODD && r0 == 0) lds[idx(M)] = src[M];
   } else {
@@ -353,6 +418,10 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int k = r0 + i * L;
+      if (blue && k >= M) {
+        v[i] = cx<F>{F(0), F(0)};
+        continue;
+      }
       cx<F> xk = lds[idx(k)];
       cx<F> xm = lds[idx(M - k)];
       if (k == 0) {
@@ -368,12 +437,16 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int k = r0 + i * L;
+      if (blue && k >= M) {
+        v[i] = cx<F>{F(0), F(0)};
+        continue;
+      }
       cx<F> y = k < a.nh ? lds[idx(k)] : conj(lds[idx(a.nz - k)]);
       if (k == 0) y.y = F(0);
       v[i] = y;
     }
   }
-  fft_line<E, +1, F>(v, a.lp, tw, lds, idx, r0);
+  line_dft<E, +1, F>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   if (valid) {
     const double shift = a.shiftp ? a.shift * *a.shiftp : a.shift;
     OUT* out = reinterpret_cast<OUT*>(a.dst) + row * a.nz;
@@ -381,6 +454,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int r = r0 + i * L;
+      if (blue && r >= M) continue;
       if constexpr (!ODD) {
         c2r_out2<EPI, OUT>(out, f, r, to_d(v[i].x), to_d(v[i].y), a.scale, shift);
       } else {

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <complex>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -95,6 +96,69 @@ void twiddle_ld(long long t, long long n, long double& c, long double& s) {
   s = -ss;
 }
 
+// ---- Bluestein tables ---------------------------------------------------------
+
+// In-place long-double DFT of a 2^a 3^b 5^c length by recursive
+// decimation in time (exact twiddles from twiddle_ld).
+static void fft_ld(std::vector<std::complex<long double>>& a, int sign) {
+  const long long n = (long long)a.size();
+  if (n == 1) return;
+  const int p = n % 2 == 0 ? 2 : (n % 3 == 0 ? 3 : 5);
+  const long long q = n / p;
+  std::vector<std::vector<std::complex<long double>>> sub(p, std::vector<std::complex<long double>>(q));
+  for (long long j = 0; j < q; ++j)
+    for (int r = 0; r < p; ++r) sub[r][j] = a[j * p + r];
+  for (int r = 0; r < p; ++r) fft_ld(sub[r], sign);
+  for (long long k = 0; k < n; ++k) {
+    std::complex<long double> acc = 0;
+    for (int r = 0; r < p; ++r) {
+      long double c, s;
+      twiddle_ld((long long)r * k, n, c, s);  // exp(-2 pi i r k/n)
+      acc += sub[r][k % q] * std::complex<long double>(c, sign < 0 ? s : -s);
+    }
+    a[k] = acc;
+  }
+}
+
+static size_t push_cx(std::vector<unsigned char>& buf, const std::vector<std::complex<long double>>& v, int prec) {
+  const size_t elem = prec ? 8 : 16;
+  const size_t off = (buf.size() + 255) / 256 * 256;
+  buf.resize(off + v.size() * elem);
+  for (size_t t = 0; t < v.size(); ++t) {
+    if (prec) {
+      const float w[2] = {(float)v[t].real(), (float)v[t].imag()};
+      memcpy(&buf[off + t * elem], w, sizeof w);
+    } else {
+      const double w[2] = {(double)v[t].real(), (double)v[t].imag()};
+      memcpy(&buf[off + t * elem], w, sizeof w);
+    }
+  }
+  return off;
+}
+
+void push_bluestein(std::vector<unsigned char>& buf, int n, int m, int prec, size_t& chirp, size_t& bsf,
+                    size_t& bsi) {
+  // w[t] = exp(-pi i t^2/n) = W_{2n}^(t^2 mod 2n), with the exact reduction
+  std::vector<std::complex<long double>> w(n);
+  for (long long t = 0; t < n; ++t) {
+    long double c, s;
+    twiddle_ld((t * t) % (2LL * n), 2LL * n, c, s);
+    w[t] = {c, s};
+  }
+  chirp = push_cx(buf, w, prec);
+  for (int dir = 0; dir < 2; ++dir) {
+    std::vector<std::complex<long double>> b(m, 0);
+    for (int t = 0; t < n; ++t) {
+      const std::complex<long double> v = dir == 0 ? std::conj(w[t]) : w[t];
+      b[t] = v;
+      if (t) b[m - t] = v;
+    }
+    fft_ld(b, -1);
+    for (auto& x : b) x /= (long double)m;
+    (dir == 0 ? bsf : bsi) = push_cx(buf, b, prec);
+  }
+}
+
 // ---- radix plans ------------------------------------------------------------
 
 static const int kRadices[] = {16, 12, 10, 8, 6, 5, 4, 3, 2};
@@ -151,8 +215,29 @@ static void fill_plan(int n, int E, const std::vector<int>& seq, LineCfg& out) {
   }
 }
 
+static bool plan_smooth(int n, LineCfg& out, int forceE);
+
+static bool smooth235(long long m) {
+  for (int p : {2, 3, 5})
+    while (m % p == 0) m /= p;
+  return m == 1;
+}
+
 bool plan_line(int n, LineCfg& out, int forceE) {
   if (n < 1) return false;
+  if (plan_smooth(n, out, forceE)) return true;
+  if (smooth235(n)) return false;  // a 2^a 3^b 5^c length too long for one workgroup
+  for (long long m = 2LL * n - 1; m <= 4LL * n; ++m) {
+    if (!smooth235(m)) continue;
+    if (plan_smooth((int)m, out, forceE)) {
+      out.n = n;
+      return true;
+    }
+  }
+  return false;
+}
+
+static bool plan_smooth(int n, LineCfg& out, int forceE) {
   if (n == 1) {
     fill_plan(1, 1, {}, out);
     return true;
@@ -226,7 +311,8 @@ bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, i
   out.split = (flags & FF_TUNE_SPLIT_LDS) ? 1 : 0;
   out.groups = (flags & FF_TUNE_TWO_GROUPS) ? 2 : 1;
   if (out.groups == 2 && out.line.E != 8 && out.line.E != 16) return false;
-  const size_t elems = (size_t)n * T + ((size_t)(n - 1) >> out.ps) + 1;
+  const size_t ne = (size_t)out.line.lp.n;  // engine length (Bluestein: m >= 2n - 1)
+  const size_t elems = ne * T + ((ne - 1) >> out.ps) + 1;
   out.lds = elems * (size_t)(out.split ? elem_bytes / 2 : elem_bytes);
   return out.lds <= 160 * 1024;
 }
@@ -241,7 +327,14 @@ bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T) {
   if (T < 1 || T * L > max_threads_for(out.line.E)) return false;
   out.T = T;
   out.ps = 4;
-  out.pitch = (M + 1) + (M >> out.ps) + 1;
+  // a line holds the staged half spectrum (M + 1 entries) and the engine
+  // transform (length M, or m >= 2M - 1 for Bluestein)
+  if (out.line.bluestein()) {
+    const int span = out.line.lp.n;
+    out.pitch = span + (span >> out.ps) + 1;
+  } else {
+    out.pitch = (M + 1) + (M >> out.ps) + 1;
+  }
   out.threads = T * L;
   out.Linv = L > 1 ? (uint32_t)(((1ull << 32) + (uint64_t)L - 1) / (uint64_t)L) : 0u;
   out.lds = (size_t)T * out.pitch * elem_bytes;

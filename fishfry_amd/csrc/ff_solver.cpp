@@ -115,24 +115,35 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
       delete p;
       char buf[160];
       snprintf(buf, sizeof buf,
-               "no transform plan for %dx%dx%d: every extent (nz/2 for even nz) must be 2^a 3^b 5^c and "
-               "<= 4096",
+               "no transform plan for %dx%dx%d: an extent (nz/2 for even nz) is too long for one "
+               "workgroup's line transform",
                n[0], n[1], n[2]);
       return set_error(FF_ERR_UNSUPPORTED, buf);
     }
   }
 
-  // twiddle tables
+  // twiddle tables of the engine transforms (length m for Bluestein lines),
+  // the even-nz post-twiddles and the Bluestein tables
   std::vector<unsigned char> buf;
-  size_t oz[2], ozp[2], oy[2], ox[2];
+  size_t oz[2], ozi[2], ozp[2], oy[2], ox[2];
+  size_t bz[2][3], bzi[2][3], by[2][3], bx[2][3];
   const int nz = n[2];
   const bool odd = nz % 2;
+  auto blue = [&](const LineCfg& c, int prec, size_t (&o)[3]) {
+    o[0] = o[1] = o[2] = (size_t)-1;
+    if (c.bluestein()) push_bluestein(buf, c.n, c.lp.n, prec, o[0], o[1], o[2]);
+  };
   for (int prec = 0; prec < nprec; ++prec) {
     const long long M = odd ? nz : nz / 2;
-    push_table(buf, M, M, prec, oz[prec]);
+    push_table(buf, p->zr[prec].line.lp.n, p->zr[prec].line.lp.n, prec, oz[prec]);
+    push_table(buf, p->zi[prec].line.lp.n, p->zi[prec].line.lp.n, prec, ozi[prec]);
     push_table(buf, nz, odd ? 1 : M, prec, ozp[prec]);
-    push_table(buf, n[1], n[1], prec, oy[prec]);
-    push_table(buf, n[0], n[0], prec, ox[prec]);
+    push_table(buf, p->cy[prec].line.lp.n, p->cy[prec].line.lp.n, prec, oy[prec]);
+    push_table(buf, p->cx[prec].line.lp.n, p->cx[prec].line.lp.n, prec, ox[prec]);
+    blue(p->zr[prec].line, prec, bz[prec]);
+    blue(p->zi[prec].line, prec, bzi[prec]);
+    blue(p->cy[prec].line, prec, by[prec]);
+    blue(p->cx[prec].line, prec, bx[prec]);
   }
   st = hip_check(hipMalloc(&p->tw_mem, buf.size()), "twiddle hipMalloc");
   if (st == FF_OK) st = hip_check(hipMemcpy(p->tw_mem, buf.data(), buf.size(), hipMemcpyHostToDevice), "twiddle copy");
@@ -143,9 +154,16 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   for (int prec = 0; prec < nprec; ++prec) {
     unsigned char* b = static_cast<unsigned char*>(p->tw_mem);
     p->tw_z[prec] = b + oz[prec];
+    p->tw_zi[prec] = b + ozi[prec];
     p->tw_zpost[prec] = b + ozp[prec];
     p->tw_y[prec] = b + oy[prec];
     p->tw_x[prec] = b + ox[prec];
+    for (int k = 0; k < 3; ++k) {
+      p->bl_z[prec][k] = bz[prec][k] == (size_t)-1 ? nullptr : b + bz[prec][k];
+      p->bl_zi[prec][k] = bzi[prec][k] == (size_t)-1 ? nullptr : b + bzi[prec][k];
+      p->bl_y[prec][k] = by[prec][k] == (size_t)-1 ? nullptr : b + by[prec][k];
+      p->bl_x[prec][k] = bx[prec][k] == (size_t)-1 ? nullptr : b + bx[prec][k];
+    }
   }
   if (o.precision == FF_FP32_REFINE) {
     const size_t spec = (size_t)p->Nh * 16;
@@ -249,9 +267,14 @@ static RowArgs row_args(const ff_plan* p, int prec, bool inverse) {
   a.scale = 1.0 / (double(p->n[0]) * double(p->n[1]) * double(p->n[2]));  // ParisPencil.cpp:408
   a.shift = 0.0;
   a.shiftp = nullptr;
-  a.tw = p->tw_z[prec];
+  a.tw = inverse ? p->tw_zi[prec] : p->tw_z[prec];
   a.twpost = p->tw_zpost[prec];
   a.lp = c.line.lp;
+  a.nline = c.line.n;
+  const void* const* bl = inverse ? p->bl_zi[prec] : p->bl_z[prec];
+  a.chirp = bl[0];
+  a.bsf = bl[1];
+  a.bsi = bl[2];
   return a;
 }
 
@@ -332,6 +355,11 @@ ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* sr
   a.dc_out = dc_out;
   a.tw = axis == 1 ? p->tw_y[prec] : p->tw_x[prec];
   a.lp = c.line.lp;
+  a.nline = c.line.n;
+  const void* const* bl = axis == 1 ? p->bl_y[prec] : p->bl_x[prec];
+  a.chirp = bl[0];
+  a.bsf = bl[1];
+  a.bsi = bl[2];
   const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
   return hip_check(launch_col(c.line.E, prec, mode, c.split, c.groups, l, a), "column pass launch");
 }

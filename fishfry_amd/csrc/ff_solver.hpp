@@ -19,10 +19,17 @@ struct ff_plan {
   ff::RowCfg zr[2];  // z forward rows
   ff::RowCfg zi[2];  // z inverse rows
   ff::ColCfg cy[2], cx[2];
-  const void* tw_z[2];
+  const void* tw_z[2];   // z forward engine twiddles
+  const void* tw_zi[2];  // z inverse engine twiddles (its E, hence m, may differ)
   const void* tw_zpost[2];
   const void* tw_y[2];
   const void* tw_x[2];
+  // Bluestein tables (chirp, forward and inverse kernel spectra) of the z, y
+  // and x line transforms; null when the line length has a direct plan
+  const void* bl_z[2][3];
+  const void* bl_zi[2][3];
+  const void* bl_y[2][3];
+  const void* bl_x[2][3];
   void* tw_mem;
   // FP32 + refinement work area: FP64 spectrum (also hosts the FP32 one),
   // the FP32 residual and one double for the mean of phi.

@@ -169,7 +169,7 @@ ff_status ff_solve_pass(const ff_plan* plan, int pass, const double* src, double
 /* ParisPencil::solve's contract and numerics.                              */
 
 typedef struct ff_batch_item {
-  int n[3];         /* cells per axis (each >= 2; 2^a 3^b 5^c, nz/2 for even nz) */
+  int n[3];         /* cells per axis (each >= 2)                                */
   double lo[3];     /* physical bounds as in ff_plan_create                       */
   double hi[3];
   int precision;    /* ff_precision                                               */

@@ -107,7 +107,9 @@ def gpu_solve(f, **kw):
 
 
 PASS_SHAPES = [(16, 24, 20), (32, 32, 32), (12, 10, 9), (64, 48, 40), (5, 6, 4), (9, 8, 12), (2, 8, 16),
-               (30, 25, 27), (32, 1024, 512), (24, 600, 1000)]
+               (30, 25, 27), (32, 1024, 512), (24, 600, 1000),
+               # prime factors above 5 (Bluestein lines): odd and even nz, nz/2 prime
+               (7, 11, 13), (14, 22, 26), (37, 20, 18), (12, 10, 14), (13, 509, 8), (4, 6, 1018)]
 
 
 @pytest.mark.parametrize("flags", [0, ff.FF_BLOCKED_LAYOUT])
@@ -253,7 +255,10 @@ def test_slab_pipeline_bitwise(n, slab):
 SOLVE_SHAPES = [(8, 8, 8), (12, 10, 9), (16, 24, 20), (32, 32, 32), (64, 64, 64), (6, 9, 15), (5, 6, 4),
                 (9, 8, 12), (48, 40, 36), (128, 64, 96), (2, 2, 2), (2, 3, 4), (2, 8, 16), (16, 2, 9),
                 (30, 25, 27), (100, 96, 80), (64, 64, 2), (4, 4, 2), (3, 3, 3), (250, 2, 6), (96, 160, 48),
-                (48, 600, 1000), (40, 1024, 300)]
+                (48, 600, 1000), (40, 1024, 300),
+                # prime factors above 5 (Bluestein lines)
+                (7, 5, 11), (11, 13, 17), (101, 8, 9), (37, 41, 43), (509, 4, 6), (14, 49, 22), (2, 7, 7),
+                (97, 96, 194)]
 
 
 @pytest.mark.parametrize("n", SOLVE_SHAPES)
@@ -267,21 +272,18 @@ def test_solve_matches_oracle(n):
 
 
 def test_solve_matches_golden(golden):
+    """Every committed golden solve, including extents with prime factors
+    above 5 (7x5x11), against the GPU solve."""
     tags = sorted({k[len("solve_rand_in_"):] for k in golden if k.startswith("solve_rand_in_")})
     checked = 0
     for kind in ("rand", "field"):
         for tag in tags:
             n = tuple(int(v) for v in tag.split("x"))
-            try:
-                ff.ParisPencil(n, *ff.benchmark_grid(n)[:2]).close()
-            except ff.FishFryError as e:
-                assert e.status == ff.FF_ERR_UNSUPPORTED  # prime factor > 5
-                continue
             say("golden", kind, tag)
             got = gpu_solve(golden[f"solve_{kind}_in_{tag}"])
             assert rel(got, golden[f"solve_{kind}_out_{tag}"]) < TOL, (kind, tag)
             checked += 1
-    assert checked >= 12
+    assert checked == 2 * len(tags) and len(tags) >= 8
 
 
 @pytest.mark.parametrize("n", [(32, 32, 32), (64, 64, 64), (48, 40, 36), (128, 96, 64)])
@@ -354,9 +356,9 @@ def test_stamps_and_contract():
         plan.solve(nb, a, a)
     assert e.value.status == ff.FF_ERR_INVALID
     plan.close()
-    # extents with a prime factor above 5 have no radix plan yet
+    # a line too long for one workgroup's transform, even through Bluestein
     with pytest.raises(ff.FishFryError) as e:
-        ff.ParisPencil((14, 8, 8), *ff.benchmark_grid((14, 8, 8))[:2])
+        ff.ParisPencil((5003, 4, 4), *ff.benchmark_grid((5003, 4, 4))[:2])
     assert e.value.status == ff.FF_ERR_UNSUPPORTED
 
 
@@ -385,7 +387,7 @@ def test_large_benchmark_norms(n):
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("n", [(64, 64, 64), (48, 40, 36)])
+@pytest.mark.parametrize("n", [(64, 64, 64), (48, 40, 36), (22, 14, 26)])
 def test_fp32_refinement_converges(n):
     say("refine", n)
     rng = np.random.default_rng(7)

@@ -1,8 +1,8 @@
 """GPU tests of the serving loop: results equal alone and in a mix (bitwise),
 host-array requests match the CPU oracle, admission by HBM footprint under a
 small budget (rejection and memory reuse), and the serving workload -- Poisson
-arrivals, log-uniform edges in [32, 512] rounded to the nearest 2^a 3^b 5^c,
-half FP64 and half FP32 + 2 refinement steps, f = the benchmark field plus 16
+arrivals, log-uniform integer edges in [32, 512] (primes included: those
+lines go through Bluestein), half FP64 and half FP32 + 2 refinement steps, f = the benchmark field plus 16
 random modes -- with every result checked against its exact potential."""
 import math
 import time
@@ -19,20 +19,6 @@ torch = pytest.importorskip("torch")
 
 def say(*a):
     print(*a, flush=True)
-
-
-def smooth_edge(e):
-    """Nearest integer >= 2 whose only prime factors are 2, 3 and 5."""
-    def ok(m):
-        for p in (2, 3, 5):
-            while m % p == 0:
-                m //= p
-        return m == 1
-    for d in range(0, 64):
-        for c in (e - d, e + d):
-            if c >= 2 and ok(c):
-                return c
-    return e
 
 
 def random_field(n, rng, nmodes=16, kmax=12):
@@ -70,7 +56,7 @@ def test_alone_equals_in_mix_bitwise():
             ids.append(("t%d" % j, srv.submit(n, prec, steps, field=fields[j], phi_host=phi)))
             outs["t%d" % j] = phi
         else:
-            n = tuple(smooth_edge(int(v)) for v in rng.integers(16, 96, size=3))
+            n = tuple(int(v) for v in rng.integers(16, 96, size=3))
             prec = ff.FF_FP64 if k % 2 else ff.FF_FP32_REFINE
             ids.append(("o", srv.submit(n, prec, 2, field=random_field(n, rng))))
     for tag, rid in ids:
@@ -128,10 +114,10 @@ def test_serving_workload():
     reqs = []
     for k in range(nreq):
         if rng.random() < 0.5:
-            e = smooth_edge(int(round(math.exp(rng.uniform(math.log(32), math.log(512))))))
+            e = int(round(math.exp(rng.uniform(math.log(32), math.log(512)))))
             n = (e, e, e)
         else:
-            n = tuple(smooth_edge(int(round(math.exp(rng.uniform(math.log(32), math.log(512)))))) for _ in range(3))
+            n = tuple(int(round(math.exp(rng.uniform(math.log(32), math.log(512))))) for _ in range(3))
         prec = ff.FF_FP64 if k % 2 == 0 else ff.FF_FP32_REFINE
         reqs.append((n, prec))
     t0 = time.perf_counter()
