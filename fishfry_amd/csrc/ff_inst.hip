@@ -18,21 +18,21 @@ static hipError_t launch(const void* fn, const Launch& l, const void* args, size
   return hipLaunchKernel(fn, l.grid, l.block, argv, l.lds, l.stream);
 }
 
-template <int E, bool S, int G>
+template <int E, bool S, int G, bool BL>
 static const void* col_fn(int f32, int mode) {
   if (f32) {
     switch (mode) {
-      case COL_FWD: return (const void*)&k_col<float, E, COL_FWD, S, G>;
-      case COL_INV: return (const void*)&k_col<float, E, COL_INV, S, G>;
-      case COL_FILTER: return (const void*)&k_col<float, E, COL_FILTER, S, G>;
+      case COL_FWD: return (const void*)&k_col<float, E, COL_FWD, S, G, BL>;
+      case COL_INV: return (const void*)&k_col<float, E, COL_INV, S, G, BL>;
+      case COL_FILTER: return (const void*)&k_col<float, E, COL_FILTER, S, G, BL>;
       default: return nullptr;
     }
   }
   switch (mode) {
-    case COL_FWD: return (const void*)&k_col<double, E, COL_FWD, S, G>;
-    case COL_INV: return (const void*)&k_col<double, E, COL_INV, S, G>;
-    case COL_FILTER: return (const void*)&k_col<double, E, COL_FILTER, S, G>;
-    case COL_SYMBOL: return (const void*)&k_col<double, E, COL_SYMBOL, S, G>;
+    case COL_FWD: return (const void*)&k_col<double, E, COL_FWD, S, G, BL>;
+    case COL_INV: return (const void*)&k_col<double, E, COL_INV, S, G, BL>;
+    case COL_FILTER: return (const void*)&k_col<double, E, COL_FILTER, S, G, BL>;
+    case COL_SYMBOL: return (const void*)&k_col<double, E, COL_SYMBOL, S, G, BL>;
     default: return nullptr;
   }
 }
@@ -41,51 +41,65 @@ static const void* col_fn(int f32, int mode) {
 // values that long column transforms use.
 constexpr bool kTwoGroups = FF_E == 8 || FF_E == 16;
 
+// Bluestein lines use the default kernel shape (one group, complex exchange).
 template <>
-hipError_t PassLaunchers<FF_E>::col(int f32, int mode, int split, int groups, const Launch& l, const ColArgs& a) {
+hipError_t PassLaunchers<FF_E>::col(int f32, int mode, int split, int groups, int blue, const Launch& l,
+                                    const ColArgs& a) {
   const void* fn = nullptr;
-  if (groups == 1) {
-    fn = split ? col_fn<FF_E, true, 1>(f32, mode) : col_fn<FF_E, false, 1>(f32, mode);
+  if (blue) {
+    if (groups == 1 && !split) fn = col_fn<FF_E, false, 1, true>(f32, mode);
+  } else if (groups == 1) {
+    fn = split ? col_fn<FF_E, true, 1, false>(f32, mode) : col_fn<FF_E, false, 1, false>(f32, mode);
   } else if (groups == 2) {
-    if constexpr (kTwoGroups) fn = split ? col_fn<FF_E, true, 2>(f32, mode) : col_fn<FF_E, false, 2>(f32, mode);
+    if constexpr (kTwoGroups)
+      fn = split ? col_fn<FF_E, true, 2, false>(f32, mode) : col_fn<FF_E, false, 2, false>(f32, mode);
   }
   if (!fn) return hipErrorInvalidValue;
   return launch(fn, l, &a, sizeof(a));
 }
 
-template <>
-hipError_t PassLaunchers<FF_E>::r2c(int f32, int in_f32, int odd, const Launch& l, const RowArgs& a) {
+template <bool BL>
+static const void* r2c_fn(int f32, int in_f32, int odd) {
   constexpr int E = FF_E;
-  const void* fn = nullptr;
   if (!f32) {
-    if (in_f32) return hipErrorInvalidValue;
-    fn = odd ? (const void*)&k_r2c<double, double, E, true> : (const void*)&k_r2c<double, double, E, false>;
-  } else if (!in_f32) {
-    fn = odd ? (const void*)&k_r2c<float, double, E, true> : (const void*)&k_r2c<float, double, E, false>;
-  } else {
-    fn = odd ? (const void*)&k_r2c<float, float, E, true> : (const void*)&k_r2c<float, float, E, false>;
+    if (in_f32) return nullptr;
+    return odd ? (const void*)&k_r2c<double, double, E, true, BL> : (const void*)&k_r2c<double, double, E, false, BL>;
   }
+  if (!in_f32)
+    return odd ? (const void*)&k_r2c<float, double, E, true, BL> : (const void*)&k_r2c<float, double, E, false, BL>;
+  return odd ? (const void*)&k_r2c<float, float, E, true, BL> : (const void*)&k_r2c<float, float, E, false, BL>;
+}
+
+template <>
+hipError_t PassLaunchers<FF_E>::r2c(int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a) {
+  const void* fn = blue ? r2c_fn<true>(f32, in_f32, odd) : r2c_fn<false>(f32, in_f32, odd);
+  if (!fn) return hipErrorInvalidValue;
   return launch(fn, l, &a, sizeof(a));
 }
 
 // out_kind: 0 = double output, 1 = float output.
-template <>
-hipError_t PassLaunchers<FF_E>::c2r(int f32, int out_kind, int odd, int epi, const Launch& l, const RowArgs& a) {
+template <bool BL>
+static const void* c2r_fn(int f32, int out_kind, int odd, int epi) {
   constexpr int E = FF_E;
-  const void* fn = nullptr;
 #define FF_PICK(F, OUT, EPI) \
-  fn = odd ? (const void*)&k_c2r<F, OUT, E, true, EPI> : (const void*)&k_c2r<F, OUT, E, false, EPI>
+  return odd ? (const void*)&k_c2r<F, OUT, E, true, EPI, BL> : (const void*)&k_c2r<F, OUT, E, false, EPI, BL>
   if (!f32) {
     if (epi == EPI_STORE && out_kind == 0) FF_PICK(double, double, EPI_STORE);
-    else if (epi == EPI_RESIDUAL && out_kind == 1) FF_PICK(double, float, EPI_RESIDUAL);
-    else if (epi == EPI_RESIDUAL && out_kind == 0) FF_PICK(double, double, EPI_RESIDUAL);
-    else return hipErrorInvalidValue;
-  } else {
-    if (epi == EPI_STORE && out_kind == 0) FF_PICK(float, double, EPI_STORE);
-    else if (epi == EPI_ACCUM && out_kind == 0) FF_PICK(float, double, EPI_ACCUM);
-    else return hipErrorInvalidValue;
+    if (epi == EPI_RESIDUAL && out_kind == 1) FF_PICK(double, float, EPI_RESIDUAL);
+    if (epi == EPI_RESIDUAL && out_kind == 0) FF_PICK(double, double, EPI_RESIDUAL);
+    return nullptr;
   }
+  if (epi == EPI_STORE && out_kind == 0) FF_PICK(float, double, EPI_STORE);
+  if (epi == EPI_ACCUM && out_kind == 0) FF_PICK(float, double, EPI_ACCUM);
+  return nullptr;
 #undef FF_PICK
+}
+
+template <>
+hipError_t PassLaunchers<FF_E>::c2r(int f32, int out_kind, int odd, int epi, int blue, const Launch& l,
+                                    const RowArgs& a) {
+  const void* fn = blue ? c2r_fn<true>(f32, out_kind, odd, epi) : c2r_fn<false>(f32, out_kind, odd, epi);
+  if (!fn) return hipErrorInvalidValue;
   return launch(fn, l, &a, sizeof(a));
 }
 

@@ -80,9 +80,9 @@ bool tune_lookup(int kind, int n, int prec, int& E, int& T, int& flags);
 ff_status tune_set(const ff_tune* t);
 void tune_reset();
 
-hipError_t launch_col(int E, int f32, int mode, int split, int groups, const Launch& l, const ColArgs& a);
-hipError_t launch_r2c(int E, int f32, int in_f32, int odd, const Launch& l, const RowArgs& a);
-hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, const Launch& l, const RowArgs& a);
+hipError_t launch_col(int E, int f32, int mode, int split, int groups, int blue, const Launch& l, const ColArgs& a);
+hipError_t launch_r2c(int E, int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a);
+hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, const Launch& l, const RowArgs& a);
 
 // ---- driver kernels ---------------------------------------------------------
 ff_status field_check(const ff_field* f);

@@ -91,11 +91,11 @@ enum { COL_FWD = 0, COL_INV = 1, COL_FILTER = 2, COL_SYMBOL = 3 };
 // product with the kernel spectrum (bsf / bsi, already divided by m) and
 // inverse transform.  Positions >= nline are zero on entry to the
 // convolution and undefined on exit.
-template <int E, int DIR, typename F, typename Idx, bool SPLIT = false>
+template <int E, int DIR, typename F, typename Idx, bool SPLIT, bool BL>
 __device__ __forceinline__ void line_dft(cx<F> (&v)[E], const LinePlan& lp, int nline, const cx<F>* __restrict__ tw,
                                          const void* chirp_p, const void* bsf_p, const void* bsi_p, cx<F>* lds,
                                          const Idx& idx, int r0) {
-  if (nline == lp.n) {
+  if constexpr (!BL) {
     fft_line<E, DIR, F, Idx, SPLIT>(v, lp, tw, lds, idx, r0);
     return;
   }
@@ -128,16 +128,16 @@ __device__ __forceinline__ void line_dft(cx<F> (&v)[E], const LinePlan& lp, int 
 
 // One column line held in registers through the transform, filter or symbol
 // and the inverse transform of MODE.
-template <typename F, int E, int MODE, bool SPLIT>
+template <typename F, int E, int MODE, bool SPLIT, bool BL>
 __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const cx<F>* __restrict__ tw, cx<F>* lds,
                                          const ColIdx& idx, int r0, int outer, int col, bool valid) {
   const int L = a.lp.L;
   if constexpr (MODE == COL_FWD) {
-    line_dft<E, -1, F, ColIdx, SPLIT>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    line_dft<E, -1, F, ColIdx, SPLIT, BL>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   } else if constexpr (MODE == COL_INV) {
-    line_dft<E, +1, F, ColIdx, SPLIT>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    line_dft<E, +1, F, ColIdx, SPLIT, BL>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   } else {
-    line_dft<E, -1, F, ColIdx, SPLIT>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    line_dft<E, -1, F, ColIdx, SPLIT, BL>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
     // spectral filter of ParisPencil.cpp:269-284 at (i, j, kh) = (r, outer, col)
     const int jj = outer, kk = col;
     const double j2 = sqr(double(jj < a.ny - jj ? jj : a.ny - jj) * a.ddj);
@@ -154,7 +154,7 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
       const F df = F(d);
       v[i] = cx<F>{df * v[i].x, df * v[i].y};
     }
-    line_dft<E, +1, F, ColIdx, SPLIT>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    line_dft<E, +1, F, ColIdx, SPLIT, BL>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   }
 }
 
@@ -164,7 +164,9 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
 // it), but all loads are issued first and all stores last, so each row of the
 // tile -- G*T*sizeof(complex) contiguous bytes -- is read and written whole.
 // SPLIT: exchange real and imaginary parts separately through half the LDS.
-template <typename F, int E, int MODE, bool SPLIT, int G>
+// BL: Bluestein lines (nline < lp.n); a separate instantiation so the common
+// kernels carry none of its registers.
+template <typename F, int E, int MODE, bool SPLIT, int G, bool BL>
 __global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
     long long ib, is;
     side_addr(a.in, outer, col < a.ncols ? col : a.ncols - 1, r0, L, ib, is);
     const cx<F>* src = reinterpret_cast<const cx<F>*>(a.src) + ib;
-    if (a.nline == a.lp.n) {
+    if constexpr (!BL) {
 #pragma unroll
       for (int i = 0; i < E; ++i) v[g][i] = src[(long long)i * is];
     } else {  // Bluestein: the engine line is longer than the data line
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     const int col = (tile * G + g) * a.T + c;
-    col_line<F, E, MODE, SPLIT>(v[g], a, tw, lds, idx, r0, outer, col, col < a.ncols);
+    col_line<F, E, MODE, SPLIT, BL>(v[g], a, tw, lds, idx, r0, outer, col, col < a.ncols);
   }
 #pragma unroll
   for (int g = 0; g < G; ++g) {
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
       cx<F>* dst = reinterpret_cast<cx<F>*>(a.dst) + ob;
 #pragma unroll
       for (int i = 0; i < E; ++i)
-        if (a.nline == a.lp.n || r0 + i * L < a.nline) dst[(long long)i * os] = v[g][i];
+        if (!BL || r0 + i * L < a.nline) dst[(long long)i * os] = v[g][i];
     }
   }
 }
@@ -251,7 +253,7 @@ __device__ __forceinline__ long long spec_row(const RowArgs& a, long long row) {
 // z forward: real rows (IN precision) -> half-spectrum rows (F precision).
 // Even nz: pack (x[2r], x[2r+1]) as one complex, an nz/2-point transform, then
 // split the even/odd spectra.  Odd nz: a full nz-point transform of (x, 0).
-template <typename F, typename IN, int E, bool ODD>
+template <typename F, typename IN, int E, bool ODD, bool BL>
 __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
   const bool valid = row < a.nrows;
   const long long rowc = valid ? row : a.nrows - 1;  // loads stay in bounds; stores are predicated
   const int M = a.nline;                  // transform length (the engine's lp.n is longer for Bluestein)
-  const bool blue = a.nline != a.lp.n;    // positions >= M exist only for Bluestein
+  constexpr bool blue = BL;               // positions >= M exist only for Bluestein
   const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
   const RowIdx idx{line * a.pitch, a.ps};
   cx<F> v[E];
@@ -284,7 +286,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
         v[i] = cx<F>{F(0), F(0)};
       }
     }
-    line_dft<E, -1, F>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    line_dft<E, -1, F, RowIdx, false, BL>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
     __syncthreads();
     put_natural<E, F>(v, lds, idx, r0, L);
     __syncthreads();
@@ -314,7 +316,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
       const int k = r0 + i * L;
       v[i] = cx<F>{(!blue || k < M) ? F(src[k]) : F(0), F(0)};
     }
-    line_dft<E, -1, F>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    line_dft<E, -1, F, RowIdx, false, BL>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
     cx<F>* __restrict__ dst = reinterpret_cast<cx<F>*>(a.dst) + (valid ? spec_row(a, row) : 0);
     if (valid) {
 #pragma unroll
@@ -361,7 +363,7 @@ __device__ __forceinline__ void c2r_out2(OUT* out, const double* f, long long r,
 // z inverse: half-spectrum rows (F) -> real rows (OUT) with an epilogue.
 // The imaginary parts of the DC and (even nz) Nyquist terms are ignored,
 // i.e. the rows are read as Hermitian half spectra.
-template <typename F, typename OUT, int E, bool ODD, int EPI>
+template <typename F, typename OUT, int E, bool ODD, int EPI, bool BL>
 __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
@@ -372,7 +374,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
   const long long row = (long long)xcd_remap(blockIdx.x, a.nblocks) * a.T + line;
   const bool valid = row < a.nrows;
   const int M = a.nline;                // transform length (the engine's lp.n is longer for Bluestein)
-  const bool blue = a.nline != a.lp.n;  // positions >= M exist only for Bluestein
+  constexpr bool blue = BL;             // positions >= M exist only for Bluestein
   const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
   const RowIdx idx{line * a.pitch, a.ps};
   // stage the half spectrum of the workgroup's rows in LDS; rows past the end
@@ -382,7 +384,8 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int k = r0 + i * L;
-      if (k < a.nh) lds[idx(k)] = src[k];
+      // even nz without Bluestein: k < M < nh always
+      if ((!ODD && !BL) || k < a.nh) lds[idx(k)] = src[k];
     }
     if (!ODD && r0 == 0) lds[idx(M)] = src[M];
   } else {
@@ -446,7 +449,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
       v[i] = y;
     }
   }
-  line_dft<E, +1, F>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+  line_dft<E, +1, F, RowIdx, false, BL>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   if (valid) {
     const double shift = a.shiftp ? a.shift * *a.shiftp : a.shift;
     OUT* out = reinterpret_cast<OUT*>(a.dst) + row * a.nz;
@@ -475,9 +478,9 @@ struct Launch {
 
 template <int E>
 struct PassLaunchers {
-  static hipError_t col(int f32, int mode, int split, int groups, const Launch& l, const ColArgs& a);
-  static hipError_t r2c(int f32, int in_f32, int odd, const Launch& l, const RowArgs& a);
-  static hipError_t c2r(int f32, int out_kind, int odd, int epi, const Launch& l, const RowArgs& a);
+  static hipError_t col(int f32, int mode, int split, int groups, int blue, const Launch& l, const ColArgs& a);
+  static hipError_t r2c(int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a);
+  static hipError_t c2r(int f32, int out_kind, int odd, int epi, int blue, const Launch& l, const RowArgs& a);
 };
 
 }  // namespace ff

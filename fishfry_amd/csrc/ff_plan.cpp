@@ -311,6 +311,7 @@ bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, i
   out.split = (flags & FF_TUNE_SPLIT_LDS) ? 1 : 0;
   out.groups = (flags & FF_TUNE_TWO_GROUPS) ? 2 : 1;
   if (out.groups == 2 && out.line.E != 8 && out.line.E != 16) return false;
+  if (out.line.bluestein() && (out.split || out.groups != 1)) return false;  // Bluestein: default shape only
   const size_t ne = (size_t)out.line.lp.n;  // engine length (Bluestein: m >= 2n - 1)
   const size_t elems = ne * T + ((ne - 1) >> out.ps) + 1;
   out.lds = elems * (size_t)(out.split ? elem_bytes / 2 : elem_bytes);
@@ -415,11 +416,11 @@ void tune_reset() {
 
 #define FF_DECL(E)                                                                                         \
   template <>                                                                                              \
-  hipError_t PassLaunchers<E>::col(int, int, int, int, const Launch&, const ColArgs&);                               \
+  hipError_t PassLaunchers<E>::col(int, int, int, int, int, const Launch&, const ColArgs&);                          \
   template <>                                                                                              \
-  hipError_t PassLaunchers<E>::r2c(int, int, int, const Launch&, const RowArgs&);                          \
+  hipError_t PassLaunchers<E>::r2c(int, int, int, int, const Launch&, const RowArgs&);                     \
   template <>                                                                                              \
-  hipError_t PassLaunchers<E>::c2r(int, int, int, int, const Launch&, const RowArgs&);
+  hipError_t PassLaunchers<E>::c2r(int, int, int, int, int, const Launch&, const RowArgs&);
 FF_DECL(1)
 FF_DECL(2)
 FF_DECL(3)
@@ -455,14 +456,14 @@ FF_DECL(30)
     default: return hipErrorInvalidValue;    \
   }
 
-hipError_t launch_col(int E, int f32, int mode, int split, int groups, const Launch& l, const ColArgs& a) {
-  FF_SWITCH_E(E, col(f32, mode, split, groups, l, a))
+hipError_t launch_col(int E, int f32, int mode, int split, int groups, int blue, const Launch& l, const ColArgs& a) {
+  FF_SWITCH_E(E, col(f32, mode, split, groups, blue, l, a))
 }
-hipError_t launch_r2c(int E, int f32, int in_f32, int odd, const Launch& l, const RowArgs& a) {
-  FF_SWITCH_E(E, r2c(f32, in_f32, odd, l, a))
+hipError_t launch_r2c(int E, int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a) {
+  FF_SWITCH_E(E, r2c(f32, in_f32, odd, blue, l, a))
 }
-hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, const Launch& l, const RowArgs& a) {
-  FF_SWITCH_E(E, c2r(f32, out_kind, odd, epi, l, a))
+hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, const Launch& l, const RowArgs& a) {
+  FF_SWITCH_E(E, c2r(f32, out_kind, odd, epi, blue, l, a))
 }
 #undef FF_SWITCH_E
 

@@ -290,7 +290,7 @@ ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void
   a.src = static_cast<const unsigned char*>(src) + row0 * p->n[2] * (in_f32 ? 4 : 8);
   a.dst = static_cast<unsigned char*>(dst) + row0 * p->nh * (prec ? 8 : 16);
   const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
-  return hip_check(launch_r2c(c.line.E, prec, in_f32, p->n[2] % 2, l, a), "z forward launch");
+  return hip_check(launch_r2c(c.line.E, prec, in_f32, p->n[2] % 2, c.line.bluestein(), l, a), "z forward launch");
 }
 
 ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void* src, void* dst, const double* f,
@@ -312,7 +312,8 @@ ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void
   }
   const RowCfg& c = p->zi[prec];
   const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
-  return hip_check(launch_c2r(c.line.E, prec, out_kind, p->n[2] % 2, epi, l, a), "z inverse launch");
+  return hip_check(launch_c2r(c.line.E, prec, out_kind, p->n[2] % 2, epi, c.line.bluestein(), l, a),
+                   "z inverse launch");
 }
 
 ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* src, void* dst, hipStream_t s,
@@ -361,7 +362,8 @@ ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* sr
   a.bsf = bl[1];
   a.bsi = bl[2];
   const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
-  return hip_check(launch_col(c.line.E, prec, mode, c.split, c.groups, l, a), "column pass launch");
+  return hip_check(launch_col(c.line.E, prec, mode, c.split, c.groups, c.line.bluestein(), l, a),
+                   "column pass launch");
 }
 
 static ff_status solve_fp32_refine(const ff_plan* p, const double* f, double* phi, hipStream_t s) {
