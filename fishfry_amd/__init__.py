@@ -419,8 +419,9 @@ class Server:
         r.want_norms = 1 if want_norms else 0
         return r
 
-    def footprint(self, n, precision=FF_FP64, refine_steps=0) -> int:
-        r = self.request(n, precision, refine_steps)
+    def footprint(self, n, precision=FF_FP64, refine_steps=0, field: Optional[Field] = None,
+                  want_norms=False) -> int:
+        r = self.request(n, precision, refine_steps, field, None, None, want_norms)
         return int(lib().ff_server_footprint(ctypes.byref(r)))
 
     def submit(self, n, precision=FF_FP64, refine_steps=0, field: Optional[Field] = None, f_host=None,

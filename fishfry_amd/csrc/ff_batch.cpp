@@ -98,52 +98,10 @@ double dd_const(int n, double lo, double hi) { return 2.0 * M_PI * double(n - 1)
 
 size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 
-// ---- resident twiddle cache --------------------------------------------------
-
-struct TwKey {
-  int dev, n, prec, post;
-  bool operator<(const TwKey& o) const {
-    return std::tie(dev, n, prec, post) < std::tie(o.dev, o.n, o.prec, o.post);
-  }
-};
-
-const void* cached_table(int n, int prec, int post) {
-  static std::mutex m;
-  static std::map<TwKey, void*> cache;
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-  std::lock_guard<std::mutex> g(m);
-  const TwKey key{dev, n, prec, post};
-  auto it = cache.find(key);
-  if (it != cache.end()) return it->second;
-  const int count = post ? n / 2 : n;
-  const size_t eb = prec ? 8 : 16;
-  std::vector<unsigned char> buf(std::max<size_t>(eb, eb * count));
-  for (int t = 0; t < count; ++t) {
-    long double c, s;
-    twiddle_ld(t, n, c, s);
-    if (prec) {
-      const float v[2] = {(float)c, (float)s};
-      memcpy(&buf[t * eb], v, sizeof v);
-    } else {
-      const double v[2] = {(double)c, (double)s};
-      memcpy(&buf[t * eb], v, sizeof v);
-    }
-  }
-  void* d = nullptr;
-  if (hipMalloc(&d, buf.size()) != hipSuccess) return nullptr;
-  if (hipMemcpy(d, buf.data(), buf.size(), hipMemcpyHostToDevice) != hipSuccess) {
-    (void)hipFree(d);
-    return nullptr;
-  }
-  cache[key] = d;
-  return d;
-}
-
 }  // namespace
 
-const void* twiddle_table(int n, int prec) { return cached_table(n, prec, 0); }
-const void* twiddle_post(int nz, int prec) { return cached_table(nz, prec, 1); }
+const void* twiddle_table(int n, int prec) { return table_twiddle(n, prec); }
+const void* twiddle_post(int nz, int prec) { return table_post(nz, prec); }
 
 size_t batch_item_bytes(const ff_batch_item& it) {
   const int ones[3] = {1, 1, 1};

@@ -210,12 +210,23 @@ ff_status field_init(const ff_field* f, double* out, hipStream_t s) {
   return hip_check(hipGetLastError(), "k_field_init launch");
 }
 
+static_assert(kDiffDoubles == 3 * DIFF_BLOCKS + 3 && kDiffResult == 3 * DIFF_BLOCKS, "diff work area");
+
 ff_status field_diff(const ff_field* f, const double* phi, double out[3], hipStream_t s) {
-  ff_status st = field_check(f);
-  if (st != FF_OK) return st;
   if (!out) return set_error(FF_ERR_INVALID, "ff_field_diff: null output");
   double* partial = nullptr;
-  st = device_scratch(3 * DIFF_BLOCKS + 3, &partial);
+  ff_status st = device_scratch(3 * DIFF_BLOCKS + 3, &partial);
+  if (st != FF_OK) return st;
+  st = field_diff_enqueue(f, phi, partial, s);
+  if (st != FF_OK) return st;
+  st = hip_check(hipMemcpyAsync(out, partial + 3 * DIFF_BLOCKS, 3 * sizeof(double), hipMemcpyDeviceToHost, s),
+                 "diff copy");
+  if (st != FF_OK) return st;
+  return hip_check(hipStreamSynchronize(s), "diff sync");
+}
+
+ff_status field_diff_enqueue(const ff_field* f, const double* phi, double* partial, hipStream_t s) {
+  ff_status st = field_check(f);
   if (st != FF_OK) return st;
   FieldArgs a;
   field_args(*f, a);
@@ -225,12 +236,7 @@ ff_status field_diff(const ff_field* f, const double* phi, double out[3], hipStr
   st = hip_check(hipGetLastError(), "k_field_diff launch");
   if (st != FF_OK) return st;
   hipLaunchKernelGGL(k_field_diff_final, dim3(1), dim3(DIFF_THREADS), 0, s, partial, nb, partial + 3 * DIFF_BLOCKS);
-  st = hip_check(hipGetLastError(), "k_field_diff_final launch");
-  if (st != FF_OK) return st;
-  st = hip_check(hipMemcpyAsync(out, partial + 3 * DIFF_BLOCKS, 3 * sizeof(double), hipMemcpyDeviceToHost, s),
-                 "diff copy");
-  if (st != FF_OK) return st;
-  return hip_check(hipStreamSynchronize(s), "diff sync");
+  return hip_check(hipGetLastError(), "k_field_diff_final launch");
 }
 
 }  // namespace ff

@@ -49,6 +49,15 @@ bool plan_line(int n, LineCfg& out, int forceE = 0);
 // b[t] = conj(w[|t|]); inverse: b[t] = w[|t|]; |t| < n, indices mod m).
 void push_bluestein(std::vector<unsigned char>& buf, int n, int m, int prec, size_t& chirp, size_t& bsf, size_t& bsi);
 
+// Device-resident tables shared by every plan, batch and server on the
+// current device (built on first use, kept until the process ends; null /
+// false when the allocation fails): W_n[t] (t < n); the even-nz post-twiddles
+// exp(-2 pi i k/nz) (k < nz/2); the Bluestein tables of length n over m
+// (chirp, forward and inverse kernel spectra).
+const void* table_twiddle(int n, int prec);
+const void* table_post(int nz, int prec);
+bool table_bluestein(int n, int m, int prec, const void* out[3]);
+
 // exp(-2 pi i t / n) in long double with exact integer argument reduction.
 void twiddle_ld(long long t, long long n, long double& c, long double& s);
 
@@ -88,5 +97,10 @@ hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, 
 ff_status field_check(const ff_field* f);
 ff_status field_init(const ff_field* f, double* out, hipStream_t s);
 ff_status field_diff(const ff_field* f, const double* phi, double out[3], hipStream_t s);
+// Enqueues the same reduction on s over a caller-provided device work area of
+// kDiffDoubles doubles; the three sums land at work + kDiffResult.
+constexpr int kDiffDoubles = 3 * 1024 + 3;
+constexpr int kDiffResult = 3 * 1024;
+ff_status field_diff_enqueue(const ff_field* f, const double* phi, double* work, hipStream_t s);
 
 }  // namespace ff

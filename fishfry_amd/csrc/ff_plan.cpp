@@ -159,6 +159,78 @@ void push_bluestein(std::vector<unsigned char>& buf, int n, int m, int prec, siz
   }
 }
 
+// ---- resident table cache ---------------------------------------------------
+
+namespace {
+struct TableKey {
+  int dev, kind, n, m, prec;
+  bool operator<(const TableKey& o) const {
+    return std::tie(dev, kind, n, m, prec) < std::tie(o.dev, o.kind, o.n, o.m, o.prec);
+  }
+};
+struct TableEntry {
+  void* mem = nullptr;
+  size_t off[3] = {0, 0, 0};
+};
+}  // namespace
+
+// Builds a table on the host, uploads it once per device and keeps it until
+// the process ends; kind 0: W_n[t], t < n; 1: exp(-2 pi i k/n), k < n/2 (at
+// least one entry); 2: the Bluestein tables of length n over m.
+static const TableEntry* table_entry(int kind, int n, int m, int prec) {
+  static std::mutex mu;
+  static std::map<TableKey, TableEntry> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> g(mu);
+  const TableKey key{dev, kind, n, m, prec};
+  auto it = cache.find(key);
+  if (it != cache.end()) return &it->second;
+  std::vector<unsigned char> buf;
+  TableEntry e;
+  if (kind == 2) {
+    push_bluestein(buf, n, m, prec, e.off[0], e.off[1], e.off[2]);
+  } else {
+    const long long count = kind == 0 ? n : std::max(1, n / 2);
+    const size_t elem = prec ? 8 : 16;
+    buf.resize((size_t)count * elem);
+    for (long long t = 0; t < count; ++t) {
+      long double c, s;
+      twiddle_ld(t, n, c, s);
+      if (prec) {
+        const float v[2] = {(float)c, (float)s};
+        memcpy(&buf[t * elem], v, sizeof v);
+      } else {
+        const double v[2] = {(double)c, (double)s};
+        memcpy(&buf[t * elem], v, sizeof v);
+      }
+    }
+  }
+  if (hipMalloc(&e.mem, buf.size()) != hipSuccess) return nullptr;
+  if (hipMemcpy(e.mem, buf.data(), buf.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(e.mem);
+    return nullptr;
+  }
+  return &(cache[key] = e);
+}
+
+const void* table_twiddle(int n, int prec) {
+  const TableEntry* e = table_entry(0, n, 0, prec);
+  return e ? e->mem : nullptr;
+}
+
+const void* table_post(int nz, int prec) {
+  const TableEntry* e = table_entry(1, nz, 0, prec);
+  return e ? e->mem : nullptr;
+}
+
+bool table_bluestein(int n, int m, int prec, const void* out[3]) {
+  const TableEntry* e = table_entry(2, n, m, prec);
+  if (!e) return false;
+  for (int k = 0; k < 3; ++k) out[k] = static_cast<const unsigned char*>(e->mem) + e->off[k];
+  return true;
+}
+
 // ---- radix plans ------------------------------------------------------------
 
 static const int kRadices[] = {16, 12, 10, 8, 6, 5, 4, 3, 2};

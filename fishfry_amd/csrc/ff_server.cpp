@@ -102,10 +102,11 @@ struct Req {
   ff_field field{};
   bool has_field = false;
   double t_submit = 0, t_dispatch = 0, t_done = 0;
-  size_t off_d = 0, off_p = 0, off_s = 0;
-  bool admitted = false;  // off_d / off_p (and off_s if has_s) hold arena blocks
-  bool has_s = false;
-  size_t bytes = 0, scratch = 0;
+  size_t off_d = 0, off_p = 0, off_s = 0, off_n = 0;
+  bool admitted = false;  // off_d / off_p (and off_s / off_n if has_s / has_n) hold arena blocks
+  bool has_s = false, has_n = false;
+  size_t bytes = 0, scratch = 0, normbytes = 0;
+  int slot = -1;  // pinned host slot receiving the six norm sums
   ff_status status = FF_OK;
   double norms[3] = {0, 0, 0};
   bool done = false;
@@ -136,6 +137,10 @@ struct ff_server {
   std::vector<Req*> pending;
   std::vector<Group> inflight;
   std::map<uint64_t, Req*> all;
+  // pinned host slots (6 doubles each) the norm sums of in-flight requests
+  // are copied into; owned by the scheduler thread
+  double* pinned = nullptr;
+  std::vector<int> free_slots;
   std::atomic<bool> stop{false};
   std::thread worker;
   uint64_t next_id = 1;
@@ -147,6 +152,11 @@ struct ff_server {
 };
 
 namespace {
+
+// Pinned host slots for in-flight norm results, and the device work area of
+// one request's two norm reductions.
+constexpr int kNormSlots = 4096;
+size_t norm_bytes() { return 2 * sizeof(double) * kDiffDoubles; }
 
 int size_class(const ff_request& r) {
   const double n = double(r.n[0]) * r.n[1] * r.n[2];
@@ -178,7 +188,12 @@ ff_plan* plan_for(ff_server* s, const Req& q, ff_status& st) {
 // Frees the request's memory, then publishes its result (after which a
 // waiter may delete it).  count: include it in the latency statistics.
 void finish(ff_server* s, Req* q, ff_status st, bool count) {
+  if (q->slot >= 0) {
+    s->free_slots.push_back(q->slot);
+    q->slot = -1;
+  }
   if (q->admitted) {
+    if (q->has_n) s->arena.release(q->off_n);
     if (q->has_s) s->arena.release(q->off_s);
     s->arena.release(q->off_d);
     s->arena.release(q->off_p);
@@ -225,6 +240,24 @@ void dispatch(ff_server* s, std::vector<Req*>& group, int k) {
       st = hip_check(hipMemcpyAsync(q->r.phi_host, phi, sizeof(double) * q->r.n[0] * q->r.n[1] * q->r.n[2],
                                     hipMemcpyDeviceToHost, hs),
                      "request output copy");
+    if (st == FF_OK && q->has_n) {
+      // error norms against the exact potential, reduced on the request's own
+      // stream into its arena block and copied to a pinned host slot, so
+      // completion needs no synchronization
+      double* w = reinterpret_cast<double*>(base + q->off_n);
+      st = field_diff_enqueue(&q->field, phi, w, hs);
+      if (st == FF_OK) st = field_diff_enqueue(&q->field, nullptr, w + kDiffDoubles, hs);
+      if (st == FF_OK && !s->free_slots.empty()) {
+        q->slot = s->free_slots.back();
+        s->free_slots.pop_back();
+        double* h = s->pinned + 6 * q->slot;
+        st = hip_check(hipMemcpyAsync(h, w + kDiffResult, 3 * sizeof(double), hipMemcpyDeviceToHost, hs), "norms copy");
+        if (st == FF_OK)
+          st = hip_check(hipMemcpyAsync(h + 3, w + kDiffDoubles + kDiffResult, 3 * sizeof(double),
+                                        hipMemcpyDeviceToHost, hs),
+                         "norms copy");
+      }
+    }
     if (st != FF_OK) {
       finish(s, q, st, false);
       continue;
@@ -250,12 +283,20 @@ void dispatch(ff_server* s, std::vector<Req*>& group, int k) {
 void complete(ff_server* s, Group& g) {
   for (Req* q : g.reqs) {
     ff_status st = FF_OK;
-    if (q->r.want_norms && q->has_field) {
-      unsigned char* base = static_cast<unsigned char*>(s->arena_mem);
+    if (q->has_n) {
       double e[3], z[3];
-      std::lock_guard<std::mutex> sg(scratch_mutex());
-      st = field_diff(&q->field, reinterpret_cast<double*>(base + q->off_p), e, s->streams[g.stream]);
-      if (st == FF_OK) st = field_diff(&q->field, nullptr, z, s->streams[g.stream]);
+      if (q->slot >= 0) {  // copied by the stream before the group's event
+        const double* h = s->pinned + 6 * q->slot;
+        for (int k = 0; k < 3; ++k) {
+          e[k] = h[k];
+          z[k] = h[3 + k];
+        }
+      } else {  // no pinned slot was free: read the completed sums directly
+        const double* w = reinterpret_cast<const double*>(static_cast<unsigned char*>(s->arena_mem) + q->off_n);
+        st = hip_check(hipMemcpy(e, w + kDiffResult, sizeof e, hipMemcpyDeviceToHost), "norms read");
+        if (st == FF_OK)
+          st = hip_check(hipMemcpy(z, w + kDiffDoubles + kDiffResult, sizeof z, hipMemcpyDeviceToHost), "norms read");
+      }
       if (st == FF_OK) {
         q->norms[0] = e[0] / z[0];
         q->norms[1] = std::sqrt(e[1] / z[1]);
@@ -326,10 +367,19 @@ void worker_main(ff_server* s) {
           s->arena.release(op);
           continue;
         }
+        size_t on = 0;
+        if (q->normbytes && !s->arena.alloc(q->normbytes, on)) {
+          if (q->scratch) s->arena.release(os);
+          s->arena.release(od);
+          s->arena.release(op);
+          continue;
+        }
         q->off_d = od;
         q->off_p = op;
         q->off_s = os;
+        q->off_n = on;
         q->has_s = q->scratch != 0;
+        q->has_n = q->normbytes != 0;
         q->admitted = true;
         group.push_back(q);
       }
@@ -383,6 +433,15 @@ ff_status ff_server_create(const ff_server_config* cfg, ff_server** out) {
     return st;
   }
   s->arena.reset(c.hbm_budget);
+  st = hip_check(hipHostMalloc((void**)&s->pinned, sizeof(double) * 6 * kNormSlots, hipHostMallocDefault),
+                 "serving pinned slots");
+  if (st != FF_OK) {
+    for (hipStream_t hs : s->streams) (void)hipStreamDestroy(hs);
+    (void)hipFree(s->arena_mem);
+    delete s;
+    return st;
+  }
+  for (int k = kNormSlots - 1; k >= 0; --k) s->free_slots.push_back(k);
   s->worker = std::thread(worker_main, s);
   *out = s;
   return FF_OK;
@@ -399,6 +458,7 @@ size_t ff_server_footprint(const ff_request* r) {
     const size_t Nh = (size_t)r->n[0] * r->n[1] * (r->n[2] / 2 + 1);
     b += align_up(Nh * 16 + align_up(N * 4, 256) + 256, 256);
   }
+  if (r->want_norms && r->field) b += align_up(norm_bytes(), 256);
   return b;
 }
 
@@ -439,6 +499,7 @@ ff_status ff_server_submit(ff_server* s, const ff_request* r, uint64_t* id) {
     const size_t Nh = (size_t)r->n[0] * r->n[1] * (r->n[2] / 2 + 1);
     q->scratch = Nh * 16 + align_up(N * 4, 256) + 256;
   }
+  if (r->want_norms && r->field) q->normbytes = norm_bytes();
   q->cls = size_class(*r);
   q->t_submit = now_s();
   {
@@ -510,6 +571,7 @@ ff_status ff_server_destroy(ff_server* s) {
   for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
   for (hipStream_t hs : s->streams) (void)hipStreamDestroy(hs);
   if (s->arena_mem) (void)hipFree(s->arena_mem);
+  if (s->pinned) (void)hipHostFree(s->pinned);
   for (auto& kv : s->all) delete kv.second;
   delete s;
   return FF_OK;

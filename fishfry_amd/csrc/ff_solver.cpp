@@ -45,24 +45,6 @@ static double dd_const(int n, double lo, double hi) {
   return 2.0 * M_PI * double(n - 1) / (double(n) * (hi - lo));
 }
 
-// Append the table W_n[t] = exp(-2 pi i t/n), t < count, in precision prec.
-static void push_table(std::vector<unsigned char>& buf, long long n, long long count, int prec, size_t& off) {
-  const size_t elem = prec ? 8 : 16;
-  off = (buf.size() + 255) / 256 * 256;
-  buf.resize(off + (size_t)count * elem);
-  for (long long t = 0; t < count; ++t) {
-    long double c, s;
-    twiddle_ld(t, n, c, s);
-    if (prec) {
-      const float v[2] = {(float)c, (float)s};
-      memcpy(&buf[off + t * elem], v, sizeof v);
-    } else {
-      const double v[2] = {(double)c, (double)s};
-      memcpy(&buf[off + t * elem], v, sizeof v);
-    }
-  }
-}
-
 ff_status plan_create(const int n[3], const double lo[3], const double hi[3], const ff_options* opts,
                       ff_plan** out) {
   *out = nullptr;
@@ -123,47 +105,29 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   }
 
   // twiddle tables of the engine transforms (length m for Bluestein lines),
-  // the even-nz post-twiddles and the Bluestein tables
-  std::vector<unsigned char> buf;
-  size_t oz[2], ozi[2], ozp[2], oy[2], ox[2];
-  size_t bz[2][3], bzi[2][3], by[2][3], bx[2][3];
+  // the even-nz post-twiddles and the Bluestein tables, from the resident
+  // per-device cache (shared by every plan with the same line lengths)
   const int nz = n[2];
-  const bool odd = nz % 2;
-  auto blue = [&](const LineCfg& c, int prec, size_t (&o)[3]) {
-    o[0] = o[1] = o[2] = (size_t)-1;
-    if (c.bluestein()) push_bluestein(buf, c.n, c.lp.n, prec, o[0], o[1], o[2]);
+  bool ok = true;
+  auto blue = [&](const LineCfg& c, int prec, const void* (&o)[3]) {
+    o[0] = o[1] = o[2] = nullptr;
+    if (c.bluestein()) ok = ok && table_bluestein(c.n, c.lp.n, prec, o);
   };
-  for (int prec = 0; prec < nprec; ++prec) {
-    const long long M = odd ? nz : nz / 2;
-    push_table(buf, p->zr[prec].line.lp.n, p->zr[prec].line.lp.n, prec, oz[prec]);
-    push_table(buf, p->zi[prec].line.lp.n, p->zi[prec].line.lp.n, prec, ozi[prec]);
-    push_table(buf, nz, odd ? 1 : M, prec, ozp[prec]);
-    push_table(buf, p->cy[prec].line.lp.n, p->cy[prec].line.lp.n, prec, oy[prec]);
-    push_table(buf, p->cx[prec].line.lp.n, p->cx[prec].line.lp.n, prec, ox[prec]);
-    blue(p->zr[prec].line, prec, bz[prec]);
-    blue(p->zi[prec].line, prec, bzi[prec]);
-    blue(p->cy[prec].line, prec, by[prec]);
-    blue(p->cx[prec].line, prec, bx[prec]);
+  for (int prec = 0; prec < nprec && ok; ++prec) {
+    p->tw_z[prec] = table_twiddle(p->zr[prec].line.lp.n, prec);
+    p->tw_zi[prec] = table_twiddle(p->zi[prec].line.lp.n, prec);
+    p->tw_zpost[prec] = table_post(nz, prec);
+    p->tw_y[prec] = table_twiddle(p->cy[prec].line.lp.n, prec);
+    p->tw_x[prec] = table_twiddle(p->cx[prec].line.lp.n, prec);
+    ok = p->tw_z[prec] && p->tw_zi[prec] && p->tw_zpost[prec] && p->tw_y[prec] && p->tw_x[prec];
+    blue(p->zr[prec].line, prec, p->bl_z[prec]);
+    blue(p->zi[prec].line, prec, p->bl_zi[prec]);
+    blue(p->cy[prec].line, prec, p->bl_y[prec]);
+    blue(p->cx[prec].line, prec, p->bl_x[prec]);
   }
-  st = hip_check(hipMalloc(&p->tw_mem, buf.size()), "twiddle hipMalloc");
-  if (st == FF_OK) st = hip_check(hipMemcpy(p->tw_mem, buf.data(), buf.size(), hipMemcpyHostToDevice), "twiddle copy");
-  if (st != FF_OK) {
+  if (!ok) {
     plan_destroy(p);
-    return st;
-  }
-  for (int prec = 0; prec < nprec; ++prec) {
-    unsigned char* b = static_cast<unsigned char*>(p->tw_mem);
-    p->tw_z[prec] = b + oz[prec];
-    p->tw_zi[prec] = b + ozi[prec];
-    p->tw_zpost[prec] = b + ozp[prec];
-    p->tw_y[prec] = b + oy[prec];
-    p->tw_x[prec] = b + ox[prec];
-    for (int k = 0; k < 3; ++k) {
-      p->bl_z[prec][k] = bz[prec][k] == (size_t)-1 ? nullptr : b + bz[prec][k];
-      p->bl_zi[prec][k] = bzi[prec][k] == (size_t)-1 ? nullptr : b + bzi[prec][k];
-      p->bl_y[prec][k] = by[prec][k] == (size_t)-1 ? nullptr : b + by[prec][k];
-      p->bl_x[prec][k] = bx[prec][k] == (size_t)-1 ? nullptr : b + bx[prec][k];
-    }
+    return set_error(FF_ERR_NOMEM, "twiddle table allocation failed");
   }
   if (o.precision == FF_FP32_REFINE) {
     const size_t spec = (size_t)p->Nh * 16;
@@ -204,7 +168,6 @@ void plan_bind_scratch(ff_plan* p, void* base) {
 
 void plan_destroy(ff_plan* p) {
   if (!p) return;
-  if (p->tw_mem) (void)hipFree(p->tw_mem);
   if (p->scratch) (void)hipFree(p->scratch);
   if (p->work) (void)hipFree(p->work);
   delete p;

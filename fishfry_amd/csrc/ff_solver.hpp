@@ -26,11 +26,11 @@ struct ff_plan {
   const void* tw_x[2];
   // Bluestein tables (chirp, forward and inverse kernel spectra) of the z, y
   // and x line transforms; null when the line length has a direct plan
+  // (all tables live in the resident per-device cache; the plan owns none)
   const void* bl_z[2][3];
   const void* bl_zi[2][3];
   const void* bl_y[2][3];
   const void* bl_x[2][3];
-  void* tw_mem;
   // FP32 + refinement work area: FP64 spectrum (also hosts the FP32 one),
   // the FP32 residual and one double for the mean of phi.
   void* scratch;

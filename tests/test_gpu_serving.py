@@ -89,7 +89,7 @@ def test_admission_by_footprint():
     rng = np.random.default_rng(13)
     n = (64, 64, 64)
     probe = ff.Server(hbm_budget=1 << 20)
-    fp = probe.footprint(n)
+    fp = probe.footprint(n, field=ff.field_default(n), want_norms=True)
     with pytest.raises(ff.FishFryError) as e:
         probe.submit(n, field=ff.field_default(n))
     assert e.value.status == ff.FF_ERR_REJECTED
