@@ -27,6 +27,7 @@
 #include <tuple>
 #include <vector>
 
+#include "ff_arena.hpp"
 #include "ff_solver.hpp"
 
 using namespace ff;
@@ -36,65 +37,6 @@ namespace {
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
-
-size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-// First-fit allocator over [0, size) with coalescing free list.
-class Arena {
- public:
-  void reset(size_t size) {
-    size_ = size;
-    free_.clear();
-    free_[0] = size;
-    used_ = 0;
-    peak_ = 0;
-  }
-  bool alloc(size_t bytes, size_t& off) {
-    bytes = align_up(std::max<size_t>(bytes, 256), 256);
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second >= bytes) {
-        off = it->first;
-        const size_t rest = it->second - bytes;
-        free_.erase(it);
-        if (rest) free_[off + bytes] = rest;
-        used_ += bytes;
-        peak_ = std::max(peak_, used_);
-        sizes_[off] = bytes;
-        return true;
-      }
-    }
-    return false;
-  }
-  void release(size_t off) {
-    auto s = sizes_.find(off);
-    if (s == sizes_.end()) return;
-    size_t bytes = s->second;
-    sizes_.erase(s);
-    used_ -= bytes;
-    auto next = free_.lower_bound(off);
-    if (next != free_.end() && next->first == off + bytes) {
-      bytes += next->second;
-      free_.erase(next);
-    }
-    auto it = free_.lower_bound(off);
-    if (it != free_.begin()) {
-      auto prev = std::prev(it);
-      if (prev->first + prev->second == off) {
-        prev->second += bytes;
-        return;
-      }
-    }
-    free_[off] = bytes;
-  }
-  size_t size() const { return size_; }
-  size_t used() const { return used_; }
-  size_t peak() const { return peak_; }
-
- private:
-  size_t size_ = 0, used_ = 0, peak_ = 0;
-  std::map<size_t, size_t> free_;   // offset -> length
-  std::map<size_t, size_t> sizes_;  // allocated offset -> length
-};
 
 struct Req {
   uint64_t id = 0;
@@ -158,11 +100,7 @@ namespace {
 constexpr int kNormSlots = 4096;
 size_t norm_bytes() { return 2 * sizeof(double) * kDiffDoubles; }
 
-int size_class(const ff_request& r) {
-  const double n = double(r.n[0]) * r.n[1] * r.n[2];
-  const int lg = (int)std::floor(std::log2(std::max(1.0, n)) / 3.0);  // edge octave
-  return (r.precision * 16 + std::min(r.refine_steps, 15)) * 64 + lg;
-}
+int size_class(const ff_request& r) { return ff::size_class(r.precision, r.refine_steps, r.n); }
 
 ff_plan* plan_for(ff_server* s, const Req& q, ff_status& st) {
   // The domain of a field request is the field's own; a host-array request
