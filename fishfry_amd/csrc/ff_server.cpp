@@ -71,8 +71,12 @@ struct ff_server {
   std::vector<hipStream_t> streams;
   std::vector<int> stream_load;
   std::vector<hipEvent_t> free_events;
-  // plans by (extents, precision, refinement steps, lo, hi)
-  std::map<std::tuple<int, int, int, int, int, double, double, double, double, double, double>, ff_plan*> plans;
+  // plans by (extents, precision, refinement steps, lo, hi), with the tick of
+  // their last use; at most kMaxPlans are kept (least recently used evicted)
+  std::map<std::tuple<int, int, int, int, int, double, double, double, double, double, double>,
+           std::pair<ff_plan*, uint64_t>>
+      plans;
+  uint64_t plan_tick = 0;
   std::mutex mu;
   std::condition_variable cv_submit, cv_done;
   std::deque<Req*> incoming;
@@ -98,6 +102,8 @@ namespace {
 // Pinned host slots for in-flight norm results, and the device work area of
 // one request's two norm reductions.
 constexpr int kNormSlots = 4096;
+// Cached plans (host structures only).
+constexpr size_t kMaxPlans = 64;
 size_t norm_bytes() { return 2 * sizeof(double) * kDiffDoubles; }
 
 int size_class(const ff_request& r) { return ff::size_class(r.precision, r.refine_steps, r.n); }
@@ -113,13 +119,28 @@ ff_plan* plan_for(ff_server* s, const Req& q, ff_status& st) {
   }
   const auto key = std::make_tuple(q.r.n[0], q.r.n[1], q.r.n[2], q.r.precision, q.r.refine_steps, lo[0], lo[1],
                                    lo[2], hi[0], hi[1], hi[2]);
+  const uint64_t tick = ++s->plan_tick;
   auto it = s->plans.find(key);
-  if (it != s->plans.end()) return it->second;
-  ff_options o{q.r.precision, q.r.refine_steps, 0};
+  if (it != s->plans.end()) {
+    it->second.second = tick;
+    return it->second.first;
+  }
+  // Server plans own no device memory (their twiddle tables are the resident
+  // per-device cache's and each request binds its refinement work area from
+  // the arena), so evicting one whose copies still have kernels in flight is
+  // safe.
+  if (s->plans.size() >= kMaxPlans) {
+    auto lru = s->plans.begin();
+    for (auto e = s->plans.begin(); e != s->plans.end(); ++e)
+      if (e->second.second < lru->second.second) lru = e;
+    plan_destroy(lru->second.first);
+    s->plans.erase(lru);
+  }
+  ff_options o{q.r.precision, q.r.refine_steps, 0, 0};
   ff_plan* p = nullptr;
-  st = plan_create(q.r.n, lo, hi, &o, &p);
+  st = plan_create(q.r.n, lo, hi, &o, &p, /*own_scratch=*/false);
   if (st != FF_OK) return nullptr;
-  s->plans[key] = p;
+  s->plans[key] = {p, tick};
   return p;
 }
 
@@ -197,6 +218,8 @@ void dispatch(ff_server* s, std::vector<Req*>& group, int k) {
       }
     }
     if (st != FF_OK) {
+      // work already enqueued for this request may still use its blocks
+      (void)hipStreamSynchronize(hs);
       finish(s, q, st, false);
       continue;
     }
@@ -456,17 +479,22 @@ ff_status ff_server_wait(ff_server* s, uint64_t id, double timeout_s, ff_result*
   clear_error();
   if (!s || !out) return set_error(FF_ERR_INVALID, "ff_server_wait: null argument");
   std::unique_lock<std::mutex> lk(s->mu);
-  auto it = s->all.find(id);
-  if (it == s->all.end()) return set_error(FF_ERR_NOT_FOUND, "ff_server_wait: unknown request id");
-  Req* q = it->second;
   const auto deadline = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
-  while (!q->done) {
-    if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout && !q->done)
+  // looked up again after every wake: another thread waiting on the same id
+  // may have taken the result (and freed the request) meanwhile
+  auto it = s->all.find(id);
+  while (it != s->all.end() && !it->second->done) {
+    const bool timed_out = s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout;
+    it = s->all.find(id);
+    if (timed_out && it != s->all.end() && !it->second->done)
       return set_error(FF_ERR_TIMEOUT, "ff_server_wait: timed out");
   }
+  if (it == s->all.end()) return set_error(FF_ERR_NOT_FOUND, "ff_server_wait: unknown request id");
+  Req* q = it->second;
   out->status = q->status;
   out->latency_s = q->t_done - q->t_submit;
-  out->queue_s = q->t_dispatch - q->t_submit;
+  // a request that failed before dispatch spent all its time queued
+  out->queue_s = (q->t_dispatch > 0 ? q->t_dispatch : q->t_done) - q->t_submit;
   out->points = (long long)q->r.n[0] * q->r.n[1] * q->r.n[2];
   for (int k = 0; k < 3; ++k) out->norms[k] = q->norms[k];
   s->all.erase(it);
@@ -505,7 +533,7 @@ ff_status ff_server_destroy(ff_server* s) {
   s->cv_submit.notify_all();
   if (s->worker.joinable()) s->worker.join();
   for (hipStream_t hs : s->streams) (void)hipStreamSynchronize(hs);
-  for (auto& kv : s->plans) plan_destroy(kv.second);
+  for (auto& kv : s->plans) plan_destroy(kv.second.first);
   for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
   for (hipStream_t hs : s->streams) (void)hipStreamDestroy(hs);
   if (s->arena_mem) (void)hipFree(s->arena_mem);

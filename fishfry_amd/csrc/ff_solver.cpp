@@ -46,7 +46,7 @@ static double dd_const(int n, double lo, double hi) {
 }
 
 ff_status plan_create(const int n[3], const double lo[3], const double hi[3], const ff_options* opts,
-                      ff_plan** out) {
+                      ff_plan** out, bool own_scratch) {
   *out = nullptr;
   for (int a = 0; a < 3; ++a) {
     if (n[a] < 1) return set_error(FF_ERR_INVALID, "grid extents must be >= 1");
@@ -129,7 +129,7 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     plan_destroy(p);
     return set_error(FF_ERR_NOMEM, "twiddle table allocation failed");
   }
-  if (o.precision == FF_FP32_REFINE) {
+  if (o.precision == FF_FP32_REFINE && own_scratch) {
     const size_t spec = (size_t)p->Nh * 16;
     const size_t res = ((size_t)p->N * 4 + 255) / 256 * 256;
     p->scratch_bytes = spec + res + 256;
@@ -359,7 +359,10 @@ static ff_status solve_fp32_refine(const ff_plan* p, const double* f, double* ph
 }
 
 ff_status solve_enqueue(const ff_plan* p, double* density, double* potential, hipStream_t s, hipEvent_t* ev) {
-  if (p->opt.precision == FF_FP32_REFINE) return solve_fp32_refine(p, density, potential, s);
+  if (p->opt.precision == FF_FP32_REFINE) {
+    if (!p->spec) return set_error(FF_ERR_INVALID, "FP32 plan has no refinement work area bound");
+    return solve_fp32_refine(p, density, potential, s);
+  }
   ff_status st;
   // ping-pong density <-> potential (or potential <-> the plan's work array
   // with FF_PRESERVE_INPUT); the result lands in potential as in

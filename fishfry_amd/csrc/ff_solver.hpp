@@ -68,8 +68,11 @@ void pass_layouts(const ff_plan* p, int pass, int& lay_in, int& lay_out);
 // FP64 passes.
 ff_status solve_enqueue(const ff_plan* p, double* density, double* potential, hipStream_t s, hipEvent_t* ev);
 
+// own_scratch false: an FP32 plan gets no refinement work area of its own and
+// must be bound to one (plan_bind_scratch) before it is enqueued; callers that
+// sub-allocate device memory themselves (the server's arena) use this.
 ff_status plan_create(const int n[3], const double lo[3], const double hi[3], const ff_options* opts,
-                      ff_plan** out);
+                      ff_plan** out, bool own_scratch = true);
 void plan_destroy(ff_plan* p);
 // FP32 refinement work area: its size, and pointing a plan (or a copy of one)
 // at a work area of that size.

@@ -255,7 +255,9 @@ size_t ff_server_footprint(const ff_request* r);
  * The field descriptor is copied; f_host / phi_host must stay valid until the
  * request completes. */
 ff_status ff_server_submit(ff_server* s, const ff_request* r, uint64_t* id);
-/* Waits for a request (once per id) and returns its result. */
+/* Waits for a request and returns its result; the result is handed out once
+ * (a later wait on the same id, or one that loses the race with another
+ * thread's, returns FF_ERR_NOT_FOUND). */
 ff_status ff_server_wait(ff_server* s, uint64_t id, double timeout_s, ff_result* out);
 ff_status ff_server_stats(ff_server* s, ff_serving_stats* out);
 /* Drains the queue, then stops the scheduler and frees everything. */

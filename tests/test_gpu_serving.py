@@ -136,3 +136,44 @@ def test_serving_workload():
     assert st["completed"] == nreq
     assert worst[ff.FF_FP64] < 1e-13 and worst[ff.FF_FP32_REFINE] < 1e-13
     srv.close()
+
+
+def test_plan_cache_eviction():
+    # more distinct shapes than the server keeps plans for: the first shape's
+    # plan is evicted and rebuilt, and its result is bitwise the same
+    rng = np.random.default_rng(14)
+    srv = ff.Server(hbm_budget=1 << 30, streams=2, max_batch=4)
+    first = (20, 18, 22)
+    fld = random_field(first, rng)
+    a, b = np.empty(first), np.empty(first)
+    srv.wait(srv.submit(first, ff.FF_FP32_REFINE, 2, field=fld, phi_host=a))
+    ids = []
+    for k in range(70):
+        n = (32 + k % 7, 32 + (k // 7) % 10, 32 + k % 5)  # the benchmark f is at round-off from 32 up
+        prec = ff.FF_FP32_REFINE if k % 2 else ff.FF_FP64
+        ids.append(srv.submit(n, prec, 2, field=random_field(n, rng, kmax=3), want_norms=True))
+    worst = max(srv.wait(rid)["norms"][2] for rid in ids)
+    srv.wait(srv.submit(first, ff.FF_FP32_REFINE, 2, field=fld, phi_host=b))
+    st = srv.stats()
+    say("eviction", st, worst)
+    assert st["completed"] == 72 and worst < 1e-13
+    assert np.array_equal(a, b)
+    srv.close()
+
+
+def test_wait_semantics():
+    srv = ff.Server(hbm_budget=4 << 30, streams=2)
+    n = (256, 256, 256)
+    rid = srv.submit(n, field=ff.field_default(n), want_norms=True)
+    with pytest.raises(ff.FishFryError) as e:
+        srv.wait(rid, timeout=1e-6)  # the request cannot complete within a microsecond
+    assert e.value.status == ff.FF_ERR_TIMEOUT
+    r = srv.wait(rid)
+    assert r["status"] == 0 and r["norms"][2] < 1e-13 and r["queue_s"] >= 0
+    with pytest.raises(ff.FishFryError) as e:
+        srv.wait(rid)  # a result is handed out once
+    assert e.value.status == ff.FF_ERR_NOT_FOUND
+    with pytest.raises(ff.FishFryError) as e:
+        srv.wait(123456789)
+    assert e.value.status == ff.FF_ERR_NOT_FOUND
+    srv.close()
