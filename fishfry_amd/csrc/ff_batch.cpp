@@ -589,7 +589,9 @@ ff_status ff_batch_autotune(ff_batch* b, double* const* density, double* const* 
     const int eb = prec ? 8 : 16;
     double best = 1e30;
     int bE = 0, bT = 0, bF = 0;
-    const int nflags = (kind == 1 || kind == 2) ? 4 : 1;  // column passes: split exchange, two groups
+    // column passes: split exchange, two groups, and (flags 4) the generic
+    // kernel where one with the line plan compiled in exists
+    const int nflags = (kind == 1 || kind == 2) ? 5 : 1;
     for (int E : kEValues) {
       for (int cand = 0; cand < 7 * nflags && st == FF_OK; ++cand) {
         const int T = 1 << (cand % 7);
@@ -602,8 +604,9 @@ ff_status ff_batch_autotune(ff_batch* b, double* const* density, double* const* 
           ok = row_cfg_with(p->n[2], eb, c, E, T) && c.line.n == n;
           if (ok) (kind == 0 ? trial.zr : trial.zi)[prec] = c;
         } else {
-          ColCfg c;
+          ColCfg c, c0;
           ok = col_cfg_with(n, p->nh, eb, c, E, T, fl);
+          if (ok && fl == FF_TUNE_GENERIC) ok = col_cfg_with(n, p->nh, eb, c0, E, T, 0) && c0.stat;
           if (ok) (kind == 1 ? trial.cy : trial.cx)[prec] = c;
         }
         if (!ok) continue;

@@ -206,7 +206,8 @@ ff_status ff_tune_query(int kind, int n, int precision, int ncols, ff_tune* out)
       return set_error(FF_ERR_UNSUPPORTED, "ff_tune_query: no plan for that length");
     E = c.line.E;
     T = c.T;
-    flags = (c.split ? FF_TUNE_SPLIT_LDS : 0) | (c.groups == 2 ? FF_TUNE_TWO_GROUPS : 0);
+    flags = (c.split ? FF_TUNE_SPLIT_LDS : 0) | (c.groups == 2 ? FF_TUNE_TWO_GROUPS : 0) |
+            (c.stat ? 0 : FF_TUNE_GENERIC);
   }
   *out = ff_tune{kind, n, precision, E, T, flags};
   return FF_OK;

@@ -366,6 +366,86 @@ __device__ __forceinline__ void fft_line(cx<F> (&v)[E], const LinePlan& lp, cons
   }
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time line plans.  For the lengths that dominate the solve, kernels
+// are also instantiated with the plan fixed at compile time, so every LDS
+// index, modulus and twiddle stride folds to a constant and the pass loop
+// unrolls.  The plan is the one the host planner builds for (n, E): fewest
+// passes with radices dividing E (first radix of 16, 12, 10, 8, 6, 5, 4, 3, 2
+// that reaches the minimum), largest radix first, so the arithmetic is that
+// of the generic kernel.
+
+constexpr int kPlanRadices[9] = {16, 12, 10, 8, 6, 5, 4, 3, 2};
+
+constexpr int static_min_passes(int m, int E) {
+  if (m == 1) return 0;
+  int best = 1 << 20;
+  for (int t = 0; t < 9; ++t) {
+    const int R = kPlanRadices[t];
+    if (R > E || E % R || m % R) continue;
+    const int p = static_min_passes(m / R, E);
+    if (p + 1 < best) best = p + 1;
+  }
+  return best;
+}
+
+constexpr int static_best_radix(int m, int E) {
+  int best = 1 << 20, bestR = 0;
+  for (int t = 0; t < 9; ++t) {
+    const int R = kPlanRadices[t];
+    if (R > E || E % R || m % R) continue;
+    const int p = static_min_passes(m / R, E);
+    if (p + 1 < best) {
+      best = p + 1;
+      bestR = R;
+    }
+  }
+  return bestR;
+}
+
+template <int N, int E>
+constexpr LinePlan static_plan() {
+  LinePlan lp{};
+  int seq[FF_MAXP] = {};
+  int np = 0;
+  for (int m = N; m > 1 && np < FF_MAXP;) {
+    const int R = static_best_radix(m, E);
+    if (!R) break;
+    seq[np++] = R;
+    m /= R;
+  }
+  for (int a = 1; a < np; ++a)  // descending
+    for (int b = a; b > 0 && seq[b] > seq[b - 1]; --b) {
+      const int t = seq[b];
+      seq[b] = seq[b - 1];
+      seq[b - 1] = t;
+    }
+  lp.n = N;
+  lp.L = N / E;
+  lp.np = np;
+  int ns = 1;
+  for (int p = 0; p < np; ++p) {
+    lp.radix[p] = seq[p];
+    lp.ns[p] = ns;
+    lp.nsinv[p] = ns > 1 ? (uint32_t)(((1ull << 32) + (uint64_t)ns - 1) / (uint64_t)ns) : 0u;
+    lp.twstep[p] = N / (ns * seq[p]);
+    ns *= seq[p];
+  }
+  return lp;
+}
+
+// fft_line with the plan of (N, E) fixed at compile time.
+template <int N, int E, int DIR, typename F, typename Idx, bool SPLIT, int P = 0>
+__device__ __forceinline__ void fft_line_static(cx<F> (&v)[E], const cx<F>* __restrict__ tw, cx<F>* lds,
+                                                const Idx& idx, int r0) {
+  constexpr LinePlan lp = static_plan<N, E>();
+  static_assert(lp.np > 0 && N % E == 0, "no compile-time plan for this length");
+  if constexpr (P < lp.np) {
+    run_pass<E, lp.radix[P], DIR, F, SPLIT>(v, lp, P, tw, lds, idx, r0, P + 1 == lp.np);
+    fft_line_static<N, E, DIR, F, Idx, SPLIT, P + 1>(v, tw, lds, idx, r0);
+  }
+}
+
 // LDS layouts.  Column tiles: element (line c, position r) at r*T + c plus one
 // pad slot every 2^PS positions.  Row tiles: line-major with a padded pitch.
 struct ColIdx {

@@ -1,6 +1,8 @@
 // Explicit instantiation of the pass kernels for one elements-per-thread
 // value, FF_E (the build compiles this file once per E so the units build in
 // parallel).
+#include <type_traits>
+
 #include "ff_kernels.hpp"
 
 #ifndef FF_E
@@ -41,12 +43,46 @@ static const void* col_fn(int f32, int mode) {
 // values that long column transforms use.
 constexpr bool kTwoGroups = FF_E == 8 || FF_E == 16;
 
+// Kernels with the line plan compiled in: the kStaticCols entries of this E.
+template <typename F, bool S, int N, int T, int W>
+static const void* col_static_modes(int mode) {
+  constexpr int E = FF_E;
+  switch (mode) {
+    case COL_FWD: return (const void*)&k_col<F, E, COL_FWD, S, 1, false, N, T, W>;
+    case COL_INV: return (const void*)&k_col<F, E, COL_INV, S, 1, false, N, T, W>;
+    case COL_FILTER: return (const void*)&k_col<F, E, COL_FILTER, S, 1, false, N, T, W>;
+    case COL_SYMBOL:
+      if constexpr (std::is_same<F, double>::value)
+        return (const void*)&k_col<F, E, COL_SYMBOL, S, 1, false, N, T, W>;
+      return nullptr;
+    default: return nullptr;
+  }
+}
+
+template <int K>
+static const void* col_static_fn(int f32, int split, int mode, int n, int T) {
+  if constexpr (K < kNumStaticCols) {
+    constexpr StaticCol sc = kStaticCols[K];
+    if constexpr (sc.E == FF_E) {
+      if (sc.n == n && sc.T == T && sc.f32 == f32 && sc.split == split) {
+        if constexpr (sc.f32) return col_static_modes<float, sc.split != 0, sc.n, sc.T, sc.waves>(mode);
+        else return col_static_modes<double, sc.split != 0, sc.n, sc.T, sc.waves>(mode);
+      }
+    }
+    return col_static_fn<K + 1>(f32, split, mode, n, T);
+  } else {
+    return nullptr;
+  }
+}
+
 // Bluestein lines use the default kernel shape (one group, complex exchange).
 template <>
-hipError_t PassLaunchers<FF_E>::col(int f32, int mode, int split, int groups, int blue, const Launch& l,
+hipError_t PassLaunchers<FF_E>::col(int f32, int mode, int split, int groups, int blue, int stat, const Launch& l,
                                     const ColArgs& a) {
   const void* fn = nullptr;
-  if (blue) {
+  if (stat) {
+    fn = col_static_fn<0>(f32, split, mode, a.lp.n, a.T);
+  } else if (blue) {
     if (groups == 1 && !split) fn = col_fn<FF_E, false, 1, true>(f32, mode);
   } else if (groups == 1) {
     fn = split ? col_fn<FF_E, true, 1, false>(f32, mode) : col_fn<FF_E, false, 1, false>(f32, mode);

@@ -66,6 +66,7 @@ struct ColCfg {
   int T = 1, lgT = 0, ps = 30;
   int split = 0;   // FF_TUNE_SPLIT_LDS: one component per LDS slot
   int groups = 1;  // FF_TUNE_TWO_GROUPS: column groups of T per workgroup
+  int stat = 0;    // a kStaticCols kernel runs this configuration (FF_TUNE_GENERIC clears it)
   size_t lds = 0;
   int threads = 0;
 };
@@ -89,7 +90,8 @@ bool tune_lookup(int kind, int n, int prec, int& E, int& T, int& flags);
 ff_status tune_set(const ff_tune* t);
 void tune_reset();
 
-hipError_t launch_col(int E, int f32, int mode, int split, int groups, int blue, const Launch& l, const ColArgs& a);
+hipError_t launch_col(int E, int f32, int mode, int split, int groups, int blue, int stat, const Launch& l,
+                      const ColArgs& a);
 hipError_t launch_r2c(int E, int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a);
 hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, const Launch& l, const RowArgs& a);
 

@@ -91,11 +91,16 @@ enum { COL_FWD = 0, COL_INV = 1, COL_FILTER = 2, COL_SYMBOL = 3 };
 // product with the kernel spectrum (bsf / bsi, already divided by m) and
 // inverse transform.  Positions >= nline are zero on entry to the
 // convolution and undefined on exit.
-template <int E, int DIR, typename F, typename Idx, bool SPLIT, bool BL>
+// SN > 0: the plan of length SN is fixed at compile time (lp is not read).
+template <int E, int DIR, typename F, typename Idx, bool SPLIT, bool BL, int SN = 0>
 __device__ __forceinline__ void line_dft(cx<F> (&v)[E], const LinePlan& lp, int nline, const cx<F>* __restrict__ tw,
                                          const void* chirp_p, const void* bsf_p, const void* bsi_p, cx<F>* lds,
                                          const Idx& idx, int r0) {
-  if constexpr (!BL) {
+  static_assert(SN == 0 || !BL, "compile-time plans are for direct lengths");
+  if constexpr (SN > 0) {
+    fft_line_static<SN, E, DIR, F, Idx, SPLIT>(v, tw, lds, idx, r0);
+    return;
+  } else if constexpr (!BL) {
     fft_line<E, DIR, F, Idx, SPLIT>(v, lp, tw, lds, idx, r0);
     return;
   }
@@ -128,16 +133,16 @@ __device__ __forceinline__ void line_dft(cx<F> (&v)[E], const LinePlan& lp, int 
 
 // One column line held in registers through the transform, filter or symbol
 // and the inverse transform of MODE.
-template <typename F, int E, int MODE, bool SPLIT, bool BL>
+template <typename F, int E, int MODE, bool SPLIT, bool BL, int SN>
 __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const cx<F>* __restrict__ tw, cx<F>* lds,
                                          const ColIdx& idx, int r0, int outer, int col, bool valid) {
-  const int L = a.lp.L;
+  const int L = SN > 0 ? SN / E : a.lp.L;
   if constexpr (MODE == COL_FWD) {
-    line_dft<E, -1, F, ColIdx, SPLIT, BL>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    line_dft<E, -1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   } else if constexpr (MODE == COL_INV) {
-    line_dft<E, +1, F, ColIdx, SPLIT, BL>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    line_dft<E, +1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   } else {
-    line_dft<E, -1, F, ColIdx, SPLIT, BL>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    line_dft<E, -1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
     // spectral filter of ParisPencil.cpp:269-284 at (i, j, kh) = (r, outer, col)
     const int jj = outer, kk = col;
     const double j2 = sqr(double(jj < a.ny - jj ? jj : a.ny - jj) * a.ddj);
@@ -154,7 +159,19 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
       const F df = F(d);
       v[i] = cx<F>{df * v[i].x, df * v[i].y};
     }
-    line_dft<E, +1, F, ColIdx, SPLIT, BL>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    if constexpr (SN > 0) {
+      // With the plan compiled in, the inverse transform's twiddle loads and
+      // LDS addresses are the forward's; left visible, the compiler keeps them
+      // live across the filter and spills.  Opaque copies of the table
+      // pointer and of the thread's position make it recompute them.
+      const cx<F>* tw2 = tw;
+      int r0b = r0;
+      ColIdx idx2 = idx;
+      asm volatile("" : "+s"(tw2), "+v"(r0b), "+v"(idx2.c));
+      line_dft<E, +1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw2, a.chirp, a.bsf, a.bsi, lds, idx2, r0b);
+    } else {
+      line_dft<E, +1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    }
   }
 }
 
@@ -166,14 +183,21 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
 // SPLIT: exchange real and imaginary parts separately through half the LDS.
 // BL: Bluestein lines (nline < lp.n); a separate instantiation so the common
 // kernels carry none of its registers.
-template <typename F, int E, int MODE, bool SPLIT, int G, bool BL>
-__global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
+// SN, ST > 0: line length SN and ST columns per group fixed at compile time
+// (LDS pad shift 4), for the lengths that dominate the solve; the workgroup is
+// then exactly (SN/E)*ST threads and SW the waves per SIMD the register
+// allocation must allow (the LDS-limited occupancy of the tile).
+template <typename F, int E, int MODE, bool SPLIT, int G, bool BL, int SN = 0, int ST = 0, int SW = 1>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, SN > 0 ? SN / E * ST : max_threads_for(E)),
+                          amdgpu_waves_per_eu(SN > 0 ? SW : 1))) void k_col(const ColArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
+  static_assert(SN == 0 || (ST > 0 && (ST & (ST - 1)) == 0), "static column tile");
+  const int T = SN > 0 ? ST : a.T;
   const int tid = threadIdx.x;
-  const int c = tid & (a.T - 1);
-  const int r0 = tid >> a.lgT;
-  const int L = a.lp.L;
+  const int c = tid & (T - 1);
+  const int r0 = SN > 0 ? tid / ST : tid >> a.lgT;
+  const int L = SN > 0 ? SN / E : a.lp.L;
   const int b = xcd_remap(blockIdx.x, a.nblocks);
   int outer, tile;
   if (a.outer_fast) {
@@ -191,7 +215,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
   cx<F> v[G][E];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    const int col = (tile * G + g) * a.T + c;
+    const int col = (tile * G + g) * T + c;
     long long ib, is;
     side_addr(a.in, outer, col < a.ncols ? col : a.ncols - 1, r0, L, ib, is);
     const cx<F>* src = reinterpret_cast<const cx<F>*>(a.src) + ib;
@@ -203,15 +227,15 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_col(const ColArgs a) {
       for (int i = 0; i < E; ++i) v[g][i] = r0 + i * L < a.nline ? src[(long long)i * is] : cx<F>{F(0), F(0)};
     }
   }
-  const ColIdx idx{a.T, c, a.ps};
+  const ColIdx idx{T, c, SN > 0 ? 4 : a.ps};
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    const int col = (tile * G + g) * a.T + c;
-    col_line<F, E, MODE, SPLIT, BL>(v[g], a, tw, lds, idx, r0, outer, col, col < a.ncols);
+    const int col = (tile * G + g) * T + c;
+    col_line<F, E, MODE, SPLIT, BL, SN>(v[g], a, tw, lds, idx, r0, outer, col, col < a.ncols);
   }
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    const int col = (tile * G + g) * a.T + c;
+    const int col = (tile * G + g) * T + c;
     if (col < a.ncols) {
       long long ob, os;
       side_addr(a.out, outer, col, r0, L, ob, os);
@@ -476,9 +500,25 @@ struct Launch {
   hipStream_t stream;
 };
 
+// Column kernels compiled with the line plan of (n, E) and the tile width T
+// fixed (LDS pad shift 4, one group).  At 1024 (E = 16, the tile of the
+// default configuration) the folded indices save 15-20 % of the x pass and a
+// few percent of the y passes on MI355X.  Measured and not kept: 512 (E = 8,
+// three waves per SIMD: the solve as fast as with the generic kernels), other
+// tiles at 1024 (E = 8, or T = 2: slower) and the split exchange at three
+// waves per SIMD (slower).
+struct StaticCol {
+  int n, E, T, f32;
+  int split;  // FF_TUNE_SPLIT_LDS exchange
+  int waves;  // waves per SIMD the registers must allow
+};
+constexpr StaticCol kStaticCols[] = {{1024, 16, 4, 0, 0, 2}, {1024, 16, 8, 1, 0, 2}};
+constexpr int kNumStaticCols = sizeof(kStaticCols) / sizeof(kStaticCols[0]);
+
 template <int E>
 struct PassLaunchers {
-  static hipError_t col(int f32, int mode, int split, int groups, int blue, const Launch& l, const ColArgs& a);
+  static hipError_t col(int f32, int mode, int split, int groups, int blue, int stat, const Launch& l,
+                        const ColArgs& a);
   static hipError_t r2c(int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a);
   static hipError_t c2r(int f32, int out_kind, int odd, int epi, int blue, const Launch& l, const RowArgs& a);
 };

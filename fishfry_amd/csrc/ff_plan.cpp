@@ -384,6 +384,12 @@ bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, i
   out.groups = (flags & FF_TUNE_TWO_GROUPS) ? 2 : 1;
   if (out.groups == 2 && out.line.E != 8 && out.line.E != 16) return false;
   if (out.line.bluestein() && (out.split || out.groups != 1)) return false;  // Bluestein: default shape only
+  out.stat = 0;
+  if (!(flags & FF_TUNE_GENERIC) && out.groups == 1 && !out.line.bluestein() && out.ps == 4)
+    for (const StaticCol& sc : kStaticCols)
+      if (sc.n == n && sc.E == out.line.E && sc.T == T && sc.f32 == (elem_bytes == 8 ? 1 : 0) &&
+          sc.split == out.split)
+        out.stat = 1;
   const size_t ne = (size_t)out.line.lp.n;  // engine length (Bluestein: m >= 2n - 1)
   const size_t elems = ne * T + ((ne - 1) >> out.ps) + 1;
   out.lds = elems * (size_t)(out.split ? elem_bytes / 2 : elem_bytes);
@@ -464,7 +470,8 @@ ff_status tune_set(const ff_tune* t) {
     return FF_OK;
   }
   const int eb = t->precision ? 8 : 16;
-  if (t->flags & ~(FF_TUNE_SPLIT_LDS | FF_TUNE_TWO_GROUPS)) return set_error(FF_ERR_INVALID, "ff_tune_set: unknown flags");
+  if (t->flags & ~(FF_TUNE_SPLIT_LDS | FF_TUNE_TWO_GROUPS | FF_TUNE_GENERIC))
+    return set_error(FF_ERR_INVALID, "ff_tune_set: unknown flags");
   bool ok;
   if (t->kind == 0 || t->kind == 3) {
     RowCfg c;
@@ -488,7 +495,7 @@ void tune_reset() {
 
 #define FF_DECL(E)                                                                                         \
   template <>                                                                                              \
-  hipError_t PassLaunchers<E>::col(int, int, int, int, int, const Launch&, const ColArgs&);                          \
+  hipError_t PassLaunchers<E>::col(int, int, int, int, int, int, const Launch&, const ColArgs&);                     \
   template <>                                                                                              \
   hipError_t PassLaunchers<E>::r2c(int, int, int, int, const Launch&, const RowArgs&);                     \
   template <>                                                                                              \
@@ -528,8 +535,9 @@ FF_DECL(30)
     default: return hipErrorInvalidValue;    \
   }
 
-hipError_t launch_col(int E, int f32, int mode, int split, int groups, int blue, const Launch& l, const ColArgs& a) {
-  FF_SWITCH_E(E, col(f32, mode, split, groups, blue, l, a))
+hipError_t launch_col(int E, int f32, int mode, int split, int groups, int blue, int stat, const Launch& l,
+                      const ColArgs& a) {
+  FF_SWITCH_E(E, col(f32, mode, split, groups, blue, stat, l, a))
 }
 hipError_t launch_r2c(int E, int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a) {
   FF_SWITCH_E(E, r2c(f32, in_f32, odd, blue, l, a))

@@ -325,7 +325,7 @@ ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* sr
   a.bsf = bl[1];
   a.bsi = bl[2];
   const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
-  return hip_check(launch_col(c.line.E, prec, mode, c.split, c.groups, c.line.bluestein(), l, a),
+  return hip_check(launch_col(c.line.E, prec, mode, c.split, c.groups, c.line.bluestein(), c.stat, l, a),
                    "column pass launch");
 }
 

@@ -143,7 +143,7 @@ typedef struct ff_tune {
   int precision;
   int E;
   int T;
-  int flags; /* FF_TUNE_SPLIT_LDS | FF_TUNE_TWO_GROUPS (column passes) */
+  int flags; /* FF_TUNE_SPLIT_LDS | FF_TUNE_TWO_GROUPS | FF_TUNE_GENERIC (column passes) */
 } ff_tune;
 
 /* ff_tune.flags (column passes):
@@ -154,6 +154,11 @@ typedef struct ff_tune {
  *                     (E = 8 or 16)                                          */
 #define FF_TUNE_SPLIT_LDS 1
 #define FF_TUNE_TWO_GROUPS 2
+/* FF_TUNE_GENERIC     the kernel that reads its line plan at run time even
+ *                     where one with the plan of (n, E, T) compiled in exists
+ *                     (ff_tune_query reports FF_TUNE_GENERIC when no such
+ *                     kernel exists for the configuration)                 */
+#define FF_TUNE_GENERIC 4
 
 ff_status ff_tune_set(const ff_tune* t);
 void ff_tune_reset(void);
