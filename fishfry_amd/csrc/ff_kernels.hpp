@@ -162,13 +162,12 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
     if constexpr (SN > 0) {
       // With the plan compiled in, the inverse transform's twiddle loads and
       // LDS addresses are the forward's; left visible, the compiler keeps them
-      // live across the filter and spills.  Opaque copies of the table
-      // pointer and of the thread's position make it recompute them.
-      const cx<F>* tw2 = tw;
+      // live across the filter and spills.  An opaque copy of the thread's
+      // position (from which both derive) makes it recompute them.
       int r0b = r0;
       ColIdx idx2 = idx;
-      asm volatile("" : "+s"(tw2), "+v"(r0b), "+v"(idx2.c));
-      line_dft<E, +1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw2, a.chirp, a.bsf, a.bsi, lds, idx2, r0b);
+      asm volatile("" : "+v"(r0b), "+v"(idx2.c));
+      line_dft<E, +1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx2, r0b);
     } else {
       line_dft<E, +1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
     }
@@ -192,7 +191,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, SN > 0 ? SN / E * ST : 
                           amdgpu_waves_per_eu(SN > 0 ? SW : 1))) void k_col(const ColArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
-  static_assert(SN == 0 || (ST > 0 && (ST & (ST - 1)) == 0), "static column tile");
+  static_assert(SN == 0 || (ST >= 4 && (ST & (ST - 1)) == 0), "static column tile: 4, 8, ... columns");
   const int T = SN > 0 ? ST : a.T;
   const int tid = threadIdx.x;
   const int c = tid & (T - 1);
