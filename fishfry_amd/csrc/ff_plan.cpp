@@ -361,6 +361,28 @@ bool tune_lookup(int kind, int n, int prec, int& E, int& T, int& flags) {
   return true;
 }
 
+// Whether the compile-time plan of a kStaticCols entry for (n, E) is the plan
+// the planner built at run time (the compiled-in kernels must do the same
+// arithmetic as the generic ones).
+template <int K>
+static bool static_plan_matches(int n, int E, const LinePlan& lp) {
+  if constexpr (K < kNumStaticCols) {
+    constexpr StaticCol sc = kStaticCols[K];
+    if (sc.n == n && sc.E == E) {
+      constexpr LinePlan s = static_plan<sc.n, sc.E>();
+      if (s.n != lp.n || s.L != lp.L || s.np != lp.np) return false;
+      for (int p = 0; p < s.np; ++p)
+        if (s.radix[p] != lp.radix[p] || s.ns[p] != lp.ns[p] || s.nsinv[p] != lp.nsinv[p] ||
+            s.twstep[p] != lp.twstep[p])
+          return false;
+      return true;
+    }
+    return static_plan_matches<K + 1>(n, E, lp);
+  } else {
+    return false;
+  }
+}
+
 bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, int flags) {
   if (!plan_line(n, out.line, E)) return false;
   const int L = out.line.lp.L;
@@ -389,7 +411,7 @@ bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, i
     for (const StaticCol& sc : kStaticCols)
       if (sc.n == n && sc.E == out.line.E && sc.T == T && sc.f32 == (elem_bytes == 8 ? 1 : 0) &&
           sc.split == out.split)
-        out.stat = 1;
+        out.stat = static_plan_matches<0>(n, out.line.E, out.line.lp) ? 1 : 0;
   const size_t ne = (size_t)out.line.lp.n;  // engine length (Bluestein: m >= 2n - 1)
   const size_t elems = ne * T + ((ne - 1) >> out.ps) + 1;
   out.lds = elems * (size_t)(out.split ? elem_bytes / 2 : elem_bytes);

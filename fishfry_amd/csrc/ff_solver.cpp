@@ -92,8 +92,8 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   const int nprec = (o.precision == FF_FP64) ? 1 : 2;
   for (int prec = 0; prec < nprec; ++prec) {
     const int eb = prec ? 8 : 16;
-    if (!make_row_cfg(n[2], eb, p->zr[prec], 0, prec) || !make_row_cfg(n[2], eb, p->zi[prec], 3, prec) || !make_col_cfg(n[1], p->nh, eb, p->cy[prec], 1, prec) ||
-        !make_col_cfg(n[0], p->nh, eb, p->cx[prec], 2, prec)) {
+    if (!make_row_cfg(n[2], eb, p->zr[prec], 0, prec) || !make_row_cfg(n[2], eb, p->zi[prec], 3, prec) ||
+        !make_col_cfg(n[1], p->nh, eb, p->cy[prec], 1, prec) || !make_col_cfg(n[0], p->nh, eb, p->cx[prec], 2, prec)) {
       delete p;
       char buf[160];
       snprintf(buf, sizeof buf,
@@ -101,6 +101,21 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
                "workgroup's line transform",
                n[0], n[1], n[2]);
       return set_error(FF_ERR_UNSUPPORTED, buf);
+    }
+  }
+
+  // one workgroup per tile: every pass's grid must fit a 32-bit grid size
+  for (int prec = 0; prec < nprec; ++prec) {
+    const long long rows = (long long)n[0] * n[1];
+    const long long rb =
+        std::max((rows + p->zr[prec].T - 1) / p->zr[prec].T, (rows + p->zi[prec].T - 1) / p->zi[prec].T);
+    auto col_blocks = [&](const ColCfg& c, int nouter) {
+      const long long tiles = (p->nh + (long long)c.groups * c.T - 1) / ((long long)c.groups * c.T);
+      return (long long)nouter * tiles;
+    };
+    if (rb > INT_MAX || col_blocks(p->cy[prec], n[0]) > INT_MAX || col_blocks(p->cx[prec], n[1]) > INT_MAX) {
+      delete p;
+      return set_error(FF_ERR_UNSUPPORTED, "grid too large: a pass would need more than 2^31-1 workgroups");
     }
   }
 

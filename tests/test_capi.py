@@ -105,3 +105,33 @@ def test_persistent_batch_is_rejected_without_gpu_work():
         ff.Batch([{"n": (8, 8, 8)}], ff.FF_BATCH_PERSISTENT)
     assert e.value.status == ff.FF_ERR_UNSUPPORTED
     assert "persistent" in str(e.value)
+
+
+def test_tuning_host_logic_without_gpu_work():
+    """ff_tune_query / ff_tune_set are host logic: the default configurations
+    of the 1024-long column passes run the kernels with the line plan compiled
+    in (flags without FF_TUNE_GENERIC -- which also needs the compile-time plan
+    to equal the planner's), other lengths run the generic kernel, and the
+    overrides round-trip."""
+    ff.tune_reset()
+    try:
+        for kind in (1, 2):
+            t = ff.tune_query(kind, 1024, 0, 513)
+            assert (t.E, t.T, t.flags) == (16, 4, 0)
+            t = ff.tune_query(kind, 1024, 1, 513)
+            assert (t.E, t.T, t.flags) == (16, 8, 0)
+            assert ff.tune_query(kind, 512, 0, 257).flags & ff.FF_TUNE_GENERIC
+            assert ff.tune_query(kind, 1000, 0, 501).flags & ff.FF_TUNE_GENERIC
+        ff.tune_set(2, 1024, 0, 16, 4, ff.FF_TUNE_GENERIC)
+        assert ff.tune_query(2, 1024, 0, 513).flags == ff.FF_TUNE_GENERIC
+        assert ff.tune_query(1, 1024, 0, 513).flags == 0
+        ff.tune_set(2, 1024, 0, 16, 4, ff.FF_TUNE_SPLIT_LDS)
+        assert ff.tune_query(2, 1024, 0, 513).flags == ff.FF_TUNE_SPLIT_LDS | ff.FF_TUNE_GENERIC
+        with pytest.raises(ff.FishFryError):
+            ff.tune_set(2, 1024, 0, 16, 4, 8)  # unknown flag
+        with pytest.raises(ff.FishFryError):
+            ff.tune_set(0, 512, 0, 8, 2, ff.FF_TUNE_GENERIC)  # row passes take no flags
+        ff.tune_reset()
+        assert ff.tune_query(2, 1024, 0, 513).flags == 0
+    finally:
+        ff.tune_reset()
