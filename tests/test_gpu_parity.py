@@ -107,7 +107,7 @@ def gpu_solve(f, **kw):
 
 
 PASS_SHAPES = [(16, 24, 20), (32, 32, 32), (12, 10, 9), (64, 48, 40), (5, 6, 4), (9, 8, 12), (2, 8, 16),
-               (30, 25, 27), (32, 1024, 512), (24, 600, 1000), (1024, 8, 6), (512, 512, 10),
+               (30, 25, 27), (32, 1024, 512), (24, 600, 1000), (1024, 8, 6), (512, 512, 10), (8, 6, 1024),
                # prime factors above 5 (Bluestein lines): odd and even nz, nz/2 prime
                (7, 11, 13), (14, 22, 26), (37, 20, 18), (12, 10, 14), (13, 509, 8), (4, 6, 1018)]
 
