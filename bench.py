@@ -143,8 +143,9 @@ def bench_sizes(ff, torch, sizes, K):
 def bench_batching(ff, torch, K):
     """The batching configuration: 64 grids drawn (seeded) from {32^3, 48^3,
     64^3, 96^3, 128^3, 160^3, 128x64x96}, f_b(x) = s_b f(x + delta_b), one
-    HIP-graph batch per precision, autotuned; aggregate points/s and the worst
-    Linf against phi_b = s_b phi(x + delta_b)."""
+    HIP-graph batch per precision, autotuned, then the same batch grouped (one
+    launch per pass for each shape); aggregate points/s and the worst Linf
+    against phi_b = s_b phi(x + delta_b)."""
     import numpy as np
 
     sizes = [(32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 128, 128), (160, 160, 160),
@@ -183,12 +184,33 @@ def bench_batching(ff, torch, K):
         b.wait(stream)
         stream.synchronize()
         ms = 1e3 * (time.perf_counter() - t0) / K
+        b.close()
+        # the same batch grouped: one launch per pass for each shape (tuned configurations stay in force)
+        g = ff.Batch(items, ff.FF_BATCH_GRAPH | ff.FF_BATCH_GROUPED)
+        for k in range(64):
+            ff.field_init(fields[k], d[k])
+        torch.cuda.synchronize()
+        g.solve(d, p, stream)
+        g.wait(stream)
+        stream.synchronize()
+        gworst = max(ff.field_norms(fields[k], p[k])[2] for k in range(64))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            g.solve(d, p, stream)
+        g.wait(stream)
+        stream.synchronize()
+        gms = 1e3 * (time.perf_counter() - t0) / K
+        ngroups = len(json.loads(g.describe())["groups"])
+        g.close()
         out.append({"precision": "fp64" if precision == ff.FF_FP64 else "fp32+2", "grids": 64, "points": points,
                     "ms_per_batch": round(ms, 4), "points_per_s": points / (ms * 1e-3), "worst_linf": worst,
-                    "autotuned_passes": tuned, "ms_before_tuning": round(before, 4), "ms_after_tuning": round(after, 4)})
+                    "autotuned_passes": tuned, "ms_before_tuning": round(before, 4), "ms_after_tuning": round(after, 4),
+                    "grouped": {"groups": ngroups, "ms_per_batch": round(gms, 4), "points_per_s": points / (gms * 1e-3),
+                                "worst_linf": gworst}})
         log(f"batch {out[-1]['precision']}: {ms:.3f} ms per 64-grid batch, {points / (ms * 1e-3):.4e} points/s, "
-            f"worst Linf {worst:.3e}, {tuned} passes tuned ({before:.3f} -> {after:.3f} ms)")
-        b.close()
+            f"worst Linf {worst:.3e}, {tuned} passes tuned ({before:.3f} -> {after:.3f} ms); grouped ({ngroups} "
+            f"groups): {gms:.3f} ms, {points / (gms * 1e-3):.4e} points/s, worst Linf {gworst:.3e}")
         del d, p
         torch.cuda.empty_cache()
     return out

@@ -398,9 +398,23 @@ using namespace ff;
 // FF_BATCH_GRAPH the whole fork/join sequence is captured once as a HIP graph
 // and replayed while the buffers stay the same.
 //
+// With FF_BATCH_GROUPED, instances of one shape, precision and refinement
+// count form a group that runs each of its passes as ONE launch, grid row
+// blockIdx.y being the instance: the pass arguments of the group's instances
+// are equal except for their buffers and domain constants, which the kernels
+// read from a per-instance table (PassInst) in device memory.  The launch
+// sequence comes from recording one ordinary solve per instance (the
+// launchers' recorder), so a group runs exactly the kernels and arithmetic of
+// its instances' single solves.
+//
 // The single persistent launch over every instance (BatchCore, k_batch) is
 // switched off: its kernel did not complete on MI355X and the cause is not
 // established, so FF_BATCH_PERSISTENT is rejected.
+struct BatchGroup {
+  std::vector<int> inst;         // instances, in batch order
+  std::vector<PassLaunch> ops;   // the group's launches (per-instance fields from the table)
+};
+
 struct ff_batch {
   int count = 0;
   int flags = 0;
@@ -411,6 +425,13 @@ struct ff_batch {
   hipEvent_t fork = nullptr;
   std::vector<double*> bd, bp;
   bool bound = false;
+  // FF_BATCH_GROUPED: groups and their per-instance tables, valid for the
+  // plans and the buffers gd / gp they were recorded with
+  std::vector<BatchGroup> groups;
+  void* inst_dev = nullptr;
+  size_t inst_cap = 0;
+  bool groups_ok = false;
+  std::vector<double*> gd, gp;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -434,6 +455,7 @@ void destroy_plans(ff_batch* b) {
 
 ff_status make_plans(ff_batch* b) {
   destroy_plans(b);
+  b->groups_ok = false;
   for (const ff_batch_item& it : b->items) {
     ff_options o{it.precision, it.refine_steps, 0};
     ff_plan* p = nullptr;
@@ -444,12 +466,135 @@ ff_status make_plans(ff_batch* b) {
   return FF_OK;
 }
 
-// Fork onto the batch's streams, enqueue every instance, join back onto s.
+// The per-instance fields of a recorded launch.
+PassInst inst_fields(const PassLaunch& r) {
+  if (r.kind == 0)
+    return PassInst{r.col.src, r.col.dst, nullptr, r.col.dc_out, nullptr, r.col.ddi, r.col.ddj, r.col.ddk};
+  return PassInst{r.row.src, r.row.dst, r.row.aux, nullptr, r.row.shiftp, 0.0, 0.0, 0.0};
+}
+
+// Whether two recorded launches differ only in their per-instance fields and
+// stream (the recorded argument blocks are zero-filled before use, so their
+// bytes compare).
+bool same_launch(const PassLaunch& x, const PassLaunch& y) {
+  PassLaunch a, b;
+  memcpy(&a, &x, sizeof a);
+  memcpy(&b, &y, sizeof b);
+  for (PassLaunch* r : {&a, &b}) {
+    r->l.stream = nullptr;
+    r->col.src = nullptr;
+    r->col.dst = nullptr;
+    r->col.dc_out = nullptr;
+    r->col.ddi = r->col.ddj = r->col.ddk = 0.0;
+    r->row.src = nullptr;
+    r->row.dst = nullptr;
+    r->row.aux = nullptr;
+    r->row.shiftp = nullptr;
+  }
+  return memcmp(&a, &b, sizeof a) == 0;
+}
+
+// Records one solve per instance, forms the groups and uploads their
+// per-instance tables (blocking: waits for the device before rewriting them).
+ff_status prepare_groups(ff_batch* b, double* const* d, double* const* p) {
+  std::vector<std::vector<PassLaunch>> rec(b->count);
+  for (int i = 0; i < b->count; ++i) {
+    set_launch_recorder(&rec[i]);
+    const ff_status st = solve_enqueue(b->plans[i], d[i], p[i], nullptr, nullptr);
+    set_launch_recorder(nullptr);
+    if (st != FF_OK) return st;
+  }
+  std::vector<BatchGroup> groups;
+  for (int i = 0; i < b->count; ++i) {
+    const ff_batch_item& it = b->items[i];
+    BatchGroup* home = nullptr;
+    for (BatchGroup& g : groups) {
+      const int l = g.inst[0];
+      const ff_batch_item& lt = b->items[l];
+      if (lt.n[0] != it.n[0] || lt.n[1] != it.n[1] || lt.n[2] != it.n[2] || lt.precision != it.precision ||
+          lt.refine_steps != it.refine_steps || rec[l].size() != rec[i].size())
+        continue;
+      bool same = true;
+      for (size_t k = 0; k < rec[i].size() && same; ++k) same = same_launch(rec[l][k], rec[i][k]);
+      if (same) {
+        home = &g;
+        break;
+      }
+    }
+    if (!home) {
+      groups.emplace_back();
+      home = &groups.back();
+    }
+    home->inst.push_back(i);
+  }
+  std::vector<PassInst> tab;
+  std::vector<size_t> off;
+  for (const BatchGroup& g : groups) {
+    off.push_back(tab.size());
+    for (size_t k = 0; k < rec[g.inst[0]].size(); ++k)
+      for (int i : g.inst) tab.push_back(inst_fields(rec[i][k]));
+  }
+  // kernels of earlier solves may still read the old tables
+  ff_status st = hip_check(hipDeviceSynchronize(), "batch table sync");
+  const size_t bytes = std::max<size_t>(1, tab.size()) * sizeof(PassInst);
+  if (st == FF_OK && bytes > b->inst_cap) {
+    if (b->inst_dev) (void)hipFree(b->inst_dev);
+    b->inst_dev = nullptr;
+    b->inst_cap = 0;
+    st = hip_check(hipMalloc(&b->inst_dev, bytes), "batch table hipMalloc");
+    if (st == FF_OK) b->inst_cap = bytes;
+  }
+  if (st == FF_OK && !tab.empty())
+    st = hip_check(hipMemcpy(b->inst_dev, tab.data(), tab.size() * sizeof(PassInst), hipMemcpyHostToDevice),
+                   "batch table upload");
+  if (st != FF_OK) return st;
+  const PassInst* base = static_cast<const PassInst*>(b->inst_dev);
+  for (size_t gi = 0; gi < groups.size(); ++gi) {
+    BatchGroup& g = groups[gi];
+    const std::vector<PassLaunch>& lead = rec[g.inst[0]];
+    for (size_t k = 0; k < lead.size(); ++k) {
+      PassLaunch r = lead[k];
+      r.l.grid.y = (unsigned)g.inst.size();
+      const PassInst* t = base + off[gi] + k * g.inst.size();
+      if (r.kind == 0) r.col.inst = t;
+      else r.row.inst = t;
+      g.ops.push_back(r);
+    }
+  }
+  b->groups.swap(groups);
+  b->gd.assign(d, d + b->count);
+  b->gp.assign(p, p + b->count);
+  b->groups_ok = true;
+  return FF_OK;
+}
+
+bool groups_current(const ff_batch* b, double* const* d, double* const* p) {
+  return b->groups_ok && std::equal(b->gd.begin(), b->gd.end(), d) && std::equal(b->gp.begin(), b->gp.end(), p);
+}
+
+// Fork onto the batch's streams, enqueue every instance (or every group),
+// join back onto s.
 ff_status enqueue_all(ff_batch* b, double* const* d, double* const* p, hipStream_t s) {
+  const bool grouped = (b->flags & FF_BATCH_GROUPED) != 0;
+  if (grouped && !groups_current(b, d, p)) {
+    ff_status st = prepare_groups(b, d, p);
+    if (st != FF_OK) return st;
+  }
   ff_status st = hip_check(hipEventRecord(b->fork, s), "batch fork");
   for (size_t k = 0; k < b->streams.size() && st == FF_OK; ++k)
     st = hip_check(hipStreamWaitEvent(b->streams[k], b->fork, 0), "batch fork wait");
-  for (int i = 0; i < b->count && st == FF_OK; ++i)
+  if (grouped) {
+    for (size_t gi = 0; gi < b->groups.size() && st == FF_OK; ++gi) {
+      hipStream_t hs = b->streams[gi % b->streams.size()];
+      for (const PassLaunch& op : b->groups[gi].ops) {
+        PassLaunch r = op;
+        r.l.stream = hs;
+        st = hip_check(replay_launch(r), "grouped pass launch");
+        if (st != FF_OK) break;
+      }
+    }
+  }
+  for (int i = 0; i < b->count && st == FF_OK && !grouped; ++i)
     st = solve_enqueue(b->plans[i], d[i], p[i], b->streams[i % b->streams.size()], nullptr);
   for (size_t k = 0; k < b->streams.size() && st == FF_OK; ++k) {
     st = hip_check(hipEventRecord(b->joins[k], b->streams[k]), "batch join");
@@ -471,7 +616,7 @@ ff_status ff_batch_create(int count, const ff_batch_item* items, int flags, ff_b
     return set_error(FF_ERR_UNSUPPORTED,
                      "ff_batch_create: the single persistent launch is disabled in this build (its kernel did not "
                      "complete on MI355X); use the default stream-parallel batch");
-  if (flags & ~FF_BATCH_GRAPH) return set_error(FF_ERR_INVALID, "ff_batch_create: bad flags");
+  if (flags & ~(FF_BATCH_GRAPH | FF_BATCH_GROUPED)) return set_error(FF_ERR_INVALID, "ff_batch_create: bad flags");
   for (int i = 0; i < count; ++i)
     for (int a = 0; a < 3; ++a)
       if (items[i].n[a] < 2) return set_error(FF_ERR_INVALID, "ff_batch_create: extents must be >= 2");
@@ -517,6 +662,11 @@ ff_status ff_batch_solve(ff_batch* b, double* const* density, double* const* pot
                     std::equal(b->bp.begin(), b->bp.end(), potential);
   if (!same || !b->exec || b->graph_stream != s) {
     drop_graph(b);
+    // grouped tables are recorded and uploaded before capture (blocking calls)
+    if ((b->flags & FF_BATCH_GROUPED) && !groups_current(b, density, potential)) {
+      ff_status st = prepare_groups(b, density, potential);
+      if (st != FF_OK) return st;
+    }
     ff_status st = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
     if (st != FF_OK) return st;
     st = enqueue_all(b, density, potential, s);
@@ -659,7 +809,12 @@ ff_status ff_batch_describe(const ff_batch* b, char* buf, size_t len) {
   if (!b || !buf || !len) return set_error(FF_ERR_INVALID, "ff_batch_describe: bad arguments");
   std::ostringstream os;
   os << "{\"count\":" << b->count << ",\"flags\":" << b->flags << ",\"streams\":" << b->streams.size()
-     << ",\"graph\":" << (b->exec ? 1 : 0) << ",\"plans\":[";
+     << ",\"graph\":" << (b->exec ? 1 : 0) << ",\"groups\":[";
+  for (size_t g = 0; g < b->groups.size(); ++g) {
+    os << (g ? "," : "") << "{\"instances\":" << b->groups[g].inst.size() << ",\"launches\":" << b->groups[g].ops.size()
+       << "}";
+  }
+  os << "],\"plans\":[";
   char tmp[4096];
   for (int i = 0; i < b->count; ++i) {
     if (ff_plan_describe(b->plans[i], tmp, sizeof tmp) != FF_OK) return FF_ERR_INVALID;
@@ -677,6 +832,7 @@ ff_status ff_batch_destroy(ff_batch* b) {
   drop_graph(b);
   for (hipStream_t s : b->streams) (void)hipStreamSynchronize(s);
   destroy_plans(b);
+  if (b->inst_dev) (void)hipFree(b->inst_dev);
   for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
   for (hipEvent_t e : b->joins) (void)hipEventDestroy(e);
   if (b->fork) (void)hipEventDestroy(b->fork);

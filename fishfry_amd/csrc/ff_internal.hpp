@@ -92,6 +92,20 @@ void tune_reset();
 
 hipError_t launch_col(int E, int f32, int mode, int split, int groups, int blue, int stat, const Launch& l,
                       const ColArgs& a);
+
+// Launch recording.  While a recorder is installed on the calling thread, the
+// three pass launchers append their launches to it instead of launching; a
+// recorded launch can be replayed later, e.g. with a per-instance table and
+// one grid row per instance (the batch's grouped mode).
+struct PassLaunch {
+  int kind;  // 0 column pass, 1 z forward, 2 z inverse
+  int E, f32, sel, split, groups, blue, stat, odd, epi;  // sel: mode / in_f32 / out_kind
+  Launch l;
+  ColArgs col;
+  RowArgs row;
+};
+void set_launch_recorder(std::vector<PassLaunch>* rec);
+hipError_t replay_launch(const PassLaunch& r);
 hipError_t launch_r2c(int E, int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a);
 hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, const Launch& l, const RowArgs& a);
 

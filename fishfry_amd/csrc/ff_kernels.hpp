@@ -57,6 +57,19 @@ __device__ __forceinline__ void side_addr(const SpecSide& s, int outer, int col,
   step = (long long)L * s.fr * W;
 }
 
+// Per-instance fields of a grouped launch: grid row blockIdx.y runs instance
+// blockIdx.y of a group of solves of one shape, precision and refinement
+// count, whose other pass arguments are all equal (their buffers and domain
+// constants differ).
+struct PassInst {
+  const void* src;
+  void* dst;
+  const void* aux;       // rows: f of the residual epilogue
+  double* dc_out;        // columns: sum of the input
+  const double* shiftp;  // rows: sum of phi
+  double ddi, ddj, ddk;  // columns: filter constants
+};
+
 struct ColArgs {
   const void* src;
   void* dst;
@@ -75,7 +88,49 @@ struct ColArgs {
   LinePlan lp;
   int nline;                     // transform length (lp.n: engine length; larger for Bluestein)
   const void *chirp, *bsf, *bsi;  // Bluestein tables (see line_dft)
+  const PassInst* inst;           // grouped launch: per-instance fields (null: one solve)
 };
+
+// Buffers in global memory.  The per-instance fields come either from the
+// kernel arguments or from a table in memory, so the compiler cannot infer
+// their address space (it would emit flat accesses): the kernels address
+// them through these global-space views.
+// (The host pass of the HIP compilation parses these functions too; address
+// spaces exist only in the device pass.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FF_GLB __attribute__((address_space(1)))
+#else
+#define FF_GLB
+#endif
+template <typename T>
+__device__ __forceinline__ FF_GLB T* glb(const void* p) {
+  return (FF_GLB T*)(p);
+}
+
+// The per-instance fields are the same for the whole workgroup: keeping them
+// in scalar registers keeps the address arithmetic scalar as well.
+__device__ __forceinline__ unsigned long long uniform64(unsigned long long v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+template <typename P>
+__device__ __forceinline__ P uniform_ptr(P p) {
+  return reinterpret_cast<P>(uniform64(reinterpret_cast<unsigned long long>(p)));
+}
+__device__ __forceinline__ double uniform_dbl(double x) {
+  return __longlong_as_double((long long)uniform64((unsigned long long)__double_as_longlong(x)));
+}
+__device__ __forceinline__ PassInst uniform_inst(const PassInst& t) {
+  return PassInst{uniform_ptr(t.src),    uniform_ptr(t.dst),    uniform_ptr(t.aux), uniform_ptr(t.dc_out),
+                  uniform_ptr(t.shiftp), uniform_dbl(t.ddi), uniform_dbl(t.ddj), uniform_dbl(t.ddk)};
+}
+
+// The per-instance fields of this workgroup's solve.
+__device__ __forceinline__ PassInst inst_of(const ColArgs& a) {
+  if (a.inst) return uniform_inst(glb<const PassInst>(a.inst)[blockIdx.y]);
+  return PassInst{a.src, a.dst, nullptr, a.dc_out, nullptr, a.ddi, a.ddj, a.ddk};
+}
 
 // MODE: 0 forward, 1 inverse, 2 forward * filter * inverse (the solve),
 //       3 forward * symbol * inverse (applies L_h for a residual).
@@ -134,8 +189,9 @@ __device__ __forceinline__ void line_dft(cx<F> (&v)[E], const LinePlan& lp, int 
 // One column line held in registers through the transform, filter or symbol
 // and the inverse transform of MODE.
 template <typename F, int E, int MODE, bool SPLIT, bool BL, int SN>
-__device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const cx<F>* __restrict__ tw, cx<F>* lds,
-                                         const ColIdx& idx, int r0, int outer, int col, bool valid) {
+__device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const PassInst& io,
+                                         const cx<F>* __restrict__ tw, cx<F>* lds, const ColIdx& idx, int r0,
+                                         int outer, int col, bool valid) {
   const int L = SN > 0 ? SN / E : a.lp.L;
   if constexpr (MODE == COL_FWD) {
     line_dft<E, -1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
@@ -145,14 +201,14 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
     line_dft<E, -1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
     // spectral filter of ParisPencil.cpp:269-284 at (i, j, kh) = (r, outer, col)
     const int jj = outer, kk = col;
-    const double j2 = sqr(double(jj < a.ny - jj ? jj : a.ny - jj) * a.ddj);
-    const double k2 = sqr(double(kk) * a.ddk);
+    const double j2 = sqr(double(jj < a.ny - jj ? jj : a.ny - jj) * io.ddj);
+    const double k2 = sqr(double(kk) * io.ddk);
     // the (0,0,0) term before the multiply is the sum of the input field
-    if (a.dc_out && valid && r0 == 0 && jj == 0 && kk == 0) *a.dc_out = double(v[0].x);
+    if (io.dc_out && valid && r0 == 0 && jj == 0 && kk == 0) *glb<double>(io.dc_out) = double(v[0].x);
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int ii = r0 + i * L;
-      const double i2 = sqr(double(ii < a.nx - ii ? ii : a.nx - ii) * a.ddi);
+      const double i2 = sqr(double(ii < a.nx - ii ? ii : a.nx - ii) * io.ddi);
       double d;
       if constexpr (MODE == COL_FILTER) d = (ii | jj | kk) ? -1.0 / (i2 + j2 + k2) : 0.0;
       else d = -(i2 + j2 + k2);
@@ -189,6 +245,7 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
 template <typename F, int E, int MODE, bool SPLIT, int G, bool BL, int SN = 0, int ST = 0, int SW = 1>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, SN > 0 ? SN / E * ST : max_threads_for(E)),
                           amdgpu_waves_per_eu(SN > 0 ? SW : 1))) void k_col(const ColArgs a) {
+  const PassInst io = inst_of(a);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
   static_assert(SN == 0 || (ST >= 4 && (ST & (ST - 1)) == 0), "static column tile: 4, 8, ... columns");
@@ -217,20 +274,20 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, SN > 0 ? SN / E * ST : 
     const int col = (tile * G + g) * T + c;
     long long ib, is;
     side_addr(a.in, outer, col < a.ncols ? col : a.ncols - 1, r0, L, ib, is);
-    const cx<F>* src = reinterpret_cast<const cx<F>*>(a.src) + ib;
+    const FF_GLB cx<F>* src = glb<const cx<F>>(io.src) + ib;
     if constexpr (!BL) {
 #pragma unroll
       for (int i = 0; i < E; ++i) v[g][i] = src[(long long)i * is];
     } else {  // Bluestein: the engine line is longer than the data line
 #pragma unroll
-      for (int i = 0; i < E; ++i) v[g][i] = r0 + i * L < a.nline ? src[(long long)i * is] : cx<F>{F(0), F(0)};
+      for (int i = 0; i < E; ++i) v[g][i] = r0 + i * L < a.nline ? cx<F>(src[(long long)i * is]) : cx<F>{F(0), F(0)};
     }
   }
   const ColIdx idx{T, c, SN > 0 ? 4 : a.ps};
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     const int col = (tile * G + g) * T + c;
-    col_line<F, E, MODE, SPLIT, BL, SN>(v[g], a, tw, lds, idx, r0, outer, col, col < a.ncols);
+    col_line<F, E, MODE, SPLIT, BL, SN>(v[g], a, io, tw, lds, idx, r0, outer, col, col < a.ncols);
   }
 #pragma unroll
   for (int g = 0; g < G; ++g) {
@@ -238,7 +295,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, SN > 0 ? SN / E * ST : 
     if (col < a.ncols) {
       long long ob, os;
       side_addr(a.out, outer, col, r0, L, ob, os);
-      cx<F>* dst = reinterpret_cast<cx<F>*>(a.dst) + ob;
+      FF_GLB cx<F>* dst = glb<cx<F>>(io.dst) + ob;
 #pragma unroll
       for (int i = 0; i < E; ++i)
         if (!BL || r0 + i * L < a.nline) dst[(long long)i * os] = v[g][i];
@@ -265,7 +322,13 @@ struct RowArgs {
   LinePlan lp;
   int nline;                     // transform length M (lp.n: engine length; larger for Bluestein)
   const void *chirp, *bsf, *bsi;  // Bluestein tables (see line_dft)
+  const PassInst* inst;           // grouped launch: per-instance fields (null: one solve)
 };
+
+__device__ __forceinline__ PassInst inst_of(const RowArgs& a) {
+  if (a.inst) return uniform_inst(glb<const PassInst>(a.inst)[blockIdx.y]);
+  return PassInst{a.src, a.dst, a.aux, nullptr, a.shiftp, 0.0, 0.0, 0.0};
+}
 
 template <typename T>
 __device__ __forceinline__ double to_d(T x) { return double(x); }
@@ -278,6 +341,7 @@ __device__ __forceinline__ long long spec_row(const RowArgs& a, long long row) {
 // split the even/odd spectra.  Odd nz: a full nz-point transform of (x, 0).
 template <typename F, typename IN, int E, bool ODD, bool BL>
 __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
+  const PassInst io = inst_of(a);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
   const int tid = threadIdx.x;
@@ -293,16 +357,16 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
   const RowIdx idx{line * a.pitch, a.ps};
   cx<F> v[E];
   if constexpr (!ODD) {
-    const IN* __restrict__ src = reinterpret_cast<const IN*>(a.src) + rowc * a.nz;
+    const FF_GLB IN* __restrict__ src = glb<const IN>(io.src) + rowc * a.nz;
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int k = r0 + i * L;
       if (!blue || k < M) {
         if constexpr (sizeof(IN) == 8) {
-          const cx<double> p = reinterpret_cast<const cx<double>*>(src)[k];
+          const cx<double> p = ((const FF_GLB cx<double>*)src)[k];
           v[i] = cx<F>{F(p.x), F(p.y)};
         } else {
-          const cx<float> p = reinterpret_cast<const cx<float>*>(src)[k];
+          const cx<float> p = ((const FF_GLB cx<float>*)src)[k];
           v[i] = cx<F>{F(p.x), F(p.y)};
         }
       } else {
@@ -314,7 +378,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
     put_natural<E, F>(v, lds, idx, r0, L);
     __syncthreads();
     const cx<F>* __restrict__ twp = reinterpret_cast<const cx<F>*>(a.twpost);
-    cx<F>* __restrict__ dst = reinterpret_cast<cx<F>*>(a.dst) + (valid ? spec_row(a, row) : 0);
+    FF_GLB cx<F>* __restrict__ dst = glb<cx<F>>(io.dst) + (valid ? spec_row(a, row) : 0);
     if (valid) {
 #pragma unroll
       for (int i = 0; i < E; ++i) {
@@ -333,14 +397,14 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
       }
     }
   } else {
-    const IN* __restrict__ src = reinterpret_cast<const IN*>(a.src) + rowc * a.nz;
+    const FF_GLB IN* __restrict__ src = glb<const IN>(io.src) + rowc * a.nz;
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int k = r0 + i * L;
       v[i] = cx<F>{(!blue || k < M) ? F(src[k]) : F(0), F(0)};
     }
     line_dft<E, -1, F, RowIdx, false, BL>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
-    cx<F>* __restrict__ dst = reinterpret_cast<cx<F>*>(a.dst) + (valid ? spec_row(a, row) : 0);
+    FF_GLB cx<F>* __restrict__ dst = glb<cx<F>>(io.dst) + (valid ? spec_row(a, row) : 0);
     if (valid) {
 #pragma unroll
       for (int i = 0; i < E; ++i) {
@@ -356,8 +420,8 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
 enum { EPI_STORE = 0, EPI_RESIDUAL = 1, EPI_ACCUM = 2 };
 
 template <int EPI, typename OUT>
-__device__ __forceinline__ void c2r_out1(OUT* out, const double* f, long long off, double val, double scale,
-                                         double shift) {
+__device__ __forceinline__ void c2r_out1(FF_GLB OUT* out, const FF_GLB double* f, long long off, double val,
+                                         double scale, double shift) {
   if constexpr (EPI == EPI_STORE) {
     out[off] = OUT(scale * val);
   } else if constexpr (EPI == EPI_RESIDUAL) {
@@ -369,13 +433,13 @@ __device__ __forceinline__ void c2r_out1(OUT* out, const double* f, long long of
 
 // Two neighbouring outputs (2r, 2r+1) with one vector access.
 template <int EPI, typename OUT>
-__device__ __forceinline__ void c2r_out2(OUT* out, const double* f, long long r, double v0, double v1,
-                                         double scale, double shift) {
-  cx<OUT>* o = reinterpret_cast<cx<OUT>*>(out) + r;
+__device__ __forceinline__ void c2r_out2(FF_GLB OUT* out, const FF_GLB double* f, long long r, double v0,
+                                         double v1, double scale, double shift) {
+  FF_GLB cx<OUT>* o = (FF_GLB cx<OUT>*)out + r;
   if constexpr (EPI == EPI_STORE) {
     *o = cx<OUT>{OUT(scale * v0), OUT(scale * v1)};
   } else if constexpr (EPI == EPI_RESIDUAL) {
-    const cx<double> fp = reinterpret_cast<const cx<double>*>(f)[r];
+    const cx<double> fp = ((const FF_GLB cx<double>*)f)[r];
     *o = cx<OUT>{OUT((fp.x - shift) - scale * v0), OUT((fp.y - shift) - scale * v1)};
   } else {
     const cx<OUT> p = *o;
@@ -388,6 +452,7 @@ __device__ __forceinline__ void c2r_out2(OUT* out, const double* f, long long r,
 // i.e. the rows are read as Hermitian half spectra.
 template <typename F, typename OUT, int E, bool ODD, int EPI, bool BL>
 __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
+  const PassInst io = inst_of(a);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
   const int tid = threadIdx.x;
@@ -403,7 +468,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
   // stage the half spectrum of the workgroup's rows in LDS; rows past the end
   // load the last row (in bounds), their stores are predicated
   if (a.spec.lgKT >= 30) {
-    const cx<F>* __restrict__ src = reinterpret_cast<const cx<F>*>(a.src) + spec_row(a, valid ? row : a.nrows - 1);
+    const FF_GLB cx<F>* __restrict__ src = glb<const cx<F>>(io.src) + spec_row(a, valid ? row : a.nrows - 1);
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int k = r0 + i * L;
@@ -415,7 +480,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
     // y-blocked spectrum: the workgroup's T rows are contiguous inside each
     // kh block (T*KT elements), so all threads copy element e of the tile in
     // (block, row, kt) order
-    const cx<F>* __restrict__ src = reinterpret_cast<const cx<F>*>(a.src);
+    const FF_GLB cx<F>* __restrict__ src = glb<const cx<F>>(io.src);
     const long long row0 = (long long)xcd_remap(blockIdx.x, a.nblocks) * a.T;
     const int KT = 1 << a.spec.lgKT;
     const int TK = a.T * KT;
@@ -474,9 +539,9 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
   }
   line_dft<E, +1, F, RowIdx, false, BL>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   if (valid) {
-    const double shift = a.shiftp ? a.shift * *a.shiftp : a.shift;
-    OUT* out = reinterpret_cast<OUT*>(a.dst) + row * a.nz;
-    const double* f = reinterpret_cast<const double*>(a.aux) + (EPI == EPI_RESIDUAL ? row * a.nz : 0);
+    const double shift = io.shiftp ? a.shift * *glb<const double>(io.shiftp) : a.shift;
+    FF_GLB OUT* out = glb<OUT>(io.dst) + row * a.nz;
+    const FF_GLB double* f = glb<const double>(io.aux) + (EPI == EPI_RESIDUAL ? row * a.nz : 0);
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int r = r0 + i * L;

@@ -557,15 +557,60 @@ FF_DECL(30)
     default: return hipErrorInvalidValue;    \
   }
 
-hipError_t launch_col(int E, int f32, int mode, int split, int groups, int blue, int stat, const Launch& l,
-                      const ColArgs& a) {
+static thread_local std::vector<PassLaunch>* t_recorder = nullptr;
+
+void set_launch_recorder(std::vector<PassLaunch>* rec) { t_recorder = rec; }
+
+static hipError_t do_col(int E, int f32, int mode, int split, int groups, int blue, int stat, const Launch& l,
+                         const ColArgs& a) {
   FF_SWITCH_E(E, col(f32, mode, split, groups, blue, stat, l, a))
 }
-hipError_t launch_r2c(int E, int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a) {
+static hipError_t do_r2c(int E, int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a) {
   FF_SWITCH_E(E, r2c(f32, in_f32, odd, blue, l, a))
 }
-hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, const Launch& l, const RowArgs& a) {
+static hipError_t do_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, const Launch& l,
+                         const RowArgs& a) {
   FF_SWITCH_E(E, c2r(f32, out_kind, odd, epi, blue, l, a))
+}
+
+hipError_t launch_col(int E, int f32, int mode, int split, int groups, int blue, int stat, const Launch& l,
+                      const ColArgs& a) {
+  if (t_recorder) {
+    PassLaunch r{};
+    r.kind = 0, r.E = E, r.f32 = f32, r.sel = mode, r.split = split, r.groups = groups, r.blue = blue;
+    r.stat = stat, r.l = l, r.col = a;
+    t_recorder->push_back(r);
+    return hipSuccess;
+  }
+  return do_col(E, f32, mode, split, groups, blue, stat, l, a);
+}
+hipError_t launch_r2c(int E, int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a) {
+  if (t_recorder) {
+    PassLaunch r{};
+    r.kind = 1, r.E = E, r.f32 = f32, r.sel = in_f32, r.odd = odd, r.blue = blue, r.l = l, r.row = a;
+    t_recorder->push_back(r);
+    return hipSuccess;
+  }
+  return do_r2c(E, f32, in_f32, odd, blue, l, a);
+}
+hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, const Launch& l, const RowArgs& a) {
+  if (t_recorder) {
+    PassLaunch r{};
+    r.kind = 2, r.E = E, r.f32 = f32, r.sel = out_kind, r.odd = odd, r.epi = epi, r.blue = blue, r.l = l;
+    r.row = a;
+    t_recorder->push_back(r);
+    return hipSuccess;
+  }
+  return do_c2r(E, f32, out_kind, odd, epi, blue, l, a);
+}
+
+hipError_t replay_launch(const PassLaunch& r) {
+  switch (r.kind) {
+    case 0: return do_col(r.E, r.f32, r.sel, r.split, r.groups, r.blue, r.stat, r.l, r.col);
+    case 1: return do_r2c(r.E, r.f32, r.sel, r.odd, r.blue, r.l, r.row);
+    case 2: return do_c2r(r.E, r.f32, r.sel, r.odd, r.epi, r.blue, r.l, r.row);
+    default: return hipErrorInvalidValue;
+  }
 }
 #undef FF_SWITCH_E
 

@@ -185,6 +185,11 @@ typedef struct ff_batch_item {
                                  first solve and replay it while buffers are unchanged */
 #define FF_BATCH_PERSISTENT 4 /* one persistent kernel over every instance: disabled in
                                  this build, rejected with FF_ERR_UNSUPPORTED           */
+#define FF_BATCH_GROUPED 8    /* instances of one shape, precision and refinement count
+                                 run each pass as one launch (one grid row per instance,
+                                 per-instance buffers and domains from a device table);
+                                 the table is rebuilt, after a device synchronization,
+                                 when the buffers change                                 */
 
 typedef struct ff_batch ff_batch;
 

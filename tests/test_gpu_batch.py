@@ -211,3 +211,80 @@ def test_batching_workload_analytic():
         b.close()
         del d, p
         torch.cuda.empty_cache()
+
+
+GROUP_SHAPES = [(32, 32, 32), (48, 40, 36), (32, 32, 32), (12, 10, 9), (48, 40, 36), (32, 32, 32), (7, 5, 11),
+                (64, 32, 16), (7, 5, 11), (12, 10, 9)]
+
+
+def grouped_items(precision, refine):
+    items = []
+    for k, n in enumerate(GROUP_SHAPES):
+        it = {"n": n, "precision": precision, "refine_steps": refine}
+        if k % 3 == 1:  # some instances of a shape on another domain (other filter constants)
+            it["lo"] = [0.0, -1.0, 0.5]
+            it["hi"] = [2.0 + 0.25 * k, 3.0, 4.5]
+        items.append(it)
+    return items
+
+
+def batch_outputs(items, fields, flags, stream=None):
+    b = ff.Batch(items, flags)
+    d, p = alloc(b)
+    for k, f in enumerate(fields):
+        put(d[k], f)
+    b.solve(d, p, stream)
+    b.wait(stream)
+    out = [get(p[k], it["n"]) for k, it in enumerate(items)]
+    desc = json.loads(b.describe())
+    b.close()
+    return out, desc
+
+
+@pytest.mark.parametrize("precision,refine", [(ff.FF_FP64, 0), (ff.FF_FP32_REFINE, 2)])
+def test_grouped_batch_bitwise(precision, refine):
+    """FF_BATCH_GROUPED runs each pass of a group of same-shape instances as
+    one launch with a per-instance table: the same kernels and arithmetic as
+    the stream-parallel batch, so the results are bit-identical, with
+    instances of one shape on different domains in the same launch."""
+    rng = np.random.default_rng(21)
+    items = grouped_items(precision, refine)
+    fields = [rng.standard_normal(it["n"]) for it in items]
+    say("grouped", precision, refine)
+    ref, _ = batch_outputs(items, fields, 0)
+    out, desc = batch_outputs(items, fields, ff.FF_BATCH_GROUPED)
+    groups = desc["groups"]
+    say("  groups", groups)
+    assert sum(g["instances"] for g in groups) == len(items)
+    assert len(groups) == len(set(GROUP_SHAPES))  # one group per shape: domains differ only in the table
+    for k, it in enumerate(items):
+        assert np.array_equal(out[k], ref[k]), (k, it["n"])
+        if "lo" not in it:
+            assert rel(out[k], oracle.solve(fields[k])) < (TOL if precision == ff.FF_FP64 else 1e-12)
+
+
+def test_grouped_batch_graph_replays_and_rebinding():
+    """Grouped launches captured in a graph replay bit-identically (eager
+    reloads between replays), and new buffers rebuild the per-instance table."""
+    rng = np.random.default_rng(22)
+    items = grouped_items(ff.FF_FP64, 0)
+    fields = [rng.standard_normal(it["n"]) for it in items]
+    eager, _ = batch_outputs(items, fields, 0)
+    s = torch.cuda.Stream()
+    b = ff.Batch(items, ff.FF_BATCH_GROUPED | ff.FF_BATCH_GRAPH)
+    for binding in range(2):
+        d, p = alloc(b)  # fresh buffers: the table is rebuilt for them
+        for rep in range(3):
+            with torch.cuda.stream(s):
+                for k, f in enumerate(fields):
+                    put(d[k], f)  # FP64 clobbers density: reload it eagerly between replays
+                    p[k].zero_()
+            s.synchronize()
+            b.solve(d, p, s)
+            b.wait(s)
+            torch.cuda.synchronize()
+            for k, it in enumerate(items):
+                assert np.array_equal(get(p[k], it["n"]), eager[k]), (binding, rep, it["n"])
+        say("grouped graph", binding, json.loads(b.describe())["graph"])
+        del d, p
+    b.close()
