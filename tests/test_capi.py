@@ -107,6 +107,15 @@ def test_persistent_batch_is_rejected_without_gpu_work():
     assert "persistent" in str(e.value)
 
 
+def test_batch_flags_are_validated_without_gpu_work():
+    with pytest.raises(ff.FishFryError) as e:
+        ff.Batch([{"n": (8, 8, 8)}], 16)  # no such flag
+    assert e.value.status == ff.FF_ERR_INVALID
+    with pytest.raises(ff.FishFryError) as e:
+        ff.Batch([{"n": (8, 8, 1)}], ff.FF_BATCH_GROUPED)  # extents below 2 are refused first
+    assert e.value.status == ff.FF_ERR_INVALID
+
+
 def test_tuning_host_logic_without_gpu_work():
     """ff_tune_query / ff_tune_set are host logic: the default configurations
     of the 1024-long column passes run the kernels with the line plan compiled
