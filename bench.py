@@ -238,7 +238,10 @@ def bench_serving(ff, nreq=96, seed=20261202):
         modes = random_modes(int(rng.integers(1 << 30)))
         return n, prec, ff.field_set_modes(ff.field_default(n), modes)
 
-    srv = ff.Server(hbm_budget=48 << 30, streams=4, max_batch=8, batch_window_us=300.0)
+    # size-class groups of at most two requests, gathered for 100 us: each
+    # request runs its own solve, so larger groups only queue their later
+    # members behind the earlier ones on one stream
+    srv = ff.Server(hbm_budget=48 << 30, streams=4, max_batch=2, batch_window_us=100.0)
     # closed-loop burst: capacity in requests/s for this mix
     burst = [request(k) for k in range(32)]
     t0 = time.perf_counter()
@@ -248,7 +251,7 @@ def bench_serving(ff, nreq=96, seed=20261202):
     cap = len(burst) / (time.perf_counter() - t0)
     srv.close()
     rate = 0.75 * cap
-    srv = ff.Server(hbm_budget=48 << 30, streams=4, max_batch=8, batch_window_us=300.0)
+    srv = ff.Server(hbm_budget=48 << 30, streams=4, max_batch=2, batch_window_us=100.0)
     reqs = [request(k) for k in range(nreq)]
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
     t0 = time.perf_counter()

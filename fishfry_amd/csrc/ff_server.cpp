@@ -59,7 +59,15 @@ struct Group {
   std::vector<Req*> reqs;
   int stream = 0;
   hipEvent_t ev = nullptr;
+  double work = 0;  // estimated cost of the group's solves (see work_of)
 };
+
+// Estimated cost of a request's solve, in point-passes: five passes per
+// solve, FP32 + refinement adds ten per step at about half the bytes.
+double work_of(const ff_request& r) {
+  const double N = double(r.n[0]) * r.n[1] * r.n[2];
+  return r.precision == FF_FP32_REFINE ? N * (2.5 + 7.5 * r.refine_steps) : N * 5.0;
+}
 
 }  // namespace
 
@@ -70,6 +78,7 @@ struct ff_server {
   Arena arena;
   std::vector<hipStream_t> streams;
   std::vector<int> stream_load;
+  std::vector<double> stream_work;  // estimated work queued on each stream
   std::vector<hipEvent_t> free_events;
   // plans by (extents, precision, refinement steps, lo, hi), with the tick of
   // their last use; at most kMaxPlans are kept (least recently used evicted)
@@ -237,6 +246,8 @@ void dispatch(ff_server* s, std::vector<Req*>& group, int k) {
   }
   (void)hipEventRecord(g.ev, hs);
   s->stream_load[k]++;
+  for (Req* q : g.reqs) g.work += work_of(q->r);
+  s->stream_work[k] += g.work;
   s->dispatched_groups++;
   s->inflight.push_back(g);
 }
@@ -267,6 +278,7 @@ void complete(ff_server* s, Group& g) {
     finish(s, q, st, true);
   }
   s->stream_load[g.stream]--;
+  s->stream_work[g.stream] = std::max(0.0, s->stream_work[g.stream] - g.work);
   s->free_events.push_back(g.ev);
   std::lock_guard<std::mutex> lk(s->mu);
   s->cv_done.notify_all();
@@ -310,10 +322,12 @@ void worker_main(ff_server* s) {
       const bool full = (int)reqs.size() >= s->cfg.max_batch;
       const bool due = t - reqs.front()->t_submit >= window || s->inflight.empty();
       if (!full && !due) continue;
-      int k = 0;
-      for (int j = 1; j < (int)s->streams.size(); ++j)
-        if (s->stream_load[j] < s->stream_load[k]) k = j;
-      if (s->stream_load[k] >= 2) break;  // every stream has two groups queued
+      // the stream with the least work queued among those with fewer than two
+      // groups, so a small request does not wait behind a large one
+      int k = -1;
+      for (int j = 0; j < (int)s->streams.size(); ++j)
+        if (s->stream_load[j] < 2 && (k < 0 || s->stream_work[j] < s->stream_work[k])) k = j;
+      if (k < 0) break;  // every stream has two groups queued
       std::vector<Req*> group;
       for (Req* q : reqs) {
         if ((int)group.size() >= s->cfg.max_batch) break;
@@ -385,6 +399,7 @@ ff_status ff_server_create(const ff_server_config* cfg, ff_server** out) {
     if (st == FF_OK) {
       s->streams.push_back(hs);
       s->stream_load.push_back(0);
+      s->stream_work.push_back(0.0);
     }
   }
   if (st != FF_OK) {
