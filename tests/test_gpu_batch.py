@@ -160,12 +160,14 @@ def test_batch_graph_replays():
     b.close()
 
 
-def test_batch_autotune_keeps_results():
+@pytest.mark.parametrize("flags", [0, ff.FF_BATCH_GROUPED])
+def test_batch_autotune_keeps_results(flags):
+    # grouped: the autotuner re-plans, which must rebuild the recorded groups
     rng = np.random.default_rng(6)
-    shapes = MIX[:5]
+    shapes = MIX[:5] + [MIX[0], MIX[1]]
     fields = [rng.standard_normal(n) for n in shapes]
     items = [{"n": n} for n in shapes]
-    b = ff.Batch(items)
+    b = ff.Batch(items, flags)
     d, p = alloc(b)
     try:
         tuned, before, after = b.autotune(d, p, reps=2)
