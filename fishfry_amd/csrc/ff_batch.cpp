@@ -497,6 +497,8 @@ bool same_launch(const PassLaunch& x, const PassLaunch& y) {
 // Records one solve per instance, forms the groups and uploads their
 // per-instance tables (blocking: waits for the device before rewriting them).
 ff_status prepare_groups(ff_batch* b, double* const* d, double* const* p) {
+  b->groups_ok = false;  // the old groups' table may be freed below
+  b->groups.clear();
   std::vector<std::vector<PassLaunch>> rec(b->count);
   for (int i = 0; i < b->count; ++i) {
     set_launch_recorder(&rec[i]);
