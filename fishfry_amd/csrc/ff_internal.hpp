@@ -38,8 +38,9 @@ struct LineCfg {
 
 // Chooses E and a radix sequence for a length-n transform.  A length with a
 // prime factor above 5 is transformed by Bluestein's algorithm over the
-// smallest 2^a 3^b 5^c length m >= 2n - 1 that has a plan (forceE then
-// applies to m).  False when no plan fits FF_MAX_THREADS threads per line.
+// 2^a 3^b 5^c length m >= 2n - 1 with a plan and the least work m * (passes +
+// 0.04 E) (forceE then applies to m).  False when no plan fits one workgroup
+// per line.
 bool plan_line(int n, LineCfg& out, int forceE = 0);
 
 // Bluestein tables of a length-n transform computed over length m, in

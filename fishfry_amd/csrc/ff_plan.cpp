@@ -299,14 +299,28 @@ bool plan_line(int n, LineCfg& out, int forceE) {
   if (n < 1) return false;
   if (plan_smooth(n, out, forceE)) return true;
   if (smooth235(n)) return false;  // a 2^a 3^b 5^c length too long for one workgroup
+  // Bluestein: of the 2^a 3^b 5^c engine lengths m >= 2n - 1 that have a plan,
+  // the one with the least work, m * (passes + 0.04 E).  A slightly longer m
+  // with fewer passes beats the shortest one (n = 458: m = 960 takes six
+  // passes, m = 1000 three).
+  double best = 1e300;
+  bool found = false;
+  LineCfg bestc;
   for (long long m = 2LL * n - 1; m <= 4LL * n; ++m) {
     if (!smooth235(m)) continue;
-    if (plan_smooth((int)m, out, forceE)) {
-      out.n = n;
-      return true;
+    LineCfg c;
+    if (!plan_smooth((int)m, c, forceE)) continue;
+    const double cost = double(m) * (double(c.lp.np) + 0.04 * c.E);
+    if (cost < best) {
+      best = cost;
+      bestc = c;
+      found = true;
     }
   }
-  return false;
+  if (!found) return false;
+  out = bestc;
+  out.n = n;
+  return true;
 }
 
 static bool plan_smooth(int n, LineCfg& out, int forceE) {
