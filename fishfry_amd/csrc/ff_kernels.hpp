@@ -12,6 +12,8 @@
 //   k_c2r   z inverse + the 1/(nx*ny*nz) scale                (:380, :406-421)
 #pragma once
 
+#include <climits>
+
 #include "ff_fft.hpp"
 
 namespace ff {
@@ -82,6 +84,14 @@ struct ColArgs {
   int nblocks;        // nouter * tiles
   int ps;             // LDS pad shift
   int nx, ny;         // extents for the spectral symbol (x pass)
+  // Alignment of the written row segments (kernels with a compiled-in plan).
+  // When all rows of a plane start at one offset within a 64-byte block but
+  // planes start at different ones (plain x pass, odd nh), a plane's tiles
+  // are shifted left by sh = (element offset of its column 0) & cshift_mask
+  // columns: every row segment a workgroup writes then starts on a segment
+  // boundary and no 32-byte sector is written in parts by two workgroups.
+  // cshift_mask = 0: no shift.
+  int cshift_mul, cshift_mask;
   double ddi, ddj, ddk;
   double* dc_out;  // FILTER / SYMBOL: receives the forward (0,0,0) term (sum of the input)
   const void* tw;
@@ -263,17 +273,24 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, SN > 0 ? SN / E * ST : 
     outer = b / a.tiles;
     tile = b - outer * a.tiles;
   }
+  // (the generic kernels never shift: the extra live column index costs
+  // them an occupancy step)
+  int sh = 0;
+  if constexpr (SN > 0)
+    sh = (int)(((unsigned long long)(uintptr_t)io.dst / sizeof(cx<F>) + (unsigned long long)outer * a.cshift_mul) &
+               (unsigned long long)a.cshift_mask);
+  constexpr int kMinCol = SN > 0 ? 0 : INT_MIN;  // columns below 0 exist only when shifted
   const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
-  // Threads past the last column load the last column (in bounds, never
+  // Threads outside the columns load the nearest column (in bounds, never
   // stored), so the loads need no predicate.  src may equal dst only when
   // both sides use the same layout (in-place pass): every thread stores
   // exactly the elements it loaded, after all of its loads.
   cx<F> v[G][E];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    const int col = (tile * G + g) * T + c;
+    const int col = (tile * G + g) * T + c - sh;
     long long ib, is;
-    side_addr(a.in, outer, col < a.ncols ? col : a.ncols - 1, r0, L, ib, is);
+    side_addr(a.in, outer, col < kMinCol ? kMinCol : (col < a.ncols ? col : a.ncols - 1), r0, L, ib, is);
     const FF_GLB cx<F>* src = glb<const cx<F>>(io.src) + ib;
     if constexpr (!BL) {
 #pragma unroll
@@ -286,13 +303,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, SN > 0 ? SN / E * ST : 
   const ColIdx idx{T, c, SN > 0 ? 4 : a.ps};
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    const int col = (tile * G + g) * T + c;
-    col_line<F, E, MODE, SPLIT, BL, SN>(v[g], a, io, tw, lds, idx, r0, outer, col, col < a.ncols);
+    const int col = (tile * G + g) * T + c - sh;
+    col_line<F, E, MODE, SPLIT, BL, SN>(v[g], a, io, tw, lds, idx, r0, outer, col, col >= kMinCol && col < a.ncols);
   }
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    const int col = (tile * G + g) * T + c;
-    if (col < a.ncols) {
+    const int col = (tile * G + g) * T + c - sh;
+    if (col >= kMinCol && col < a.ncols) {
       long long ob, os;
       side_addr(a.out, outer, col, r0, L, ob, os);
       FF_GLB cx<F>* dst = glb<cx<F>>(io.dst) + ob;

@@ -110,7 +110,7 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     const long long rb =
         std::max((rows + p->zr[prec].T - 1) / p->zr[prec].T, (rows + p->zi[prec].T - 1) / p->zi[prec].T);
     auto col_blocks = [&](const ColCfg& c, int nouter) {
-      const long long tiles = (p->nh + (long long)c.groups * c.T - 1) / ((long long)c.groups * c.T);
+      const long long tiles = (p->nh + 7 + (long long)c.groups * c.T - 1) / ((long long)c.groups * c.T);
       return (long long)nouter * tiles;
     };
     if (rb > INT_MAX || col_blocks(p->cy[prec], n[0]) > INT_MAX || col_blocks(p->cx[prec], n[1]) > INT_MAX) {
@@ -321,7 +321,20 @@ ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* sr
   a.outer_fast = 0;
   a.T = c.T;
   a.lgT = c.lgT;
-  a.tiles = (a.ncols + c.groups * c.T - 1) / (c.groups * c.T);
+  // Kernels with a compiled-in plan, plain layout on both sides, rows that
+  // start 64-byte aligned within a plane but planes that do not (the x pass at
+  // odd nh): shift each plane's tiles onto the 64-byte grid of its rows.  The
+  // tile count covers the shift.
+  a.cshift_mask = a.cshift_mul = 0;
+  if (c.stat) {
+    const long long mask = prec ? 7 : 3;  // complex elements per 64 bytes, minus one
+    const long long W = a.out.Wlast;
+    if (a.in.nb == 1 && a.out.nb == 1 && ((a.out.fr * W) & mask) == 0 && ((a.out.fo * W) & mask) != 0) {
+      a.cshift_mask = (int)mask;
+      a.cshift_mul = (int)((a.out.fo * W) & mask);
+    }
+  }
+  a.tiles = (a.ncols + a.cshift_mask + c.groups * c.T - 1) / (c.groups * c.T);
   const long long nb = (long long)a.nouter * a.tiles;
   if (nb > INT_MAX) return set_error(FF_ERR_INVALID, "grid too large");
   a.nblocks = (int)nb;

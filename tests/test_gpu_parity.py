@@ -157,6 +157,35 @@ def test_each_pass_matches_oracle(n, flags):
     plan.close()
 
 
+@pytest.mark.parametrize("n", [(1024, 4, 8), (1024, 8, 10), (1024, 12, 2), (1024, 6, 7), (64, 48, 40)])
+def test_x_pass_tile_shift_bitwise(n):
+    """The x pass with its line plan compiled in (length 1024) shifts each
+    plane's column tiles onto the 64-byte grid of the destination rows when
+    planes start off that grid (ny*nh a multiple of 4, nh not).  Writing one
+    spectrum to destinations 0..3 complex elements apart moves the tile
+    boundaries, never the values.  (1024, 6, 7): rows off the grid, no
+    shift; (64, 48, 40): generic kernel, no shift."""
+    rng = np.random.default_rng(11)
+    plan, nb, a, _ = make(n)
+    nh = n[2] // 2 + 1
+    spec = rng.standard_normal((n[0], n[1], nh)) + 1j * rng.standard_normal((n[0], n[1], nh))
+    put_half(a, spec)
+    lo, hi, _ = oracle.grid(n)
+    want = oracle.fft(oracle.filter_half(oracle.fft(spec, -1, axis=0), n[2], lo, hi), +1, axis=0)
+    big = torch.zeros(nb // 8 + 8, dtype=torch.float64, device="cuda")
+    outs = []
+    for k in range(4):
+        say("x pass tile shift", n, "dst offset", k)
+        big.zero_()
+        dst = big[2 * k:]
+        plan.solve_pass(2, a, dst)
+        torch.cuda.synchronize()
+        outs.append(get_half(dst, n))
+    plan.close()
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+    assert rel(outs[0], want) < TOL
+
+
 @pytest.mark.parametrize("n", [(32, 1024, 512), (48, 40, 36), (30, 25, 27), (64, 64, 2), (96, 160, 48)])
 def test_layouts_agree_bitwise(n):
     """The blocked spectrum layouts change addresses only, not arithmetic."""
