@@ -593,7 +593,9 @@ struct StaticCol {
   int split;  // FF_TUNE_SPLIT_LDS exchange
   int waves;  // waves per SIMD the registers must allow
 };
-constexpr StaticCol kStaticCols[] = {{1024, 16, 4, 0, 0, 2}, {1024, 16, 8, 1, 0, 2}};
+constexpr StaticCol kStaticCols[] = {{1024, 16, 4, 0, 0, 2}, {1024, 16, 8, 1, 0, 2},
+                                     // the FP64 x pass of 768 (its y pass uses E = 24)
+                                     {768, 12, 4, 0, 0, 3}};
 constexpr int kNumStaticCols = sizeof(kStaticCols) / sizeof(kStaticCols[0]);
 
 template <int E>
