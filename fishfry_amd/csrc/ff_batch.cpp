@@ -757,8 +757,8 @@ ff_status ff_batch_autotune(ff_batch* b, double* const* density, double* const* 
           if (ok) (kind == 0 ? trial.zr : trial.zi)[prec] = c;
         } else {
           ColCfg c, c0;
-          ok = col_cfg_with(n, p->nh, eb, c, E, T, fl);
-          if (ok && fl == FF_TUNE_GENERIC) ok = col_cfg_with(n, p->nh, eb, c0, E, T, 0) && c0.stat;
+          ok = col_cfg_with(n, p->nh, eb, c, E, T, fl, kind);
+          if (ok && fl == FF_TUNE_GENERIC) ok = col_cfg_with(n, p->nh, eb, c0, E, T, 0, kind) && c0.stat;
           if (ok) (kind == 1 ? trial.cy : trial.cx)[prec] = c;
         }
         if (!ok) continue;

@@ -44,15 +44,22 @@ static const void* col_fn(int f32, int mode) {
 constexpr bool kTwoGroups = FF_E == 8 || FF_E == 16;
 
 // Kernels with the line plan compiled in: the kStaticCols entries of this E.
-template <typename F, bool S, int N, int T, int W>
+template <typename F, bool S, int N, int T, int W, int KINDS>
 static const void* col_static_modes(int mode) {
   constexpr int E = FF_E;
+  constexpr bool y = (KINDS & 2) != 0, x = (KINDS & 4) != 0;
   switch (mode) {
-    case COL_FWD: return (const void*)&k_col<F, E, COL_FWD, S, 1, false, N, T, W>;
-    case COL_INV: return (const void*)&k_col<F, E, COL_INV, S, 1, false, N, T, W>;
-    case COL_FILTER: return (const void*)&k_col<F, E, COL_FILTER, S, 1, false, N, T, W>;
+    case COL_FWD:
+      if constexpr (y) return (const void*)&k_col<F, E, COL_FWD, S, 1, false, N, T, W>;
+      return nullptr;
+    case COL_INV:
+      if constexpr (y) return (const void*)&k_col<F, E, COL_INV, S, 1, false, N, T, W>;
+      return nullptr;
+    case COL_FILTER:
+      if constexpr (x) return (const void*)&k_col<F, E, COL_FILTER, S, 1, false, N, T, W>;
+      return nullptr;
     case COL_SYMBOL:
-      if constexpr (std::is_same<F, double>::value)
+      if constexpr (x && std::is_same<F, double>::value)
         return (const void*)&k_col<F, E, COL_SYMBOL, S, 1, false, N, T, W>;
       return nullptr;
     default: return nullptr;
@@ -65,8 +72,8 @@ static const void* col_static_fn(int f32, int split, int mode, int n, int T) {
     constexpr StaticCol sc = kStaticCols[K];
     if constexpr (sc.E == FF_E) {
       if (sc.n == n && sc.T == T && sc.f32 == f32 && sc.split == split) {
-        if constexpr (sc.f32) return col_static_modes<float, sc.split != 0, sc.n, sc.T, sc.waves>(mode);
-        else return col_static_modes<double, sc.split != 0, sc.n, sc.T, sc.waves>(mode);
+        if constexpr (sc.f32) return col_static_modes<float, sc.split != 0, sc.n, sc.T, sc.waves, sc.kinds>(mode);
+        else return col_static_modes<double, sc.split != 0, sc.n, sc.T, sc.waves, sc.kinds>(mode);
       }
     }
     return col_static_fn<K + 1>(f32, split, mode, n, T);

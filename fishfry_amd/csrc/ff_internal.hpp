@@ -85,7 +85,7 @@ struct RowCfg {
 // overrides (ff_tune_set) take precedence over the built-in heuristic.
 bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int kind, int prec);
 bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int kind, int prec);
-bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, int flags = 0);
+bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, int flags, int kind);
 bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T);
 bool tune_lookup(int kind, int n, int prec, int& E, int& T, int& flags);
 ff_status tune_set(const ff_tune* t);

@@ -592,10 +592,11 @@ struct StaticCol {
   int n, E, T, f32;
   int split;  // FF_TUNE_SPLIT_LDS exchange
   int waves;  // waves per SIMD the registers must allow
+  int kinds;  // passes it serves, as 1 << kind: 2 the y pass (forward, inverse), 4 the x pass (filter, symbol)
 };
-constexpr StaticCol kStaticCols[] = {{1024, 16, 4, 0, 0, 2}, {1024, 16, 8, 1, 0, 2},
-                                     // the FP64 x pass of 768 (its y pass uses E = 24)
-                                     {768, 12, 4, 0, 0, 3}};
+constexpr StaticCol kStaticCols[] = {{1024, 16, 4, 0, 0, 2, 6}, {1024, 16, 8, 1, 0, 2, 6},
+                                     // FP64 x passes of 768 and 1000 (the y passes are faster generic)
+                                     {768, 12, 4, 0, 0, 3, 4}, {1000, 20, 4, 0, 0, 2, 4}};
 constexpr int kNumStaticCols = sizeof(kStaticCols) / sizeof(kStaticCols[0]);
 
 template <int E>

@@ -397,7 +397,7 @@ static bool static_plan_matches(int n, int E, const LinePlan& lp) {
   }
 }
 
-bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, int flags) {
+bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, int flags, int kind) {
   if (!plan_line(n, out.line, E)) return false;
   const int L = out.line.lp.L;
   if (T <= 0) {
@@ -424,7 +424,7 @@ bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, i
   if (!(flags & FF_TUNE_GENERIC) && out.groups == 1 && !out.line.bluestein() && out.ps == 4)
     for (const StaticCol& sc : kStaticCols)
       if (sc.n == n && sc.E == out.line.E && sc.T == T && sc.f32 == (elem_bytes == 8 ? 1 : 0) &&
-          sc.split == out.split)
+          sc.split == out.split && (sc.kinds & (1 << kind)))
         out.stat = static_plan_matches<0>(n, out.line.E, out.line.lp) ? 1 : 0;
   const size_t ne = (size_t)out.line.lp.n;  // engine length (Bluestein: m >= 2n - 1)
   const size_t elems = ne * T + ((ne - 1) >> out.ps) + 1;
@@ -468,6 +468,7 @@ static const BuiltinCfg kBuiltin[] = {
     {1, 768, 24, 4},  // y, ny = 768
     {2, 768, 12, 4},  // x, nx = 768
     {1, 1000, 20, 4}, // y, ny = 1000
+    {2, 1000, 20, 4}, // x, nx = 1000 (compiled-in plan)
 };
 
 static bool builtin_lookup(int kind, int n, int prec, int& E, int& T) {
@@ -483,9 +484,10 @@ static bool builtin_lookup(int kind, int n, int prec, int& E, int& T) {
 
 bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int kind, int prec) {
   int E = 0, T = 0, flags = 0;
-  if (tune_lookup(kind, n, prec, E, T, flags) && col_cfg_with(n, ncols, elem_bytes, out, E, T, flags)) return true;
-  if (builtin_lookup(kind, n, prec, E, T) && col_cfg_with(n, ncols, elem_bytes, out, E, T)) return true;
-  return col_cfg_with(n, ncols, elem_bytes, out, 0, 0);
+  if (tune_lookup(kind, n, prec, E, T, flags) && col_cfg_with(n, ncols, elem_bytes, out, E, T, flags, kind))
+    return true;
+  if (builtin_lookup(kind, n, prec, E, T) && col_cfg_with(n, ncols, elem_bytes, out, E, T, 0, kind)) return true;
+  return col_cfg_with(n, ncols, elem_bytes, out, 0, 0, 0, kind);
 }
 
 bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int kind, int prec) {
@@ -514,7 +516,7 @@ ff_status tune_set(const ff_tune* t) {
     ok = t->flags == 0 && row_cfg_with(2 * t->n, eb, c, t->E, t->T);
   } else {
     ColCfg c;
-    ok = col_cfg_with(t->n, 1 << 20, eb, c, t->E, t->T, t->flags);
+    ok = col_cfg_with(t->n, 1 << 20, eb, c, t->E, t->T, t->flags, t->kind);
   }
   if (!ok) return set_error(FF_ERR_INVALID, "ff_tune_set: no valid configuration for that E/T/flags");
   std::lock_guard<std::mutex> g(tune_mutex());

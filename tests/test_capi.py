@@ -118,10 +118,11 @@ def test_batch_flags_are_validated_without_gpu_work():
 
 def test_tuning_host_logic_without_gpu_work():
     """ff_tune_query / ff_tune_set are host logic: the default configurations
-    of the 1024-long column passes run the kernels with the line plan compiled
-    in (flags without FF_TUNE_GENERIC -- which also needs the compile-time plan
-    to equal the planner's), other lengths run the generic kernel, and the
-    overrides round-trip."""
+    of the 1024-long column passes and of the 768- and 1000-long FP64 x
+    passes run the kernels with the line plan compiled in (flags without
+    FF_TUNE_GENERIC -- which also needs the compile-time plan to equal the
+    planner's), the y passes of those lengths and other lengths run the
+    generic kernel, and the overrides round-trip."""
     ff.tune_reset()
     try:
         for kind in (1, 2):
@@ -130,7 +131,17 @@ def test_tuning_host_logic_without_gpu_work():
             t = ff.tune_query(kind, 1024, 1, 513)
             assert (t.E, t.T, t.flags) == (16, 8, 0)
             assert ff.tune_query(kind, 512, 0, 257).flags & ff.FF_TUNE_GENERIC
-            assert ff.tune_query(kind, 1000, 0, 501).flags & ff.FF_TUNE_GENERIC
+        t = ff.tune_query(2, 1000, 0, 501)
+        assert (t.E, t.T, t.flags) == (20, 4, 0)
+        t = ff.tune_query(2, 768, 0, 385)
+        assert (t.E, t.T, t.flags) == (12, 4, 0)
+        for n, E in ((1000, 20), (768, 24)):
+            t = ff.tune_query(1, n, 0, n // 2 + 1)
+            assert (t.E, t.T, t.flags) == (E, 4, ff.FF_TUNE_GENERIC)
+        assert ff.tune_query(2, 1000, 1, 501).flags & ff.FF_TUNE_GENERIC  # FP32: generic
+        ff.tune_set(1, 1000, 0, 20, 4, 0)  # the x-only kernel is not used for the y pass
+        assert ff.tune_query(1, 1000, 0, 501).flags == ff.FF_TUNE_GENERIC
+        ff.tune_reset()
         ff.tune_set(2, 1024, 0, 16, 4, ff.FF_TUNE_GENERIC)
         assert ff.tune_query(2, 1024, 0, 513).flags == ff.FF_TUNE_GENERIC
         assert ff.tune_query(1, 1024, 0, 513).flags == 0
