@@ -118,8 +118,8 @@ def test_batch_flags_are_validated_without_gpu_work():
 
 def test_tuning_host_logic_without_gpu_work():
     """ff_tune_query / ff_tune_set are host logic: the default configurations
-    of the 1024-long column passes and of the 768- and 1000-long FP64 x
-    passes run the kernels with the line plan compiled in (flags without
+    of the 1024-long column passes and of the 128-, 256-, 512-, 768- and
+    1000-long FP64 x passes run the kernels with the line plan compiled in (flags without
     FF_TUNE_GENERIC -- which also needs the compile-time plan to equal the
     planner's), the y passes of those lengths and other lengths run the
     generic kernel, and the overrides round-trip."""
@@ -130,7 +130,8 @@ def test_tuning_host_logic_without_gpu_work():
             assert (t.E, t.T, t.flags) == (16, 4, 0)
             t = ff.tune_query(kind, 1024, 1, 513)
             assert (t.E, t.T, t.flags) == (16, 8, 0)
-            assert ff.tune_query(kind, 512, 0, 257).flags & ff.FF_TUNE_GENERIC
+        assert ff.tune_query(1, 512, 0, 257).flags & ff.FF_TUNE_GENERIC
+        assert ff.tune_query(2, 600, 0, 301).flags & ff.FF_TUNE_GENERIC
         t = ff.tune_query(2, 1000, 0, 501)
         assert (t.E, t.T, t.flags) == (20, 4, 0)
         t = ff.tune_query(2, 768, 0, 385)
