@@ -118,8 +118,8 @@ def test_batch_flags_are_validated_without_gpu_work():
 
 def test_tuning_host_logic_without_gpu_work():
     """ff_tune_query / ff_tune_set are host logic: the default configurations
-    of the 1024-long column passes and of the 128-, 256-, 512-, 768- and
-    1000-long FP64 x passes run the kernels with the line plan compiled in (flags without
+    of the 1024-long column passes and of the x passes of 32..160, 256,
+    512, 768 and 1000 run the kernels with the line plan compiled in (flags without
     FF_TUNE_GENERIC -- which also needs the compile-time plan to equal the
     planner's), the y passes of those lengths and other lengths run the
     generic kernel, and the overrides round-trip."""

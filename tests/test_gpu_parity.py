@@ -209,7 +209,8 @@ def test_layouts_agree_bitwise(n):
 @pytest.mark.parametrize("n,E", [((64, 48, 40), 0), ((30, 25, 27), 0), ((96, 1000, 64), 10), ((1024, 16, 8), 16),
                                  ((768, 12, 6), 12), ((128, 64, 96), 8), ((256, 1024, 20), 16),
                                  ((512, 512, 12), 0), ((1024, 1024, 4), 0), ((64, 1000, 10), 20),
-                                 ((1000, 16, 10), 20), ((768, 8, 6), 12), ((128, 16, 64), 0)])
+                                 ((1000, 16, 10), 20), ((768, 8, 6), 12), ((128, 16, 64), 0), ((64, 32, 64), 0),
+                                 ((96, 16, 128), 0), ((48, 8, 64), 0), ((160, 8, 64), 0), ((32, 8, 40), 0)])
 def test_column_variants_agree(n, E, variant):
     """Column-pass variants -- the split real/imaginary LDS exchange, two
     column groups per workgroup, and the generic kernel in place of the one
@@ -217,8 +218,8 @@ def test_column_variants_agree(n, E, variant):
     values as the default kernel.  They are separate kernels, so the compiler
     may contract multiply-adds differently: results agree to round-off, and
     all match the oracle.  Two groups apply where E is 8 or 16.  Compiled-in
-    plans exist for 1024 (E 16) and for the x passes of 128, 256 (E 16),
-    512 (E 8), 768 (E 12) and 1000 (E 20)."""
+    plans exist for 1024 (E 16) and for the x passes of 32, 48, 64, 96,
+    128, 160, 256, 512, 768 and 1000."""
     rng = np.random.default_rng(5)
     f = rng.standard_normal(n)
     try:
