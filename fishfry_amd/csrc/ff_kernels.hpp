@@ -595,14 +595,15 @@ struct StaticCol {
   int kinds;  // passes it serves, as 1 << kind: 2 the y pass (forward, inverse), 4 the x pass (filter, symbol)
 };
 constexpr StaticCol kStaticCols[] = {{1024, 16, 4, 0, 0, 2, 6}, {1024, 16, 8, 1, 0, 2, 6},
-                                     // FP64 x passes of 768 and 1000 (the y passes are faster generic)
+                                     // x passes only at 512, 768 and 1000 (the y passes are as fast or
+                                     // faster generic there); both column passes at 256 and below
                                      {768, 12, 4, 0, 0, 3, 4}, {1000, 20, 4, 0, 0, 2, 4},
-                                     {512, 8, 4, 0, 0, 4, 4}, {256, 16, 16, 0, 0, 2, 4}, {128, 16, 32, 0, 0, 2, 4},
-                                     // x passes of small grids (FP64 and FP32)
-                                     {160, 20, 32, 0, 0, 2, 4}, {96, 24, 64, 0, 0, 1, 4}, {64, 8, 32, 0, 0, 4, 4},
-                                     {48, 12, 32, 0, 0, 3, 4}, {32, 8, 32, 0, 0, 4, 4}, {128, 16, 32, 1, 0, 2, 4},
-                                     {160, 20, 32, 1, 0, 2, 4}, {96, 24, 64, 1, 0, 2, 4}, {64, 8, 32, 1, 0, 4, 4},
-                                     {48, 12, 32, 1, 0, 4, 4}, {32, 8, 32, 1, 0, 4, 4}};
+                                     {512, 8, 4, 0, 0, 4, 4}, {256, 16, 16, 0, 0, 2, 6}, {128, 16, 32, 0, 0, 2, 6},
+                                     // small grids (FP64 and FP32)
+                                     {160, 20, 32, 0, 0, 2, 6}, {96, 24, 64, 0, 0, 1, 6}, {64, 8, 32, 0, 0, 4, 6},
+                                     {48, 12, 32, 0, 0, 3, 6}, {32, 8, 32, 0, 0, 4, 6}, {128, 16, 32, 1, 0, 2, 6},
+                                     {160, 20, 32, 1, 0, 2, 6}, {96, 24, 64, 1, 0, 2, 6}, {64, 8, 32, 1, 0, 4, 6},
+                                     {48, 12, 32, 1, 0, 4, 6}, {32, 8, 32, 1, 0, 4, 6}};
 constexpr int kNumStaticCols = sizeof(kStaticCols) / sizeof(kStaticCols[0]);
 
 template <int E>

@@ -118,11 +118,11 @@ def test_batch_flags_are_validated_without_gpu_work():
 
 def test_tuning_host_logic_without_gpu_work():
     """ff_tune_query / ff_tune_set are host logic: the default configurations
-    of the 1024-long column passes and of the x passes of 32..160, 256,
-    512, 768 and 1000 run the kernels with the line plan compiled in (flags without
-    FF_TUNE_GENERIC -- which also needs the compile-time plan to equal the
-    planner's), the y passes of those lengths and other lengths run the
-    generic kernel, and the overrides round-trip."""
+    of the column passes of 1024 and of 32..160 and 256, and of the x passes
+    of 512, 768 and 1000, run the kernels with the line plan compiled in
+    (flags without FF_TUNE_GENERIC -- which also needs the compile-time plan
+    to equal the planner's); the y passes of 512, 768 and 1000 and other
+    lengths run the generic kernel, and the overrides round-trip."""
     ff.tune_reset()
     try:
         for kind in (1, 2):
@@ -132,6 +132,11 @@ def test_tuning_host_logic_without_gpu_work():
             assert (t.E, t.T, t.flags) == (16, 8, 0)
         assert ff.tune_query(1, 512, 0, 257).flags & ff.FF_TUNE_GENERIC
         assert ff.tune_query(2, 600, 0, 301).flags & ff.FF_TUNE_GENERIC
+        for kind in (1, 2):
+            for n, E, T in ((128, 16, 32), (256, 16, 16), (48, 12, 32)):
+                for prec in (0, 1) if n != 256 else (0,):
+                    t = ff.tune_query(kind, n, prec, n // 2 + 1)
+                    assert (t.E, t.T, t.flags) == (E, T, 0), (kind, n, prec)
         t = ff.tune_query(2, 1000, 0, 501)
         assert (t.E, t.T, t.flags) == (20, 4, 0)
         t = ff.tune_query(2, 768, 0, 385)
