@@ -118,7 +118,7 @@ def test_batch_flags_are_validated_without_gpu_work():
 
 def test_tuning_host_logic_without_gpu_work():
     """ff_tune_query / ff_tune_set are host logic: the default configurations
-    of the column passes of 1024 and of 32..160 and 256, and of the x passes
+    of the column passes of 1024, 256 and 32, 48, 64, 96, 128, 160, and of the x passes
     of 512, 768 and 1000, run the kernels with the line plan compiled in
     (flags without FF_TUNE_GENERIC -- which also needs the compile-time plan
     to equal the planner's); the y passes of 512, 768 and 1000 and other

@@ -218,7 +218,7 @@ def test_column_variants_agree(n, E, variant):
     values as the default kernel.  They are separate kernels, so the compiler
     may contract multiply-adds differently: results agree to round-off, and
     all match the oracle.  Two groups apply where E is 8 or 16.  Compiled-in
-    plans exist for 1024 (E 16), 32..160 and 256, and for the x passes of
+    plans exist for 1024 (E 16), 32, 48, 64, 96, 128, 160 and 256, and for the x passes of
     512, 768 and 1000."""
     rng = np.random.default_rng(5)
     f = rng.standard_normal(n)
