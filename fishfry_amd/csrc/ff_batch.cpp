@@ -753,7 +753,7 @@ ff_status ff_batch_autotune(ff_batch* b, double* const* density, double* const* 
         bool ok;
         if (kind == 0 || kind == 3) {
           RowCfg c;
-          ok = row_cfg_with(p->n[2], eb, c, E, T) && c.line.n == n;
+          ok = row_cfg_with(p->n[2], eb, c, E, T, 0) && c.line.n == n;
           if (ok) (kind == 0 ? trial.zr : trial.zi)[prec] = c;
         } else {
           ColCfg c, c0;

@@ -200,6 +200,7 @@ ff_status ff_tune_query(int kind, int n, int precision, int ncols, ff_tune* out)
       return set_error(FF_ERR_UNSUPPORTED, "ff_tune_query: no plan for that length");
     E = c.line.E;
     T = c.T;
+    flags = c.stat ? 0 : FF_TUNE_GENERIC;
   } else {
     ColCfg c;
     if (!make_col_cfg(n, ncols, eb, c, kind, precision))

@@ -101,9 +101,15 @@ hipError_t PassLaunchers<FF_E>::col(int f32, int mode, int split, int groups, in
   return launch(fn, l, &a, sizeof(a));
 }
 
-template <bool BL>
+template <bool BL, int SN = 0>
 static const void* r2c_fn(int f32, int in_f32, int odd) {
   constexpr int E = FF_E;
+  if constexpr (SN > 0) {  // compiled-in plans: even nz only
+    if (odd) return nullptr;
+    if (!f32) return in_f32 ? nullptr : (const void*)&k_r2c<double, double, E, false, BL, SN>;
+    return in_f32 ? (const void*)&k_r2c<float, float, E, false, BL, SN>
+                  : (const void*)&k_r2c<float, double, E, false, BL, SN>;
+  }
   if (!f32) {
     if (in_f32) return nullptr;
     return odd ? (const void*)&k_r2c<double, double, E, true, BL> : (const void*)&k_r2c<double, double, E, false, BL>;
@@ -113,19 +119,39 @@ static const void* r2c_fn(int f32, int in_f32, int odd) {
   return odd ? (const void*)&k_r2c<float, float, E, true, BL> : (const void*)&k_r2c<float, float, E, false, BL>;
 }
 
+// Row kernels with the plan of a kStaticRows entry of this E compiled in.
+template <int K>
+static const void* r2c_static_fn(int f32, int in_f32, int odd, int M) {
+  if constexpr (K < kNumStaticRows) {
+    constexpr StaticRow sr = kStaticRows[K];
+    if constexpr (sr.E == FF_E) {
+      if (sr.M == M) return r2c_fn<false, sr.M>(f32, in_f32, odd);
+    }
+    return r2c_static_fn<K + 1>(f32, in_f32, odd, M);
+  } else {
+    return nullptr;
+  }
+}
+
 template <>
-hipError_t PassLaunchers<FF_E>::r2c(int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a) {
-  const void* fn = blue ? r2c_fn<true>(f32, in_f32, odd) : r2c_fn<false>(f32, in_f32, odd);
+hipError_t PassLaunchers<FF_E>::r2c(int f32, int in_f32, int odd, int blue, int stat, const Launch& l,
+                                    const RowArgs& a) {
+  const void* fn = stat ? r2c_static_fn<0>(f32, in_f32, odd, a.lp.n)
+                        : (blue ? r2c_fn<true>(f32, in_f32, odd) : r2c_fn<false>(f32, in_f32, odd));
   if (!fn) return hipErrorInvalidValue;
   return launch(fn, l, &a, sizeof(a));
 }
 
 // out_kind: 0 = double output, 1 = float output.
-template <bool BL>
+template <bool BL, int SN = 0>
 static const void* c2r_fn(int f32, int out_kind, int odd, int epi) {
   constexpr int E = FF_E;
-#define FF_PICK(F, OUT, EPI) \
-  return odd ? (const void*)&k_c2r<F, OUT, E, true, EPI, BL> : (const void*)&k_c2r<F, OUT, E, false, EPI, BL>
+  // (compiled-in plans: even nz only)
+#define FF_PICK(F, OUT, EPI)                                                                           \
+  if constexpr (SN > 0)                                                                                \
+    return odd ? nullptr : (const void*)&k_c2r<F, OUT, E, false, EPI, BL, SN>;                         \
+  else                                                                                                 \
+    return odd ? (const void*)&k_c2r<F, OUT, E, true, EPI, BL> : (const void*)&k_c2r<F, OUT, E, false, EPI, BL>
   if (!f32) {
     if (epi == EPI_STORE && out_kind == 0) FF_PICK(double, double, EPI_STORE);
     if (epi == EPI_RESIDUAL && out_kind == 1) FF_PICK(double, float, EPI_RESIDUAL);
@@ -138,10 +164,24 @@ static const void* c2r_fn(int f32, int out_kind, int odd, int epi) {
 #undef FF_PICK
 }
 
+template <int K>
+static const void* c2r_static_fn(int f32, int out_kind, int odd, int epi, int M) {
+  if constexpr (K < kNumStaticRows) {
+    constexpr StaticRow sr = kStaticRows[K];
+    if constexpr (sr.E == FF_E) {
+      if (sr.M == M) return c2r_fn<false, sr.M>(f32, out_kind, odd, epi);
+    }
+    return c2r_static_fn<K + 1>(f32, out_kind, odd, epi, M);
+  } else {
+    return nullptr;
+  }
+}
+
 template <>
-hipError_t PassLaunchers<FF_E>::c2r(int f32, int out_kind, int odd, int epi, int blue, const Launch& l,
+hipError_t PassLaunchers<FF_E>::c2r(int f32, int out_kind, int odd, int epi, int blue, int stat, const Launch& l,
                                     const RowArgs& a) {
-  const void* fn = blue ? c2r_fn<true>(f32, out_kind, odd, epi) : c2r_fn<false>(f32, out_kind, odd, epi);
+  const void* fn = stat ? c2r_static_fn<0>(f32, out_kind, odd, epi, a.lp.n)
+                        : (blue ? c2r_fn<true>(f32, out_kind, odd, epi) : c2r_fn<false>(f32, out_kind, odd, epi));
   if (!fn) return hipErrorInvalidValue;
   return launch(fn, l, &a, sizeof(a));
 }

@@ -78,6 +78,7 @@ struct RowCfg {
   size_t lds = 0;
   int threads = 0;
   uint32_t Linv = 0;
+  int stat = 0;  // a kStaticRows kernel runs this configuration (FF_TUNE_GENERIC clears it)
 };
 
 // kind: 0 z forward rows, 1 y columns, 2 x columns, 3 z inverse rows;
@@ -86,7 +87,7 @@ struct RowCfg {
 bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int kind, int prec);
 bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int kind, int prec);
 bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, int flags, int kind);
-bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T);
+bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T, int flags);
 bool tune_lookup(int kind, int n, int prec, int& E, int& T, int& flags);
 ff_status tune_set(const ff_tune* t);
 void tune_reset();
@@ -107,8 +108,9 @@ struct PassLaunch {
 };
 void set_launch_recorder(std::vector<PassLaunch>* rec);
 hipError_t replay_launch(const PassLaunch& r);
-hipError_t launch_r2c(int E, int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a);
-hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, const Launch& l, const RowArgs& a);
+hipError_t launch_r2c(int E, int f32, int in_f32, int odd, int blue, int stat, const Launch& l, const RowArgs& a);
+hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, int stat, const Launch& l,
+                      const RowArgs& a);
 
 // ---- driver kernels ---------------------------------------------------------
 ff_status field_check(const ff_field* f);

@@ -356,14 +356,15 @@ __device__ __forceinline__ long long spec_row(const RowArgs& a, long long row) {
 // z forward: real rows (IN precision) -> half-spectrum rows (F precision).
 // Even nz: pack (x[2r], x[2r+1]) as one complex, an nz/2-point transform, then
 // split the even/odd spectra.  Odd nz: a full nz-point transform of (x, 0).
-template <typename F, typename IN, int E, bool ODD, bool BL>
+// SN > 0: the plan of the row transform, length SN, is compiled in (even nz).
+template <typename F, typename IN, int E, bool ODD, bool BL, int SN = 0>
 __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
   const PassInst io = inst_of(a);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
   const int tid = threadIdx.x;
-  const int L = a.lp.L;
-  const int line = L == 1 ? tid : (int)__umulhi((uint32_t)tid, a.Linv);
+  const int L = SN > 0 ? SN / E : a.lp.L;
+  const int line = SN > 0 ? tid / L : (L == 1 ? tid : (int)__umulhi((uint32_t)tid, a.Linv));
   const int r0 = tid - line * L;
   const long long row = (long long)xcd_remap(blockIdx.x, a.nblocks) * a.T + line;
   const bool valid = row < a.nrows;
@@ -390,7 +391,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
         v[i] = cx<F>{F(0), F(0)};
       }
     }
-    line_dft<E, -1, F, RowIdx, false, BL>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    line_dft<E, -1, F, RowIdx, false, BL, SN>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
     __syncthreads();
     put_natural<E, F>(v, lds, idx, r0, L);
     __syncthreads();
@@ -420,7 +421,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
       const int k = r0 + i * L;
       v[i] = cx<F>{(!blue || k < M) ? F(src[k]) : F(0), F(0)};
     }
-    line_dft<E, -1, F, RowIdx, false, BL>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    line_dft<E, -1, F, RowIdx, false, BL, SN>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
     FF_GLB cx<F>* __restrict__ dst = glb<cx<F>>(io.dst) + (valid ? spec_row(a, row) : 0);
     if (valid) {
 #pragma unroll
@@ -467,14 +468,15 @@ __device__ __forceinline__ void c2r_out2(FF_GLB OUT* out, const FF_GLB double* f
 // z inverse: half-spectrum rows (F) -> real rows (OUT) with an epilogue.
 // The imaginary parts of the DC and (even nz) Nyquist terms are ignored,
 // i.e. the rows are read as Hermitian half spectra.
-template <typename F, typename OUT, int E, bool ODD, int EPI, bool BL>
+// SN > 0: the plan of the row transform, length SN, is compiled in (even nz).
+template <typename F, typename OUT, int E, bool ODD, int EPI, bool BL, int SN = 0>
 __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
   const PassInst io = inst_of(a);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
   const int tid = threadIdx.x;
-  const int L = a.lp.L;
-  const int line = L == 1 ? tid : (int)__umulhi((uint32_t)tid, a.Linv);
+  const int L = SN > 0 ? SN / E : a.lp.L;
+  const int line = SN > 0 ? tid / L : (L == 1 ? tid : (int)__umulhi((uint32_t)tid, a.Linv));
   const int r0 = tid - line * L;
   const long long row = (long long)xcd_remap(blockIdx.x, a.nblocks) * a.T + line;
   const bool valid = row < a.nrows;
@@ -554,7 +556,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
       v[i] = y;
     }
   }
-  line_dft<E, +1, F, RowIdx, false, BL>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+  line_dft<E, +1, F, RowIdx, false, BL, SN>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   if (valid) {
     const double shift = io.shiftp ? a.shift * *glb<const double>(io.shiftp) : a.shift;
     FF_GLB OUT* out = glb<OUT>(io.dst) + row * a.nz;
@@ -606,12 +608,22 @@ constexpr StaticCol kStaticCols[] = {{1024, 16, 4, 0, 0, 2, 6}, {1024, 16, 8, 1,
                                      {48, 12, 32, 1, 0, 4, 6}, {32, 8, 32, 1, 0, 4, 6}};
 constexpr int kNumStaticCols = sizeof(kStaticCols) / sizeof(kStaticCols[0]);
 
+// Row transforms (z passes at even nz, length M = nz/2) with the plan of
+// (M, E) compiled in: the default configurations of nz = 32 ... 1024.
+struct StaticRow {
+  int M, E;
+};
+constexpr StaticRow kStaticRows[] = {{512, 8}, {256, 16}, {128, 16}, {80, 20}, {64, 8},
+                                     {48, 12}, {32, 8},   {24, 12},  {16, 16}};
+constexpr int kNumStaticRows = sizeof(kStaticRows) / sizeof(kStaticRows[0]);
+
 template <int E>
 struct PassLaunchers {
   static hipError_t col(int f32, int mode, int split, int groups, int blue, int stat, const Launch& l,
                         const ColArgs& a);
-  static hipError_t r2c(int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a);
-  static hipError_t c2r(int f32, int out_kind, int odd, int epi, int blue, const Launch& l, const RowArgs& a);
+  static hipError_t r2c(int f32, int in_f32, int odd, int blue, int stat, const Launch& l, const RowArgs& a);
+  static hipError_t c2r(int f32, int out_kind, int odd, int epi, int blue, int stat, const Launch& l,
+                        const RowArgs& a);
 };
 
 }  // namespace ff

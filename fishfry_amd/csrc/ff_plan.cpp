@@ -432,7 +432,26 @@ bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, i
   return out.lds <= 160 * 1024;
 }
 
-bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T) {
+template <int K>
+static bool static_row_plan_matches(int M, int E, const LinePlan& lp) {
+  if constexpr (K < kNumStaticRows) {
+    constexpr StaticRow sr = kStaticRows[K];
+    if (sr.M == M && sr.E == E) {
+      constexpr LinePlan s = static_plan<sr.M, sr.E>();
+      if (s.n != lp.n || s.L != lp.L || s.np != lp.np) return false;
+      for (int p = 0; p < s.np; ++p)
+        if (s.radix[p] != lp.radix[p] || s.ns[p] != lp.ns[p] || s.nsinv[p] != lp.nsinv[p] ||
+            s.twstep[p] != lp.twstep[p])
+          return false;
+      return true;
+    }
+    return static_row_plan_matches<K + 1>(M, E, lp);
+  } else {
+    return false;
+  }
+}
+
+bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T, int flags) {
   const int M = (nz % 2 == 0) ? nz / 2 : nz;
   if (!plan_line(M, out.line, E)) return false;
   const int L = out.line.lp.L;
@@ -453,6 +472,9 @@ bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T) {
   out.threads = T * L;
   out.Linv = L > 1 ? (uint32_t)(((1ull << 32) + (uint64_t)L - 1) / (uint64_t)L) : 0u;
   out.lds = (size_t)T * out.pitch * elem_bytes;
+  out.stat = 0;
+  if (!(flags & FF_TUNE_GENERIC) && nz % 2 == 0 && !out.line.bluestein() && out.ps == 4)
+    out.stat = static_row_plan_matches<0>(M, out.line.E, out.line.lp) ? 1 : 0;
   return out.lds <= 160 * 1024;
 }
 
@@ -493,9 +515,9 @@ bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int kind, int p
 bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int kind, int prec) {
   const int M = (nz % 2 == 0) ? nz / 2 : nz;
   int E = 0, T = 0, flags = 0;
-  if (tune_lookup(kind, M, prec, E, T, flags) && row_cfg_with(nz, elem_bytes, out, E, T)) return true;
-  if (builtin_lookup(kind, M, prec, E, T) && row_cfg_with(nz, elem_bytes, out, E, T)) return true;
-  return row_cfg_with(nz, elem_bytes, out, 0, 0);
+  if (tune_lookup(kind, M, prec, E, T, flags) && row_cfg_with(nz, elem_bytes, out, E, T, flags)) return true;
+  if (builtin_lookup(kind, M, prec, E, T) && row_cfg_with(nz, elem_bytes, out, E, T, 0)) return true;
+  return row_cfg_with(nz, elem_bytes, out, 0, 0, 0);
 }
 
 ff_status tune_set(const ff_tune* t) {
@@ -513,7 +535,7 @@ ff_status tune_set(const ff_tune* t) {
   bool ok;
   if (t->kind == 0 || t->kind == 3) {
     RowCfg c;
-    ok = t->flags == 0 && row_cfg_with(2 * t->n, eb, c, t->E, t->T);
+    ok = (t->flags & ~FF_TUNE_GENERIC) == 0 && row_cfg_with(2 * t->n, eb, c, t->E, t->T, t->flags);
   } else {
     ColCfg c;
     ok = col_cfg_with(t->n, 1 << 20, eb, c, t->E, t->T, t->flags, t->kind);
@@ -535,9 +557,9 @@ void tune_reset() {
   template <>                                                                                              \
   hipError_t PassLaunchers<E>::col(int, int, int, int, int, int, const Launch&, const ColArgs&);                     \
   template <>                                                                                              \
-  hipError_t PassLaunchers<E>::r2c(int, int, int, int, const Launch&, const RowArgs&);                     \
+  hipError_t PassLaunchers<E>::r2c(int, int, int, int, int, const Launch&, const RowArgs&);                \
   template <>                                                                                              \
-  hipError_t PassLaunchers<E>::c2r(int, int, int, int, int, const Launch&, const RowArgs&);
+  hipError_t PassLaunchers<E>::c2r(int, int, int, int, int, int, const Launch&, const RowArgs&);
 FF_DECL(1)
 FF_DECL(2)
 FF_DECL(3)
@@ -581,12 +603,13 @@ static hipError_t do_col(int E, int f32, int mode, int split, int groups, int bl
                          const ColArgs& a) {
   FF_SWITCH_E(E, col(f32, mode, split, groups, blue, stat, l, a))
 }
-static hipError_t do_r2c(int E, int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a) {
-  FF_SWITCH_E(E, r2c(f32, in_f32, odd, blue, l, a))
-}
-static hipError_t do_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, const Launch& l,
+static hipError_t do_r2c(int E, int f32, int in_f32, int odd, int blue, int stat, const Launch& l,
                          const RowArgs& a) {
-  FF_SWITCH_E(E, c2r(f32, out_kind, odd, epi, blue, l, a))
+  FF_SWITCH_E(E, r2c(f32, in_f32, odd, blue, stat, l, a))
+}
+static hipError_t do_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, int stat, const Launch& l,
+                         const RowArgs& a) {
+  FF_SWITCH_E(E, c2r(f32, out_kind, odd, epi, blue, stat, l, a))
 }
 
 hipError_t launch_col(int E, int f32, int mode, int split, int groups, int blue, int stat, const Launch& l,
@@ -600,31 +623,34 @@ hipError_t launch_col(int E, int f32, int mode, int split, int groups, int blue,
   }
   return do_col(E, f32, mode, split, groups, blue, stat, l, a);
 }
-hipError_t launch_r2c(int E, int f32, int in_f32, int odd, int blue, const Launch& l, const RowArgs& a) {
+hipError_t launch_r2c(int E, int f32, int in_f32, int odd, int blue, int stat, const Launch& l,
+                      const RowArgs& a) {
   if (t_recorder) {
     PassLaunch r{};
-    r.kind = 1, r.E = E, r.f32 = f32, r.sel = in_f32, r.odd = odd, r.blue = blue, r.l = l, r.row = a;
-    t_recorder->push_back(r);
-    return hipSuccess;
-  }
-  return do_r2c(E, f32, in_f32, odd, blue, l, a);
-}
-hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, const Launch& l, const RowArgs& a) {
-  if (t_recorder) {
-    PassLaunch r{};
-    r.kind = 2, r.E = E, r.f32 = f32, r.sel = out_kind, r.odd = odd, r.epi = epi, r.blue = blue, r.l = l;
+    r.kind = 1, r.E = E, r.f32 = f32, r.sel = in_f32, r.odd = odd, r.blue = blue, r.stat = stat, r.l = l;
     r.row = a;
     t_recorder->push_back(r);
     return hipSuccess;
   }
-  return do_c2r(E, f32, out_kind, odd, epi, blue, l, a);
+  return do_r2c(E, f32, in_f32, odd, blue, stat, l, a);
+}
+hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, int stat, const Launch& l,
+                      const RowArgs& a) {
+  if (t_recorder) {
+    PassLaunch r{};
+    r.kind = 2, r.E = E, r.f32 = f32, r.sel = out_kind, r.odd = odd, r.epi = epi, r.blue = blue, r.stat = stat;
+    r.l = l, r.row = a;
+    t_recorder->push_back(r);
+    return hipSuccess;
+  }
+  return do_c2r(E, f32, out_kind, odd, epi, blue, stat, l, a);
 }
 
 hipError_t replay_launch(const PassLaunch& r) {
   switch (r.kind) {
     case 0: return do_col(r.E, r.f32, r.sel, r.split, r.groups, r.blue, r.stat, r.l, r.col);
-    case 1: return do_r2c(r.E, r.f32, r.sel, r.odd, r.blue, r.l, r.row);
-    case 2: return do_c2r(r.E, r.f32, r.sel, r.odd, r.epi, r.blue, r.l, r.row);
+    case 1: return do_r2c(r.E, r.f32, r.sel, r.odd, r.blue, r.stat, r.l, r.row);
+    case 2: return do_c2r(r.E, r.f32, r.sel, r.odd, r.epi, r.blue, r.stat, r.l, r.row);
     default: return hipErrorInvalidValue;
   }
 }

@@ -268,7 +268,7 @@ ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void
   a.src = static_cast<const unsigned char*>(src) + row0 * p->n[2] * (in_f32 ? 4 : 8);
   a.dst = static_cast<unsigned char*>(dst) + row0 * p->nh * (prec ? 8 : 16);
   const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
-  return hip_check(launch_r2c(c.line.E, prec, in_f32, p->n[2] % 2, c.line.bluestein(), l, a), "z forward launch");
+  return hip_check(launch_r2c(c.line.E, prec, in_f32, p->n[2] % 2, c.line.bluestein(), c.stat, l, a), "z forward launch");
 }
 
 ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void* src, void* dst, const double* f,
@@ -290,7 +290,7 @@ ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void
   }
   const RowCfg& c = p->zi[prec];
   const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
-  return hip_check(launch_c2r(c.line.E, prec, out_kind, p->n[2] % 2, epi, c.line.bluestein(), l, a),
+  return hip_check(launch_c2r(c.line.E, prec, out_kind, p->n[2] % 2, epi, c.line.bluestein(), c.stat, l, a),
                    "z inverse launch");
 }
 

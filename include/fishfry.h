@@ -143,7 +143,7 @@ typedef struct ff_tune {
   int precision;
   int E;
   int T;
-  int flags; /* FF_TUNE_SPLIT_LDS | FF_TUNE_TWO_GROUPS | FF_TUNE_GENERIC (column passes) */
+  int flags; /* FF_TUNE_SPLIT_LDS | FF_TUNE_TWO_GROUPS (column passes) | FF_TUNE_GENERIC */
 } ff_tune;
 
 /* ff_tune.flags (column passes):

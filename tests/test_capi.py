@@ -156,8 +156,19 @@ def test_tuning_host_logic_without_gpu_work():
         with pytest.raises(ff.FishFryError):
             ff.tune_set(2, 1024, 0, 16, 4, 8)  # unknown flag
         with pytest.raises(ff.FishFryError):
-            ff.tune_set(0, 512, 0, 8, 2, ff.FF_TUNE_GENERIC)  # row passes take no flags
+            ff.tune_set(0, 512, 0, 8, 2, ff.FF_TUNE_SPLIT_LDS)  # row passes take FF_TUNE_GENERIC only
         ff.tune_reset()
         assert ff.tune_query(2, 1024, 0, 513).flags == 0
+        # row passes: compiled-in plans at the default configurations of even
+        # nz = 32 ... 1024 (length nz/2), generic otherwise or on request
+        for kind in (0, 3):
+            for M, E in ((512, 8), (256, 16), (80, 20), (16, 16)):
+                for prec in (0, 1):
+                    t = ff.tune_query(kind, M, prec, 1)
+                    assert (t.E, t.flags) == (E, 0), (kind, M, prec)
+            assert ff.tune_query(kind, 500, 0, 1).flags == ff.FF_TUNE_GENERIC
+            ff.tune_set(kind, 512, 0, 8, 2, ff.FF_TUNE_GENERIC)
+            assert ff.tune_query(kind, 512, 0, 1).flags == ff.FF_TUNE_GENERIC
+        ff.tune_reset()
     finally:
         ff.tune_reset()
