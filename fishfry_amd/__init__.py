@@ -318,7 +318,7 @@ class ParisPencil:
 
 
 def tune_set(kind: int, n: int, precision: int, E: int, T: int, flags: int = 0) -> None:
-    """Override E / T (and FF_TUNE_SPLIT_LDS / FF_TUNE_TWO_GROUPS / FF_TUNE_GENERIC for column passes) for passes of
+    """Override E / T (and FF_TUNE_SPLIT_LDS / FF_TUNE_TWO_GROUPS for column passes, FF_TUNE_GENERIC for any) for passes of
     kind (0 z forward rows, 1 y, 2 x, 3 z inverse rows) and length n."""
     t = Tune(kind, n, precision, E, T, flags)
     check(lib().ff_tune_set(ctypes.byref(t)), "tune_set")
