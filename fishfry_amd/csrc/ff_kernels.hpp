@@ -609,12 +609,13 @@ constexpr StaticCol kStaticCols[] = {{1024, 16, 4, 0, 0, 2, 6}, {1024, 16, 8, 1,
 constexpr int kNumStaticCols = sizeof(kStaticCols) / sizeof(kStaticCols[0]);
 
 // Row transforms (z passes at even nz, length M = nz/2) with the plan of
-// (M, E) compiled in: the default configurations of nz = 32 ... 1024.
+// (M, E) compiled in: the default configurations of nz = 32 ... 1024
+// (768 has E = 12 in FP64 and E = 24 in FP32).
 struct StaticRow {
   int M, E;
 };
-constexpr StaticRow kStaticRows[] = {{512, 8}, {256, 16}, {128, 16}, {80, 20}, {64, 8},
-                                     {48, 12}, {32, 8},   {24, 12},  {16, 16}};
+constexpr StaticRow kStaticRows[] = {{512, 8}, {500, 10}, {384, 12}, {384, 24}, {256, 16}, {128, 16}, {80, 20},
+                                     {64, 8},  {48, 12},  {32, 8},   {24, 12},  {16, 16}};
 constexpr int kNumStaticRows = sizeof(kStaticRows) / sizeof(kStaticRows[0]);
 
 template <int E>

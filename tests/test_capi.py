@@ -166,7 +166,7 @@ def test_tuning_host_logic_without_gpu_work():
                 for prec in (0, 1):
                     t = ff.tune_query(kind, M, prec, 1)
                     assert (t.E, t.flags) == (E, 0), (kind, M, prec)
-            assert ff.tune_query(kind, 500, 0, 1).flags == ff.FF_TUNE_GENERIC
+            assert ff.tune_query(kind, 450, 0, 1).flags == ff.FF_TUNE_GENERIC
             ff.tune_set(kind, 512, 0, 8, 2, ff.FF_TUNE_GENERIC)
             assert ff.tune_query(kind, 512, 0, 1).flags == ff.FF_TUNE_GENERIC
         ff.tune_reset()
