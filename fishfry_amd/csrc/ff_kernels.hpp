@@ -605,7 +605,8 @@ constexpr StaticCol kStaticCols[] = {{1024, 16, 4, 0, 0, 2, 6}, {1024, 16, 8, 1,
                                      {160, 20, 32, 0, 0, 2, 6}, {96, 24, 64, 0, 0, 1, 6}, {64, 8, 32, 0, 0, 4, 6},
                                      {48, 12, 32, 0, 0, 3, 6}, {32, 8, 32, 0, 0, 4, 6}, {128, 16, 32, 1, 0, 2, 6},
                                      {160, 20, 32, 1, 0, 2, 6}, {96, 24, 64, 1, 0, 2, 6}, {64, 8, 32, 1, 0, 4, 6},
-                                     {48, 12, 32, 1, 0, 4, 6}, {32, 8, 32, 1, 0, 4, 6}};
+                                     {48, 12, 32, 1, 0, 4, 6}, {32, 8, 32, 1, 0, 4, 6}, {256, 16, 16, 1, 0, 2, 6},
+                                     {512, 8, 8, 1, 0, 4, 4}};
 constexpr int kNumStaticCols = sizeof(kStaticCols) / sizeof(kStaticCols[0]);
 
 // Row transforms (z passes at even nz, length M = nz/2) with the plan of
