@@ -82,31 +82,65 @@ def random_modes(seed, count=16, kmax=12):
     return out
 
 
-def cpu_baseline(size, budget_s=25.0):
-    import numpy as np
+def host_cpus():
+    """CPUs this process may run on: the affinity mask, capped by a cgroup
+    CPU quota where one is set; plus nproc and the CPU model for the record."""
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = nproc
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            q, per = fh.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(math.ceil(int(q) / int(per))))
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    use = min(aff, quota) if quota else aff
+    return use, nproc, aff, quota, model
 
+
+def cpu_baseline(size, budget_s=30.0, runs=10):
+    """The CPU oracle's FP64 solve of the benchmark f at size^3 on every CPU
+    this process may use: 1 warmup, then up to `runs` timed solves within
+    budget_s seconds (the sample is bounded so the benchmark stays short)."""
     import oracle  # the CPU oracle: timed here as the reported baseline only
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, nproc, aff, quota, model = host_cpus()
     n = (size, size, size)
     f = oracle.init_field(n)
     times = []
-    t_start = time.perf_counter()
     oracle.solve(f, nthreads=threads)  # warmup
-    while len(times) < 3 and time.perf_counter() - t_start < budget_s:
+    t_start = time.perf_counter()
+    while len(times) < runs and (not times or time.perf_counter() - t_start < budget_s):
         t0 = time.perf_counter()
         phi = oracle.solve(f, nthreads=threads)
         times.append(time.perf_counter() - t0)
     l1, l2, linf = oracle.norms(phi)
     del f, phi
-    t = sorted(times)[len(times) // 2]
+    ts = sorted(times)
+    t = pct(ts, 0.5)
     return {
         "value": float(n[0] * n[1] * n[2] / t),
         "unit": "points/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"CPU oracle FP64 solve (same algorithm: R2C z, C2C y, C2C x, filter, inverses) at {size}^3, "
-                  f"OpenMP over lines, 1 warmup + {len(times)} timed, median {t:.3f} s, Linf {linf:.2e}",
+        "sample": f"CPU oracle FP64 solve (same algorithm: R2C z, C2C y, C2C x, filter, inverses) of the "
+                  f"benchmark f at {size}^3, OpenMP over lines on {threads} threads (nproc {nproc}, affinity {aff}, "
+                  f"cgroup CPU quota {quota if quota else 'none'}; {model}), 1 warmup + {len(times)} timed "
+                  f"(at most {runs} within {budget_s:.0f} s), p50 {t:.3f} s, p99 {pct(ts, 0.99):.3f} s, "
+                  f"Linf {linf:.2e}",
     }
 
 
@@ -144,8 +178,9 @@ def bench_batching(ff, torch, K):
     """The batching configuration: 64 grids drawn (seeded) from {32^3, 48^3,
     64^3, 96^3, 128^3, 160^3, 128x64x96}, f_b(x) = s_b f(x + delta_b), one
     HIP-graph batch per precision, autotuned, then the same batch grouped (one
-    launch per pass for each shape); aggregate points/s and the worst Linf
-    against phi_b = s_b phi(x + delta_b)."""
+    launch per pass for each shape) and step by step (one multi-shape launch
+    per pass step over all 64 grids); aggregate points/s, launches per batch
+    and the worst Linf against phi_b = s_b phi(x + delta_b)."""
     import numpy as np
 
     sizes = [(32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 128, 128), (160, 160, 160),
@@ -201,16 +236,43 @@ def bench_batching(ff, torch, K):
         g.wait(stream)
         stream.synchronize()
         gms = 1e3 * (time.perf_counter() - t0) / K
-        ngroups = len(json.loads(g.describe())["groups"])
+        gdesc = json.loads(g.describe())
+        ngroups = len(gdesc["groups"])
+        glaunches = sum(x["launches"] for x in gdesc["groups"])
         g.close()
+        # the same batch step by step: one multi-shape launch per pass step
+        # over all 64 instances (descriptor table), captured in a graph
+        sb = ff.Batch(items, ff.FF_BATCH_GRAPH | ff.FF_BATCH_STEPS)
+        for k in range(64):
+            ff.field_init(fields[k], d[k])
+        torch.cuda.synchronize()
+        sb.solve(d, p, stream)
+        sb.wait(stream)
+        stream.synchronize()
+        sworst = max(ff.field_norms(fields[k], p[k])[2] for k in range(64))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            sb.solve(d, p, stream)
+        sb.wait(stream)
+        stream.synchronize()
+        sms = 1e3 * (time.perf_counter() - t0) / K
+        fams = json.loads(sb.describe())["steps"]
+        slaunches = sum(x["launches"] for x in fams)
+        sb.close()
         out.append({"precision": "fp64" if precision == ff.FF_FP64 else "fp32+2", "grids": 64, "points": points,
                     "ms_per_batch": round(ms, 4), "points_per_s": points / (ms * 1e-3), "worst_linf": worst,
+                    "launches": len(shapes) * (5 if precision == ff.FF_FP64 else 5 + 10 * steps),
                     "autotuned_passes": tuned, "ms_before_tuning": round(before, 4), "ms_after_tuning": round(after, 4),
-                    "grouped": {"groups": ngroups, "ms_per_batch": round(gms, 4), "points_per_s": points / (gms * 1e-3),
-                                "worst_linf": gworst}})
+                    "grouped": {"groups": ngroups, "launches": glaunches, "ms_per_batch": round(gms, 4),
+                                "points_per_s": points / (gms * 1e-3), "worst_linf": gworst},
+                    "steps": {"launches": slaunches, "ms_per_batch": round(sms, 4),
+                              "points_per_s": points / (sms * 1e-3), "worst_linf": sworst}})
         log(f"batch {out[-1]['precision']}: {ms:.3f} ms per 64-grid batch, {points / (ms * 1e-3):.4e} points/s, "
             f"worst Linf {worst:.3e}, {tuned} passes tuned ({before:.3f} -> {after:.3f} ms); grouped ({ngroups} "
-            f"groups): {gms:.3f} ms, {points / (gms * 1e-3):.4e} points/s, worst Linf {gworst:.3e}")
+            f"groups, {glaunches} launches): {gms:.3f} ms, {points / (gms * 1e-3):.4e} points/s, worst Linf "
+            f"{gworst:.3e}; steps ({slaunches} launches): {sms:.3f} ms, {points / (sms * 1e-3):.4e} points/s, "
+            f"worst Linf {sworst:.3e}")
         del d, p
         torch.cuda.empty_cache()
     return out
@@ -283,7 +345,7 @@ def main():
     ap.add_argument("--size", type=int, nargs=3, default=[1024, 1024, 1024])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32+refinement Pareto")
-    ap.add_argument("--cpu-size", type=int, default=512)
+    ap.add_argument("--cpu-size", type=int, default=1024)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other plain-method sizes, the batching and the serving workloads")
     args = ap.parse_args()

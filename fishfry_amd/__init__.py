@@ -103,6 +103,7 @@ class ServingStats(ctypes.Structure):
 FF_BATCH_GRAPH = 1
 FF_BATCH_PERSISTENT = 4
 FF_BATCH_GROUPED = 8
+FF_BATCH_STEPS = 16
 
 
 class Field(ctypes.Structure):
@@ -505,7 +506,7 @@ def field_norms(field: Field, phi, stream=None) -> Tuple[float, float, float]:
 from .driver import FishFry  # noqa: E402  (needs the names above)
 
 __all__ = [
-    "ParisPencil", "FishFry", "Batch", "Server", "FF_BATCH_GRAPH", "FF_BATCH_PERSISTENT", "FF_BATCH_GROUPED", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
+    "ParisPencil", "FishFry", "Batch", "Server", "FF_BATCH_GRAPH", "FF_BATCH_PERSISTENT", "FF_BATCH_GROUPED", "FF_BATCH_STEPS", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
     "field_norms", "reference_bytes", "benchmark_grid", "FishFryError", "lib", "LIB_PATH",
     "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_BLOCKED_LAYOUT", "FF_SLAB_PIPELINE", "FF_TUNE_SPLIT_LDS", "FF_TUNE_TWO_GROUPS", "FF_TUNE_GENERIC",
 ]

@@ -2,7 +2,9 @@
 // tile shapes, resident twiddle tables, the persistent launch (optionally
 // captured in a HIP graph) and the autotuner.
 #include <algorithm>
+#include <climits>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -15,6 +17,7 @@
 
 #include "ff_batch.hpp"
 #include "ff_internal.hpp"
+#include "ff_multi.hpp"
 #include "ff_solver.hpp"
 
 namespace ff {
@@ -410,9 +413,32 @@ using namespace ff;
 // The single persistent launch over every instance (BatchCore, k_batch) is
 // switched off: its kernel did not complete on MI355X and the cause is not
 // established, so FF_BATCH_PERSISTENT is rejected.
+//
+// With FF_BATCH_STEPS, pass step s of every instance of a family (instances
+// whose solves launch the same kernel roles step by step: same precision and
+// refinement count) is ONE launch of a multi-shape kernel (ff_multi.hpp): a
+// descriptor per instance holds the pass arguments its single solve would
+// launch with, and each workgroup finds its instance's descriptor and tile.
+// Instances whose line length has no compiled-in multi-shape variant run that
+// step as grouped launches (one per shape) beside it, on the same stream.
 struct BatchGroup {
   std::vector<int> inst;         // instances, in batch order
   std::vector<PassLaunch> ops;   // the group's launches (per-instance fields from the table)
+};
+
+struct StepLaunch {
+  int kind = -1;  // MULTI_COL / MULTI_R2C / MULTI_C2R: a multi-shape launch; -1: op
+  int f32 = 0, sel = 0, epi = 0;
+  size_t lds = 0;
+  MultiArgs m{};  // descriptor table and first workgroups (device)
+  PassLaunch op{};
+};
+
+struct StepFamily {
+  std::vector<int> inst;
+  int steps = 0;
+  int multi_launches = 0;
+  std::vector<StepLaunch> ops;
 };
 
 struct ff_batch {
@@ -432,6 +458,13 @@ struct ff_batch {
   size_t inst_cap = 0;
   bool groups_ok = false;
   std::vector<double*> gd, gp;
+  // FF_BATCH_STEPS: families and their descriptor tables, valid for the
+  // plans and the buffers sd / sp they were built with
+  std::vector<StepFamily> fams;
+  void* step_dev = nullptr;
+  size_t step_cap = 0;
+  bool steps_ok = false;
+  std::vector<double*> sd, sp;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -456,6 +489,7 @@ void destroy_plans(ff_batch* b) {
 ff_status make_plans(ff_batch* b) {
   destroy_plans(b);
   b->groups_ok = false;
+  b->steps_ok = false;
   for (const ff_batch_item& it : b->items) {
     ff_options o{it.precision, it.refine_steps, 0};
     ff_plan* p = nullptr;
@@ -574,18 +608,279 @@ bool groups_current(const ff_batch* b, double* const* d, double* const* p) {
   return b->groups_ok && std::equal(b->gd.begin(), b->gd.end(), d) && std::equal(b->gp.begin(), b->gp.end(), p);
 }
 
+// ---- FF_BATCH_STEPS -----------------------------------------------------------
+
+// The multi-shape descriptor of a recorded column pass, or false when no
+// multi-shape variant runs it (Bluestein lines, tuning variants, blocked
+// layouts, lengths without a compiled-in variant).  The descriptor keeps the
+// recorded arguments and takes the variant's tile width; lds: the variant's
+// LDS bytes.
+bool col_desc(const PassLaunch& r, MultiColDesc& out, size_t& lds) {
+  const ColArgs& c = r.col;
+  if (r.blue || r.groups != 1 || r.split || c.outer_fast) return false;
+  if (c.in.nb != 1 || c.out.nb != 1 || c.in.lgKT < 30 || c.out.lgKT < 30 || c.nline != c.lp.n) return false;
+  if (!multi_kernel_exists(MULTI_COL, r.f32, r.sel, 0)) return false;
+  const int v = multi_col_variant(c.lp.n, r.E, c.lp);
+  if (v < 0) return false;
+  const MultiVar mv = kMultiCols[v];
+  const int T = multi_tile(mv);
+  ColArgs a = c;
+  a.inst = nullptr;
+  a.T = T;
+  a.lgT = 0;
+  while ((1 << a.lgT) < T) a.lgT++;
+  a.ps = 4;
+  // rows of a plane 64-byte aligned but planes not: shift the tiles onto the
+  // 64-byte grid, as the single-solve kernels with compiled-in plans do
+  const long long mask = r.f32 ? 7 : 3;
+  const long long W = a.out.Wlast;
+  a.cshift_mask = a.cshift_mul = 0;
+  if (((a.out.fr * W) & mask) == 0 && ((a.out.fo * W) & mask) != 0) {
+    a.cshift_mask = (int)mask;
+    a.cshift_mul = (int)((a.out.fo * W) & mask);
+  }
+  a.tiles = (a.ncols + a.cshift_mask + T - 1) / T;
+  const long long nb = (long long)a.nouter * a.tiles;
+  if (nb > INT_MAX / 2) return false;
+  a.nblocks = (int)nb;
+  memset(&out, 0, sizeof out);
+  out.a = a;
+  out.variant = v;
+  lds = ((size_t)mv.n * T + ((mv.n - 1) >> 4) + 1) * (r.f32 ? 8 : 16);
+  return true;
+}
+
+bool row_desc(const PassLaunch& r, MultiRowDesc& out, size_t& lds) {
+  const RowArgs& c = r.row;
+  if (r.blue || r.odd || c.nline != c.lp.n) return false;
+  if (r.kind == 2 && c.spec.lgKT < 30) return false;  // z inverse of a blocked spectrum
+  const int kind = r.kind == 1 ? MULTI_R2C : MULTI_C2R;
+  if (!multi_kernel_exists(kind, r.f32, r.sel, r.kind == 2 ? r.epi : 0)) return false;
+  const int v = multi_row_variant(c.lp.n, r.E, c.lp);
+  if (v < 0) return false;
+  const MultiVar mv = kMultiRows[v];
+  const int T = multi_tile(mv);
+  RowArgs a = c;
+  a.inst = nullptr;
+  a.T = T;
+  a.ps = 4;
+  a.pitch = (mv.n + 1) + (mv.n >> 4) + 1;
+  const long long nb = (a.nrows + T - 1) / T;
+  if (nb > INT_MAX / 2) return false;
+  a.nblocks = (int)nb;
+  memset(&out, 0, sizeof out);
+  out.a = a;
+  out.variant = v;
+  lds = (size_t)T * a.pitch * (r.f32 ? 8 : 16);
+  return true;
+}
+
+// Launch role of a recorded launch: kernel kind, precision, selector, epilogue.
+std::tuple<int, int, int, int> launch_role(const PassLaunch& r) {
+  return std::make_tuple(r.kind, r.f32, r.sel, r.kind == 2 ? r.epi : 0);
+}
+
+void push_bytes(std::vector<unsigned char>& img, const void* p, size_t n, size_t& off) {
+  off = (img.size() + 255) / 256 * 256;
+  img.resize(off + n);
+  memcpy(img.data() + off, p, n);
+}
+
+// Records one solve per instance, forms the families, builds every step's
+// descriptor table (and the grouped tables of the steps' remaining
+// instances) and uploads them (blocking: waits for the device first).
+ff_status prepare_steps(ff_batch* b, double* const* d, double* const* p) {
+  b->steps_ok = false;
+  b->fams.clear();
+  std::vector<std::vector<PassLaunch>> rec(b->count);
+  for (int i = 0; i < b->count; ++i) {
+    set_launch_recorder(&rec[i]);
+    const ff_status st = solve_enqueue(b->plans[i], d[i], p[i], nullptr, nullptr);
+    set_launch_recorder(nullptr);
+    if (st != FF_OK) return st;
+  }
+  std::vector<StepFamily> fams;
+  for (int i = 0; i < b->count; ++i) {
+    StepFamily* home = nullptr;
+    for (StepFamily& f : fams) {
+      const int l = f.inst[0];
+      if (rec[l].size() != rec[i].size()) continue;
+      bool same = true;
+      for (size_t k = 0; k < rec[i].size() && same; ++k) same = launch_role(rec[l][k]) == launch_role(rec[i][k]);
+      if (same) {
+        home = &f;
+        break;
+      }
+    }
+    if (!home) {
+      fams.emplace_back();
+      home = &fams.back();
+      home->steps = (int)rec[i].size();
+    }
+    home->inst.push_back(i);
+  }
+  // Device image: per step, the descriptors and first workgroups of the
+  // multi-shape launch, then the per-instance tables of its grouped launches.
+  // Pointers into the image are stored as offsets and rebased after upload.
+  std::vector<unsigned char> img;
+  struct Fix {
+    size_t fam, op;
+    size_t desc, block0, inst;
+  };
+  std::vector<Fix> fixes;
+  for (size_t fi = 0; fi < fams.size(); ++fi) {
+    StepFamily& f = fams[fi];
+    for (int s = 0; s < f.steps; ++s) {
+      const PassLaunch& lead = rec[f.inst[0]][s];
+      struct Member {
+        int variant, inst;
+        size_t lds;
+        MultiColDesc c;
+        MultiRowDesc r;
+      };
+      std::vector<Member> multi;
+      std::vector<int> rest;
+      for (int i : f.inst) {
+        Member m;
+        m.inst = i;
+        bool ok;
+        if (lead.kind == 0) {
+          ok = col_desc(rec[i][s], m.c, m.lds);
+          m.variant = m.c.variant;
+        } else {
+          ok = row_desc(rec[i][s], m.r, m.lds);
+          m.variant = m.r.variant;
+        }
+        if (ok) multi.push_back(m);
+        else rest.push_back(i);
+      }
+      if (!multi.empty()) {
+        // one variant's tiles after another: neighbouring workgroups run the same code
+        std::stable_sort(multi.begin(), multi.end(),
+                         [](const Member& x, const Member& y) { return x.variant < y.variant; });
+        StepLaunch sl;
+        sl.kind = lead.kind == 0 ? MULTI_COL : (lead.kind == 1 ? MULTI_R2C : MULTI_C2R);
+        sl.f32 = lead.f32;
+        sl.sel = lead.sel;
+        sl.epi = lead.kind == 2 ? lead.epi : 0;
+        std::vector<int> block0;
+        long long nb = 0;
+        std::vector<unsigned char> descs;
+        for (const Member& m : multi) {
+          block0.push_back((int)nb);
+          sl.lds = std::max(sl.lds, m.lds);
+          const unsigned char* src = lead.kind == 0 ? (const unsigned char*)&m.c : (const unsigned char*)&m.r;
+          const size_t sz = lead.kind == 0 ? sizeof(MultiColDesc) : sizeof(MultiRowDesc);
+          descs.insert(descs.end(), src, src + sz);
+          nb += lead.kind == 0 ? m.c.a.nblocks : m.r.a.nblocks;
+        }
+        if (nb > INT_MAX) return set_error(FF_ERR_UNSUPPORTED, "batch step too large for one launch");
+        sl.m.ndesc = (int)multi.size();
+        sl.m.nblocks = (int)nb;
+        Fix fx{fi, f.ops.size(), 0, 0, SIZE_MAX};
+        push_bytes(img, descs.data(), descs.size(), fx.desc);
+        push_bytes(img, block0.data(), block0.size() * sizeof(int), fx.block0);
+        fixes.push_back(fx);
+        f.ops.push_back(sl);
+        f.multi_launches++;
+      }
+      // the rest: one grouped launch per set of identical launches
+      std::vector<std::vector<int>> clusters;
+      for (int i : rest) {
+        std::vector<int>* home = nullptr;
+        for (auto& c : clusters)
+          if (same_launch(rec[c[0]][s], rec[i][s])) {
+            home = &c;
+            break;
+          }
+        if (!home) {
+          clusters.emplace_back();
+          home = &clusters.back();
+        }
+        home->push_back(i);
+      }
+      for (const auto& c : clusters) {
+        std::vector<PassInst> tab;
+        for (int i : c) tab.push_back(inst_fields(rec[i][s]));
+        StepLaunch sl;
+        sl.kind = -1;
+        sl.op = rec[c[0]][s];
+        sl.op.l.grid.y = (unsigned)c.size();
+        Fix fx{fi, f.ops.size(), SIZE_MAX, SIZE_MAX, 0};
+        push_bytes(img, tab.data(), tab.size() * sizeof(PassInst), fx.inst);
+        fixes.push_back(fx);
+        f.ops.push_back(sl);
+      }
+    }
+  }
+  // kernels of earlier solves may still read the old tables
+  ff_status st = hip_check(hipDeviceSynchronize(), "batch table sync");
+  const size_t bytes = std::max<size_t>(256, img.size());
+  if (st == FF_OK && bytes > b->step_cap) {
+    if (b->step_dev) (void)hipFree(b->step_dev);
+    b->step_dev = nullptr;
+    b->step_cap = 0;
+    st = hip_check(hipMalloc(&b->step_dev, bytes), "batch step table hipMalloc");
+    if (st == FF_OK) b->step_cap = bytes;
+  }
+  if (st == FF_OK && !img.empty())
+    st = hip_check(hipMemcpy(b->step_dev, img.data(), img.size(), hipMemcpyHostToDevice), "batch step table upload");
+  if (st != FF_OK) return st;
+  unsigned char* base = static_cast<unsigned char*>(b->step_dev);
+  for (const Fix& fx : fixes) {
+    StepLaunch& sl = fams[fx.fam].ops[fx.op];
+    if (sl.kind >= 0) {
+      sl.m.desc = base + fx.desc;
+      sl.m.block0 = reinterpret_cast<const int*>(base + fx.block0);
+    } else {
+      const PassInst* t = reinterpret_cast<const PassInst*>(base + fx.inst);
+      if (sl.op.kind == 0) sl.op.col.inst = t;
+      else sl.op.row.inst = t;
+    }
+  }
+  b->fams.swap(fams);
+  b->sd.assign(d, d + b->count);
+  b->sp.assign(p, p + b->count);
+  b->steps_ok = true;
+  return FF_OK;
+}
+
+bool steps_current(const ff_batch* b, double* const* d, double* const* p) {
+  return b->steps_ok && std::equal(b->sd.begin(), b->sd.end(), d) && std::equal(b->sp.begin(), b->sp.end(), p);
+}
+
 // Fork onto the batch's streams, enqueue every instance (or every group),
 // join back onto s.
 ff_status enqueue_all(ff_batch* b, double* const* d, double* const* p, hipStream_t s) {
   const bool grouped = (b->flags & FF_BATCH_GROUPED) != 0;
+  const bool steps = (b->flags & FF_BATCH_STEPS) != 0;
   if (grouped && !groups_current(b, d, p)) {
     ff_status st = prepare_groups(b, d, p);
+    if (st != FF_OK) return st;
+  }
+  if (steps && !steps_current(b, d, p)) {
+    ff_status st = prepare_steps(b, d, p);
     if (st != FF_OK) return st;
   }
   ff_status st = hip_check(hipEventRecord(b->fork, s), "batch fork");
   for (size_t k = 0; k < b->streams.size() && st == FF_OK; ++k)
     st = hip_check(hipStreamWaitEvent(b->streams[k], b->fork, 0), "batch fork wait");
-  if (grouped) {
+  if (steps) {
+    // one stream per family: its steps in order, families side by side
+    for (size_t fi = 0; fi < b->fams.size() && st == FF_OK; ++fi) {
+      hipStream_t hs = b->streams[fi % b->streams.size()];
+      for (const StepLaunch& op : b->fams[fi].ops) {
+        if (op.kind >= 0) {
+          st = hip_check(launch_multi(op.kind, op.f32, op.sel, op.epi, op.m, op.lds, hs), "multi-shape pass launch");
+        } else {
+          PassLaunch r = op.op;
+          r.l.stream = hs;
+          st = hip_check(replay_launch(r), "grouped pass launch");
+        }
+        if (st != FF_OK) break;
+      }
+    }
+  } else if (grouped) {
     for (size_t gi = 0; gi < b->groups.size() && st == FF_OK; ++gi) {
       hipStream_t hs = b->streams[gi % b->streams.size()];
       for (const PassLaunch& op : b->groups[gi].ops) {
@@ -596,7 +891,7 @@ ff_status enqueue_all(ff_batch* b, double* const* d, double* const* p, hipStream
       }
     }
   }
-  for (int i = 0; i < b->count && st == FF_OK && !grouped; ++i)
+  for (int i = 0; i < b->count && st == FF_OK && !grouped && !steps; ++i)
     st = solve_enqueue(b->plans[i], d[i], p[i], b->streams[i % b->streams.size()], nullptr);
   for (size_t k = 0; k < b->streams.size() && st == FF_OK; ++k) {
     st = hip_check(hipEventRecord(b->joins[k], b->streams[k]), "batch join");
@@ -618,7 +913,10 @@ ff_status ff_batch_create(int count, const ff_batch_item* items, int flags, ff_b
     return set_error(FF_ERR_UNSUPPORTED,
                      "ff_batch_create: the single persistent launch is disabled in this build (its kernel did not "
                      "complete on MI355X); use the default stream-parallel batch");
-  if (flags & ~(FF_BATCH_GRAPH | FF_BATCH_GROUPED)) return set_error(FF_ERR_INVALID, "ff_batch_create: bad flags");
+  if (flags & ~(FF_BATCH_GRAPH | FF_BATCH_GROUPED | FF_BATCH_STEPS))
+    return set_error(FF_ERR_INVALID, "ff_batch_create: bad flags");
+  if ((flags & FF_BATCH_GROUPED) && (flags & FF_BATCH_STEPS))
+    return set_error(FF_ERR_INVALID, "ff_batch_create: FF_BATCH_GROUPED and FF_BATCH_STEPS are exclusive");
   for (int i = 0; i < count; ++i)
     for (int a = 0; a < 3; ++a)
       if (items[i].n[a] < 2) return set_error(FF_ERR_INVALID, "ff_batch_create: extents must be >= 2");
@@ -664,9 +962,13 @@ ff_status ff_batch_solve(ff_batch* b, double* const* density, double* const* pot
                     std::equal(b->bp.begin(), b->bp.end(), potential);
   if (!same || !b->exec || b->graph_stream != s) {
     drop_graph(b);
-    // grouped tables are recorded and uploaded before capture (blocking calls)
+    // grouped and step tables are recorded and uploaded before capture (blocking calls)
     if ((b->flags & FF_BATCH_GROUPED) && !groups_current(b, density, potential)) {
       ff_status st = prepare_groups(b, density, potential);
+      if (st != FF_OK) return st;
+    }
+    if ((b->flags & FF_BATCH_STEPS) && !steps_current(b, density, potential)) {
+      ff_status st = prepare_steps(b, density, potential);
       if (st != FF_OK) return st;
     }
     ff_status st = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
@@ -816,6 +1118,12 @@ ff_status ff_batch_describe(const ff_batch* b, char* buf, size_t len) {
     os << (g ? "," : "") << "{\"instances\":" << b->groups[g].inst.size() << ",\"launches\":" << b->groups[g].ops.size()
        << "}";
   }
+  os << "],\"steps\":[";
+  for (size_t f = 0; f < b->fams.size(); ++f) {
+    const StepFamily& fm = b->fams[f];
+    os << (f ? "," : "") << "{\"instances\":" << fm.inst.size() << ",\"steps\":" << fm.steps
+       << ",\"launches\":" << fm.ops.size() << ",\"multi_launches\":" << fm.multi_launches << "}";
+  }
   os << "],\"plans\":[";
   char tmp[4096];
   for (int i = 0; i < b->count; ++i) {
@@ -835,6 +1143,7 @@ ff_status ff_batch_destroy(ff_batch* b) {
   for (hipStream_t s : b->streams) (void)hipStreamSynchronize(s);
   destroy_plans(b);
   if (b->inst_dev) (void)hipFree(b->inst_dev);
+  if (b->step_dev) (void)hipFree(b->step_dev);
   for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
   for (hipEvent_t e : b->joins) (void)hipEventDestroy(e);
   if (b->fork) (void)hipEventDestroy(b->fork);

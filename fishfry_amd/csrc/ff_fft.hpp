@@ -48,11 +48,20 @@ __device__ __forceinline__ cx<F> scale(cx<F> a, F s) { return {a.x * s, a.y * s}
 template <typename F>
 __device__ __forceinline__ cx<F> conj(cx<F> a) { return {a.x, -a.y}; }
 
+// Fused multiply-add with one rounding.  The pass kernels are compiled with
+// floating-point contraction off, and every multiply-add of the engine is
+// written out with fmadd: each kernel variant (runtime or compiled-in plan,
+// any tile width, one solve or many instances per launch) then performs the
+// same rounded operations in the same order, so their results are
+// bit-identical.
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 // a * w for the forward direction (DIR < 0), a * conj(w) for the inverse.
 template <int DIR, typename F>
 __device__ __forceinline__ cx<F> twmul(cx<F> a, cx<F> w) {
-  if (DIR < 0) return {a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x};
-  return {a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y};
+  if (DIR < 0) return {fmadd(a.x, w.x, -(a.y * w.y)), fmadd(a.x, w.y, a.y * w.x)};
+  return {fmadd(a.x, w.x, a.y * w.y), fmadd(a.y, w.x, -(a.x * w.y))};
 }
 
 // a * (-i) forward, a * (+i) inverse.
@@ -80,12 +89,12 @@ __device__ __forceinline__ cx<F> rot(cx<F> a) {
     constexpr F cxv = (o == 1 || o == 7) ? h : -h;
     constexpr F sxv = (o == 1 || o == 3) ? h : -h;
     constexpr F s = DIR < 0 ? -sxv : sxv;
-    return {cxv * a.x - s * a.y, cxv * a.y + s * a.x};
+    return {fmadd(cxv, a.x, -(s * a.y)), fmadd(cxv, a.y, s * a.x)};
   } else {
     constexpr F c = F(kcos(R, t));
     constexpr F s0 = F(ksin(R, t));
     constexpr F s = DIR < 0 ? -s0 : s0;
-    return {c * a.x - s * a.y, c * a.y + s * a.x};
+    return {fmadd(c, a.x, -(s * a.y)), fmadd(c, a.y, s * a.x)};
   }
 }
 
@@ -114,7 +123,7 @@ struct Dft<3, DIR, F> {
   static __device__ __forceinline__ void run(cx<F>* a) {
     constexpr F h = F(0.8660254037844386467637232);
     const cx<F> t1 = a[1] + a[2];
-    const cx<F> t2 = {a[0].x - F(0.5) * t1.x, a[0].y - F(0.5) * t1.y};
+    const cx<F> t2 = {fmadd(F(-0.5), t1.x, a[0].x), fmadd(F(-0.5), t1.y, a[0].y)};
     const cx<F> t3 = mul_mi<DIR>(scale(a[1] - a[2], h));
     a[0] = a[0] + t1;
     a[1] = t2 + t3;
@@ -144,10 +153,10 @@ struct Dft<5, DIR, F> {
     const cx<F> t1 = a[1] + a[4], t2 = a[2] + a[3];
     const cx<F> t3 = a[1] - a[4], t4 = a[2] - a[3];
     const cx<F> a0 = a[0];
-    const cx<F> u1 = {a0.x + c1 * t1.x - c2 * t2.x, a0.y + c1 * t1.y - c2 * t2.y};
-    const cx<F> u2 = {a0.x - c2 * t1.x + c1 * t2.x, a0.y - c2 * t1.y + c1 * t2.y};
-    const cx<F> v1 = mul_mi<DIR>(cx<F>{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y});
-    const cx<F> v2 = mul_mi<DIR>(cx<F>{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y});
+    const cx<F> u1 = {fmadd(-c2, t2.x, fmadd(c1, t1.x, a0.x)), fmadd(-c2, t2.y, fmadd(c1, t1.y, a0.y))};
+    const cx<F> u2 = {fmadd(c1, t2.x, fmadd(-c2, t1.x, a0.x)), fmadd(c1, t2.y, fmadd(-c2, t1.y, a0.y))};
+    const cx<F> v1 = mul_mi<DIR>(cx<F>{fmadd(s1, t3.x, s2 * t4.x), fmadd(s1, t3.y, s2 * t4.y)});
+    const cx<F> v2 = mul_mi<DIR>(cx<F>{fmadd(s2, t3.x, -(s1 * t4.x)), fmadd(s2, t3.y, -(s1 * t4.y))});
     a[0] = a0 + t1 + t2;
     a[1] = u1 + v1;
     a[4] = u1 - v1;

@@ -212,16 +212,17 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
     // spectral filter of ParisPencil.cpp:269-284 at (i, j, kh) = (r, outer, col)
     const int jj = outer, kk = col;
     const double j2 = sqr(double(jj < a.ny - jj ? jj : a.ny - jj) * io.ddj);
-    const double k2 = sqr(double(kk) * io.ddk);
+    const double dk = double(kk) * io.ddk;
     // the (0,0,0) term before the multiply is the sum of the input field
     if (io.dc_out && valid && r0 == 0 && jj == 0 && kk == 0) *glb<double>(io.dc_out) = double(v[0].x);
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int ii = r0 + i * L;
-      const double i2 = sqr(double(ii < a.nx - ii ? ii : a.nx - ii) * io.ddi);
+      const double di = double(ii < a.nx - ii ? ii : a.nx - ii) * io.ddi;
+      const double s2 = fmadd(dk, dk, fmadd(di, di, j2));  // (i^2 + j^2) + k^2, as the reference sums
       double d;
-      if constexpr (MODE == COL_FILTER) d = (ii | jj | kk) ? -1.0 / (i2 + j2 + k2) : 0.0;
-      else d = -(i2 + j2 + k2);
+      if constexpr (MODE == COL_FILTER) d = (ii | jj | kk) ? -1.0 / s2 : 0.0;
+      else d = -s2;
       const F df = F(d);
       v[i] = cx<F>{df * v[i].x, df * v[i].y};
     }
@@ -252,10 +253,11 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
 // (LDS pad shift 4), for the lengths that dominate the solve; the workgroup is
 // then exactly (SN/E)*ST threads and SW the waves per SIMD the register
 // allocation must allow (the LDS-limited occupancy of the tile).
-template <typename F, int E, int MODE, bool SPLIT, int G, bool BL, int SN = 0, int ST = 0, int SW = 1>
-__global__ __attribute__((amdgpu_flat_work_group_size(1, SN > 0 ? SN / E * ST : max_threads_for(E)),
-                          amdgpu_waves_per_eu(SN > 0 ? SW : 1))) void k_col(const ColArgs a) {
-  const PassInst io = inst_of(a);
+// col_tile: the work of one workgroup, tile b of the pass (b < a.nblocks)
+// with the per-instance fields io; k_col runs one pass, the multi-shape
+// kernels (ff_multi.hip) run the tiles of many instances' passes.
+template <typename F, int E, int MODE, bool SPLIT, int G, bool BL, int SN, int ST>
+__device__ __forceinline__ void col_tile(const ColArgs& a, const PassInst& io, int b) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
   static_assert(SN == 0 || (ST >= 4 && (ST & (ST - 1)) == 0), "static column tile: 4, 8, ... columns");
@@ -264,7 +266,6 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, SN > 0 ? SN / E * ST : 
   const int c = tid & (T - 1);
   const int r0 = SN > 0 ? tid / ST : tid >> a.lgT;
   const int L = SN > 0 ? SN / E : a.lp.L;
-  const int b = xcd_remap(blockIdx.x, a.nblocks);
   int outer, tile;
   if (a.outer_fast) {
     tile = b / a.nouter;
@@ -320,6 +321,12 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, SN > 0 ? SN / E * ST : 
   }
 }
 
+template <typename F, int E, int MODE, bool SPLIT, int G, bool BL, int SN = 0, int ST = 0, int SW = 1>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, SN > 0 ? SN / E * ST : max_threads_for(E)),
+                          amdgpu_waves_per_eu(SN > 0 ? SW : 1))) void k_col(const ColArgs a) {
+  col_tile<F, E, MODE, SPLIT, G, BL, SN, ST>(a, inst_of(a), xcd_remap(blockIdx.x, a.nblocks));
+}
+
 struct RowArgs {
   const void* src;
   void* dst;
@@ -357,16 +364,17 @@ __device__ __forceinline__ long long spec_row(const RowArgs& a, long long row) {
 // Even nz: pack (x[2r], x[2r+1]) as one complex, an nz/2-point transform, then
 // split the even/odd spectra.  Odd nz: a full nz-point transform of (x, 0).
 // SN > 0: the plan of the row transform, length SN, is compiled in (even nz).
-template <typename F, typename IN, int E, bool ODD, bool BL, int SN = 0>
-__global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
-  const PassInst io = inst_of(a);
+// r2c_tile: the work of one workgroup, row block blk (< a.nblocks), with the
+// per-instance fields io (k_r2c: one pass; ff_multi.hip: many instances).
+template <typename F, typename IN, int E, bool ODD, bool BL, int SN>
+__device__ __forceinline__ void r2c_tile(const RowArgs& a, const PassInst& io, int blk) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
   const int tid = threadIdx.x;
   const int L = SN > 0 ? SN / E : a.lp.L;
   const int line = SN > 0 ? tid / L : (L == 1 ? tid : (int)__umulhi((uint32_t)tid, a.Linv));
   const int r0 = tid - line * L;
-  const long long row = (long long)xcd_remap(blockIdx.x, a.nblocks) * a.T + line;
+  const long long row = (long long)blk * a.T + line;
   const bool valid = row < a.nrows;
   const long long rowc = valid ? row : a.nrows - 1;  // loads stay in bounds; stores are predicated
   const int M = a.nline;                  // transform length (the engine's lp.n is longer for Bluestein)
@@ -433,6 +441,11 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
   }
 }
 
+template <typename F, typename IN, int E, bool ODD, bool BL, int SN = 0>
+__global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
+  r2c_tile<F, IN, E, ODD, BL, SN>(a, inst_of(a), xcd_remap(blockIdx.x, a.nblocks));
+}
+
 // Epilogues of the z inverse: store scale*x, store the residual
 // f - shift - scale*x, or accumulate out += scale*x - shift.
 enum { EPI_STORE = 0, EPI_RESIDUAL = 1, EPI_ACCUM = 2 };
@@ -443,9 +456,9 @@ __device__ __forceinline__ void c2r_out1(FF_GLB OUT* out, const FF_GLB double* f
   if constexpr (EPI == EPI_STORE) {
     out[off] = OUT(scale * val);
   } else if constexpr (EPI == EPI_RESIDUAL) {
-    out[off] = OUT((f[off] - shift) - scale * val);
+    out[off] = OUT(fmadd(-scale, val, f[off] - shift));
   } else {
-    out[off] = OUT(double(out[off]) + (scale * val - shift));
+    out[off] = OUT(double(out[off]) + fmadd(scale, val, -shift));
   }
 }
 
@@ -458,10 +471,10 @@ __device__ __forceinline__ void c2r_out2(FF_GLB OUT* out, const FF_GLB double* f
     *o = cx<OUT>{OUT(scale * v0), OUT(scale * v1)};
   } else if constexpr (EPI == EPI_RESIDUAL) {
     const cx<double> fp = ((const FF_GLB cx<double>*)f)[r];
-    *o = cx<OUT>{OUT((fp.x - shift) - scale * v0), OUT((fp.y - shift) - scale * v1)};
+    *o = cx<OUT>{OUT(fmadd(-scale, v0, fp.x - shift)), OUT(fmadd(-scale, v1, fp.y - shift))};
   } else {
     const cx<OUT> p = *o;
-    *o = cx<OUT>{OUT(double(p.x) + (scale * v0 - shift)), OUT(double(p.y) + (scale * v1 - shift))};
+    *o = cx<OUT>{OUT(double(p.x) + fmadd(scale, v0, -shift)), OUT(double(p.y) + fmadd(scale, v1, -shift))};
   }
 }
 
@@ -469,16 +482,17 @@ __device__ __forceinline__ void c2r_out2(FF_GLB OUT* out, const FF_GLB double* f
 // The imaginary parts of the DC and (even nz) Nyquist terms are ignored,
 // i.e. the rows are read as Hermitian half spectra.
 // SN > 0: the plan of the row transform, length SN, is compiled in (even nz).
-template <typename F, typename OUT, int E, bool ODD, int EPI, bool BL, int SN = 0>
-__global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
-  const PassInst io = inst_of(a);
+// c2r_tile: the work of one workgroup, row block blk (< a.nblocks), with the
+// per-instance fields io (k_c2r: one pass; ff_multi.hip: many instances).
+template <typename F, typename OUT, int E, bool ODD, int EPI, bool BL, int SN>
+__device__ __forceinline__ void c2r_tile(const RowArgs& a, const PassInst& io, int blk) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
   const int tid = threadIdx.x;
   const int L = SN > 0 ? SN / E : a.lp.L;
   const int line = SN > 0 ? tid / L : (L == 1 ? tid : (int)__umulhi((uint32_t)tid, a.Linv));
   const int r0 = tid - line * L;
-  const long long row = (long long)xcd_remap(blockIdx.x, a.nblocks) * a.T + line;
+  const long long row = (long long)blk * a.T + line;
   const bool valid = row < a.nrows;
   const int M = a.nline;                // transform length (the engine's lp.n is longer for Bluestein)
   constexpr bool blue = BL;             // positions >= M exist only for Bluestein
@@ -500,7 +514,7 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
     // kh block (T*KT elements), so all threads copy element e of the tile in
     // (block, row, kt) order
     const FF_GLB cx<F>* __restrict__ src = glb<const cx<F>>(io.src);
-    const long long row0 = (long long)xcd_remap(blockIdx.x, a.nblocks) * a.T;
+    const long long row0 = (long long)blk * a.T;
     const int KT = 1 << a.spec.lgKT;
     const int TK = a.T * KT;
     const int total = a.T * a.nh;
@@ -572,6 +586,11 @@ __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
       }
     }
   }
+}
+
+template <typename F, typename OUT, int E, bool ODD, int EPI, bool BL, int SN = 0>
+__global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
+  c2r_tile<F, OUT, E, ODD, EPI, BL, SN>(a, inst_of(a), xcd_remap(blockIdx.x, a.nblocks));
 }
 
 // ---------------------------------------------------------------------------

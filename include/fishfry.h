@@ -190,6 +190,18 @@ typedef struct ff_batch_item {
                                  per-instance buffers and domains from a device table);
                                  the table is rebuilt, after a device synchronization,
                                  when the buffers change                                 */
+#define FF_BATCH_STEPS 16     /* pass step s of every instance of one precision and
+                                 refinement count is ONE launch of a multi-shape kernel:
+                                 a device table holds each instance's pass arguments and
+                                 workgroup range, and each workgroup runs the tile of the
+                                 instance it falls in (no workgroup waits on another).
+                                 Lines without a compiled-in multi-shape variant (lengths
+                                 other than 32, 48, 64, 96, 128, 160, 256 and, for the z
+                                 passes, nz other than 32 ... 256; Bluestein and odd nz)
+                                 run that step as extra grouped launches on the same
+                                 stream.  Exclusive with FF_BATCH_GROUPED; the table is
+                                 rebuilt, after a device synchronization, when the
+                                 buffers change                                          */
 
 typedef struct ff_batch ff_batch;
 

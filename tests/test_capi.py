@@ -109,7 +109,10 @@ def test_persistent_batch_is_rejected_without_gpu_work():
 
 def test_batch_flags_are_validated_without_gpu_work():
     with pytest.raises(ff.FishFryError) as e:
-        ff.Batch([{"n": (8, 8, 8)}], 16)  # no such flag
+        ff.Batch([{"n": (8, 8, 8)}], 32)  # no such flag
+    assert e.value.status == ff.FF_ERR_INVALID
+    with pytest.raises(ff.FishFryError) as e:
+        ff.Batch([{"n": (8, 8, 8)}], ff.FF_BATCH_GROUPED | ff.FF_BATCH_STEPS)  # exclusive modes
     assert e.value.status == ff.FF_ERR_INVALID
     with pytest.raises(ff.FishFryError) as e:
         ff.Batch([{"n": (8, 8, 1)}], ff.FF_BATCH_GROUPED)  # extents below 2 are refused first

@@ -214,12 +214,13 @@ def test_layouts_agree_bitwise(n):
 def test_column_variants_agree(n, E, variant):
     """Column-pass variants -- the split real/imaginary LDS exchange, two
     column groups per workgroup, and the generic kernel in place of the one
-    with the line plan compiled in (length 1024) -- move the same
-    values as the default kernel.  They are separate kernels, so the compiler
-    may contract multiply-adds differently: results agree to round-off, and
-    all match the oracle.  Two groups apply where E is 8 or 16.  Compiled-in
-    plans exist for 1024 (E 16), 32, 48, 64, 96, 128, 160 and 256, and for the x passes of
-    512, 768 and 1000."""
+    with the line plan compiled in (length 1024) -- move the same values
+    as the default kernel.  The pass kernels are compiled without
+    floating-point contraction and write their multiply-adds out, so every
+    variant performs the same rounded operations: results are bit-identical,
+    and all match the oracle.  Two groups apply where E is 8 or 16.
+    Compiled-in plans exist for 1024 (E 16), 32, 48, 64, 96, 128, 160 and 256,
+    and for the x passes of 512, 768 and 1000."""
     rng = np.random.default_rng(5)
     f = rng.standard_normal(n)
     try:
@@ -245,9 +246,7 @@ def test_column_variants_agree(n, E, variant):
                 outs.append(gpu_solve(f, precision=prec, refine_steps=steps))
             d = rel(outs[1], outs[0])
             say("  variant vs default kernel: rel L2", d)
-            # FP64 agrees to round-off; FP32 passes + 1 refinement carry each
-            # kernel's own FP32 rounding squared, ~1e-10 at 1024-long lines
-            assert d < (1e-14 if prec == ff.FF_FP64 else 1e-9)
+            assert np.array_equal(outs[1], outs[0])
             want = oracle.solve(f)
             tol = TOL if prec == ff.FF_FP64 else 1e-9  # FP32 + 1 step: ~1e-10 at 768-long lines
             assert rel(outs[0], want) < tol and rel(outs[1], want) < tol
