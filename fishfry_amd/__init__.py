@@ -150,6 +150,8 @@ def lib():
     L.ff_solve.argtypes = [P, ctypes.c_size_t, P, P, P, ctypes.POINTER(Stamp), ctypes.POINTER(ctypes.c_int)]
     L.ff_solve_async.restype = ctypes.c_int
     L.ff_solve_async.argtypes = [P, ctypes.c_size_t, P, P, P]
+    L.ff_solve_noalloc.restype = ctypes.c_int
+    L.ff_solve_noalloc.argtypes = [P, ctypes.c_size_t, P, P, P]
     L.ff_plan_describe.restype = ctypes.c_int
     L.ff_plan_describe.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
     L.ff_solve_pass.restype = ctypes.c_int
@@ -291,6 +293,12 @@ class ParisPencil:
 
     def solve_async(self, nbytes: int, density, potential, stream=None) -> None:
         check(lib().ff_solve_async(self._h, nbytes, _ptr(density), _ptr(potential), _stream(stream)), "solve_async")
+
+    def solve_noalloc(self, nbytes: int, density, potential, stream=None) -> None:
+        """solve_async with the work area in the library's static device array
+        (no allocation; FF_ERR_INVALID when the plan's work area is too large)."""
+        check(lib().ff_solve_noalloc(self._h, nbytes, _ptr(density), _ptr(potential), _stream(stream)),
+              "solve_noalloc")
 
     def solve_pass(self, p: int, src, dst, stream=None) -> None:
         check(lib().ff_solve_pass(self._h, p, _ptr(src), _ptr(dst), _stream(stream)), f"pass {p}")

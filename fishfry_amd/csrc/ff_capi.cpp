@@ -68,6 +68,35 @@ ff_status ff_solve_async(const ff_plan* plan, size_t bytes, double* density, dou
   return solve_enqueue(plan, density, potential, (hipStream_t)stream, nullptr);
 }
 
+ff_status ff_solve_noalloc(const ff_plan* plan, size_t bytes, double* density, double* potential, void* stream) {
+  clear_error();
+  ff_status st = check_solve(plan, bytes, density, potential);
+  if (st != FF_OK) return st;
+  hipStream_t s = (hipStream_t)stream;
+  size_t need = 0;
+  if (plan->opt.precision == FF_FP32_REFINE) need = plan_scratch_bytes(plan);
+  else if (plan->work) need = (size_t)plan->Nh * 16;
+  if (need > kStaticWorkBytes) {
+    char buf[200];
+    snprintf(buf, sizeof buf,
+             "ff_solve_noalloc: this plan needs a %zu-byte work area; the static work area holds %zu bytes", need,
+             kStaticWorkBytes);
+    return set_error(FF_ERR_INVALID, buf);
+  }
+  if (need == 0) return solve_enqueue(plan, density, potential, s, nullptr);  // no work area used
+  int cur = -1;
+  if ((st = hip_check(hipGetDevice(&cur), "hipGetDevice"))) return st;
+  if (cur != plan->device)
+    return set_error(FF_ERR_INVALID, "ff_solve_noalloc: the plan's device must be the current device");
+  void* w = nullptr;
+  if ((st = static_work_area(plan->device, &w))) return st;
+  // a copy of the plan (plain data, on the stack) pointed at the static area
+  ff_plan local = *plan;
+  if (local.opt.precision == FF_FP32_REFINE) plan_bind_scratch(&local, w);
+  else local.work = w;
+  return solve_enqueue(&local, density, potential, s, nullptr);
+}
+
 ff_status ff_solve(const ff_plan* plan, size_t bytes, double* density, double* potential, void* stream,
                    ff_stamp* stamps, int* n_stamps) {
   clear_error();

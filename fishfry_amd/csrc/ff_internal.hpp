@@ -25,6 +25,12 @@ void clear_error();
 ff_status device_scratch(size_t ndoubles, double** out);
 std::mutex& scratch_mutex();
 
+// The static device work area of ff_solve_noalloc (ff_static.hip): its size,
+// and its address on a device (looked up once per device, which must be the
+// current one on that first call).
+constexpr size_t kStaticWorkBytes = size_t(64) << 20;
+ff_status static_work_area(int device, void** out);
+
 // ---- line transforms --------------------------------------------------------
 // Supported elements-per-thread values (one kernel instantiation unit each).
 constexpr int kEValues[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 15, 16, 20, 24, 30};

@@ -106,6 +106,21 @@ ff_status ff_solve(const ff_plan* plan, size_t bytes, double* density, double* p
  * stamps): for callers that overlap solves or capture them in a HIP graph. */
 ff_status ff_solve_async(const ff_plan* plan, size_t bytes, double* density, double* potential, void* stream);
 
+/* No counterpart in the reference.  ff_solve_async for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the work area the solve's kernels need
+ * beyond density and potential -- FP32 + refinement plans: spectrum, residual
+ * and two sums (ff_plan_bytes() + 4*nx*ny*nz + 512 bytes or so); FP64 plans
+ * with FF_PRESERVE_INPUT: a third ff_plan_bytes() array -- is the library's
+ * static device array of 64 MiB (one per GPU, held until the process ends)
+ * instead of memory the plan allocated.  The call allocates nothing and
+ * calls no allocating routine; plans whose work area exceeds 64 MiB are
+ * rejected with FF_ERR_INVALID.  The plan's device must be the current
+ * device.  Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU (other streams, other threads) race.  FP64 plans without
+ * FF_PRESERVE_INPUT need no work area and behave as ff_solve_async. */
+ff_status ff_solve_noalloc(const ff_plan* plan, size_t bytes, double* density, double* potential, void* stream);
+
 /* ParisPencil::~ParisPencil -- ParisPencil.cpp:84-93.  NULL is a no-op. */
 ff_status ff_plan_destroy(ff_plan* plan);
 
