@@ -17,6 +17,9 @@ Prints one JSON line on stdout (progress goes to stderr):
   fp32_refine  FP32 passes + FP64 residual, 0..3 refinement steps on the
                analytic f plus 16 random modes: points/s and Linf beside the
                FP64 solve of the same f
+  fd_symbols   the same Pareto with the 3-point and 5-point finite-difference
+               symbols (FP64 stencil residual), beside FP64 solves of the
+               same symbol
   cpu_baseline the CPU oracle's FP64 solve on the host cores (bounded sample)
   extras       the other configurations, measured beside the headline: FP64
                solves at 512^3, 768^3 and 1000^3; the 64-grid batching
@@ -241,7 +244,11 @@ def bench_batching(ff, torch, K):
         glaunches = sum(x["launches"] for x in gdesc["groups"])
         g.close()
         # the same batch step by step: one multi-shape launch per pass step
-        # over all 64 instances (descriptor table), captured in a graph
+        # over all 64 instances (descriptor table), captured in a graph.  The
+        # autotuner's choices target the single-solve kernels (some move a
+        # pass off the multi-shape variants), so this batch plans with the
+        # default configurations.
+        ff.tune_reset()
         sb = ff.Batch(items, ff.FF_BATCH_GRAPH | ff.FF_BATCH_STEPS)
         for k in range(64):
             ff.field_init(fields[k], d[k])
@@ -335,6 +342,57 @@ def bench_serving(ff, nreq=96, seed=20261202):
     log(f"serving: {nreq} requests at {rate:.1f}/s (75% of {cap:.1f}/s): p50 {res['p50_ms']:.2f} ms, "
         f"p99 {res['p99_ms']:.2f} ms, {st['points_per_s']:.4e} points/s, worst Linf {worst}")
     return res
+
+
+def timed_solves(torch, plan, nb, f, phi, K):
+    """K back-to-back solves after one warmup: (points/s, sorted latencies ms)."""
+    stream = torch.cuda.current_stream()
+    plan.solve_async(nb, f, phi)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    ta = time.perf_counter()
+    for k in range(K):
+        ev[k][0].record(stream)
+        plan.solve_async(nb, f, phi)
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    N = plan.n[0] * plan.n[1] * plan.n[2]
+    return N * K / (tb - ta), sorted(a.elapsed_time(b) for a, b in ev)
+
+
+def precision_pareto(ff, torch, n, K, rfield, flags, value64):
+    """FP32 passes with 0..3 FP64-residual refinement steps beside the FP64
+    solve of the same f and symbol (flags: 0 spectral, or FF_SYMBOL_3PT /
+    FF_SYMBOL_5PT): points/s and Linf against the exact potential.  value64:
+    the FP64 rate to compare with (None: measured here)."""
+    lo, hi, _ = ff.benchmark_grid(n)
+    p64 = ff.ParisPencil(n, lo, hi, flags=flags | ff.FF_PRESERVE_INPUT)
+    nb = p64.bytes()
+    f = torch.empty(nb // 8, dtype=torch.float64, device="cuda")
+    phi = torch.empty(nb // 8, dtype=torch.float64, device="cuda")
+    ff.field_init(rfield, f)
+    v64, _ = timed_solves(torch, p64, nb, f, phi, K)
+    linf64 = ff.field_norms(rfield, phi)[2]
+    p64.close()
+    if value64 is None:
+        value64 = v64
+    tag = "spectral" if not flags else ("3-point" if flags == ff.FF_SYMBOL_3PT else "5-point")
+    log(f"{tag} symbol, random-mode f: FP64 {v64:.4e} points/s, Linf {linf64:.3e}")
+    levels = []
+    for steps in (0, 1, 2, 3):
+        p32 = ff.ParisPencil(n, lo, hi, precision=ff.FF_FP32_REFINE, refine_steps=steps, flags=flags)
+        pts, l32 = timed_solves(torch, p32, nb, f, phi, K)
+        r = ff.field_norms(rfield, phi)
+        levels.append({"refine_steps": steps, "points_per_s": pts, "p50_ms": round(pct(l32, .5), 4),
+                       "p99_ms": round(pct(l32, .99), 4), "l1": r[0], "l2": r[1], "linf": r[2],
+                       "speed_vs_fp64": round(pts / value64, 4), "linf_vs_fp64": round(r[2] / linf64, 4)})
+        log(f"  FP32+refine steps={steps}: {pts:.4e} points/s ({pts / value64:.3f}x FP64), Linf {r[2]:.3e} "
+            f"({r[2] / linf64:.3g}x FP64)")
+        p32.close()
+    del f, phi
+    torch.cuda.empty_cache()
+    return {"fp64_points_per_s": v64, "fp64_linf": linf64, "levels": levels}
 
 
 def main():
@@ -431,44 +489,19 @@ def main():
     # ---- precision: FP32 passes + FP64 residual refinement ----------------------
     # f = the analytic field plus 16 random Fourier modes (1 <= |k| <= 12,
     # +-k deduplicated, N(0,1) amplitudes, seeded); the FP64 solve of the same
-    # f gives the Linf each refinement level is compared with.
+    # f gives the Linf each refinement level is compared with.  The spectral
+    # symbol (the reference's filter) is compared with the headline FP64 rate;
+    # the finite-difference symbols with an FP64 solve of the same symbol.
     fp32 = []
+    fd = []
     if not args.no_fp32:
         rfield = ff.field_set_modes(ff.field_default(n), random_modes(20261202))
-        f = torch.empty(nb // 8, dtype=torch.float64, device="cuda")
-        phi = torch.empty(nb // 8, dtype=torch.float64, device="cuda")
-        p64 = ff.ParisPencil(n, lo, hi)
-        ff.field_init(rfield, f)
-        p64.solve(nb, f, phi)
-        linf64 = ff.field_norms(rfield, phi)[2]
-        p64.close()
-        log(f"random-mode f, FP64 Linf {linf64:.3e}")
-        ff.field_init(rfield, f)
-        for steps in (0, 1, 2, 3):
-            p32 = ff.ParisPencil(n, lo, hi, precision=ff.FF_FP32_REFINE, refine_steps=steps)
-            p32.solve_async(nb, f, phi)
-            torch.cuda.synchronize()
-            e32 = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
-            ta = time.perf_counter()
-            for k in range(K):
-                e32[k][0].record(stream)
-                p32.solve_async(nb, f, phi)
-                e32[k][1].record(stream)
-            torch.cuda.synchronize()
-            tb = time.perf_counter()
-            l32 = sorted(a.elapsed_time(b) for a, b in e32)
-            r = ff.field_norms(rfield, phi)
-            pts = N * K / (tb - ta)
-            fp32.append({"refine_steps": steps, "points_per_s": pts, "p50_ms": round(pct(l32, .5), 4),
-                         "p99_ms": round(pct(l32, .99), 4), "l1": r[0], "l2": r[1], "linf": r[2],
-                         "speed_vs_fp64": round(pts / value, 4), "linf_vs_fp64": round(r[2] / linf64, 4)})
-            log(f"FP32+refine steps={steps}: {pts:.4e} points/s ({pts / value:.3f}x FP64), Linf {r[2]:.3e} "
-                f"({r[2] / linf64:.3g}x FP64)")
-            p32.close()
-        fp32 = {"field": "analytic f + 16 random modes, 1 <= |k| <= 12, N(0,1) amplitudes, seed 20261202",
-                "fp64_linf": linf64, "levels": fp32}
-        del f, phi
-        torch.cuda.empty_cache()
+        fp32 = precision_pareto(ff, torch, n, K, rfield, 0, value)
+        fp32["field"] = "analytic f + 16 random modes, 1 <= |k| <= 12, N(0,1) amplitudes, seed 20261202"
+        for name, flag in (("3-point", ff.FF_SYMBOL_3PT), ("5-point", ff.FF_SYMBOL_5PT)):
+            r = precision_pareto(ff, torch, n, K, rfield, flag, None)
+            r["symbol"] = name
+            fd.append(r)
 
     extras = None
     if not args.no_extras:
@@ -507,6 +540,7 @@ def main():
         "roofline": roofline,
         "passes": passes,
         "fp32_refine": fp32,
+        "fd_symbols": fd,
         "cpu_baseline": cpu,
         "extras": extras,
     }

@@ -40,6 +40,8 @@ FF_FP32_REFINE = 1
 FF_PRESERVE_INPUT = 1
 FF_BLOCKED_LAYOUT = 2
 FF_SLAB_PIPELINE = 4
+FF_SYMBOL_3PT = 8
+FF_SYMBOL_5PT = 16
 FF_TUNE_SPLIT_LDS = 1
 FF_TUNE_TWO_GROUPS = 2
 FF_TUNE_GENERIC = 4
@@ -150,6 +152,8 @@ def lib():
     L.ff_solve.argtypes = [P, ctypes.c_size_t, P, P, P, ctypes.POINTER(Stamp), ctypes.POINTER(ctypes.c_int)]
     L.ff_solve_async.restype = ctypes.c_int
     L.ff_solve_async.argtypes = [P, ctypes.c_size_t, P, P, P]
+    L.ff_stencil_apply.restype = ctypes.c_int
+    L.ff_stencil_apply.argtypes = [P, P, P, P]
     L.ff_solve_noalloc.restype = ctypes.c_int
     L.ff_solve_noalloc.argtypes = [P, ctypes.c_size_t, P, P, P]
     L.ff_plan_describe.restype = ctypes.c_int
@@ -299,6 +303,11 @@ class ParisPencil:
         (no allocation; FF_ERR_INVALID when the plan's work area is too large)."""
         check(lib().ff_solve_noalloc(self._h, nbytes, _ptr(density), _ptr(potential), _stream(stream)),
               "solve_noalloc")
+
+    def stencil_apply(self, phi, out, stream=None) -> None:
+        """out = L_h phi with the plan's finite-difference stencil (FF_SYMBOL_3PT /
+        FF_SYMBOL_5PT plans), FP64, enqueued on stream."""
+        check(lib().ff_stencil_apply(self._h, _ptr(phi), _ptr(out), _stream(stream)), "stencil_apply")
 
     def solve_pass(self, p: int, src, dst, stream=None) -> None:
         check(lib().ff_solve_pass(self._h, p, _ptr(src), _ptr(dst), _stream(stream)), f"pass {p}")
@@ -516,5 +525,5 @@ from .driver import FishFry  # noqa: E402  (needs the names above)
 __all__ = [
     "ParisPencil", "FishFry", "Batch", "Server", "FF_BATCH_GRAPH", "FF_BATCH_PERSISTENT", "FF_BATCH_GROUPED", "FF_BATCH_STEPS", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
     "field_norms", "reference_bytes", "benchmark_grid", "FishFryError", "lib", "LIB_PATH",
-    "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_BLOCKED_LAYOUT", "FF_SLAB_PIPELINE", "FF_TUNE_SPLIT_LDS", "FF_TUNE_TWO_GROUPS", "FF_TUNE_GENERIC",
+    "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_BLOCKED_LAYOUT", "FF_SLAB_PIPELINE", "FF_SYMBOL_3PT", "FF_SYMBOL_5PT", "FF_TUNE_SPLIT_LDS", "FF_TUNE_TWO_GROUPS", "FF_TUNE_GENERIC",
 ]

@@ -735,6 +735,7 @@ ff_status prepare_steps(ff_batch* b, double* const* d, double* const* p) {
       struct Member {
         int variant, inst;
         size_t lds;
+        double weight;  // bytes one tile moves
         MultiColDesc c;
         MultiRowDesc r;
       };
@@ -744,12 +745,15 @@ ff_status prepare_steps(ff_batch* b, double* const* d, double* const* p) {
         Member m;
         m.inst = i;
         bool ok;
+        const double eb = rec[i][s].f32 ? 8.0 : 16.0;
         if (lead.kind == 0) {
           ok = col_desc(rec[i][s], m.c, m.lds);
           m.variant = m.c.variant;
+          if (ok) m.weight = 2.0 * eb * kMultiCols[m.variant].n * multi_tile(kMultiCols[m.variant]);
         } else {
           ok = row_desc(rec[i][s], m.r, m.lds);
           m.variant = m.r.variant;
+          if (ok) m.weight = 2.0 * eb * kMultiRows[m.variant].n * multi_tile(kMultiRows[m.variant]);
         }
         if (ok) multi.push_back(m);
         else rest.push_back(i);
@@ -774,9 +778,33 @@ ff_status prepare_steps(ff_batch* b, double* const* d, double* const* p) {
           descs.insert(descs.end(), src, src + sz);
           nb += lead.kind == 0 ? m.c.a.nblocks : m.r.a.nblocks;
         }
-        if (nb > INT_MAX) return set_error(FF_ERR_UNSUPPORTED, "batch step too large for one launch");
+        if (nb > INT_MAX / 2) return set_error(FF_ERR_UNSUPPORTED, "batch step too large for one launch");
         sl.m.ndesc = (int)multi.size();
-        sl.m.nblocks = (int)nb;
+        // cut the tiles into kXcds contiguous chunks of about equal work
+        double total = 0.0;
+        for (const Member& m : multi) total += m.weight * (lead.kind == 0 ? m.c.a.nblocks : m.r.a.nblocks);
+        int x = 0;
+        long long t = 0;
+        double acc = 0.0;
+        sl.m.chunk0[0] = 0;
+        for (const Member& m : multi) {
+          const long long cnt = lead.kind == 0 ? m.c.a.nblocks : m.r.a.nblocks;
+          for (long long k = 0; k < cnt; ++k, ++t) {
+            if (x + 1 < kXcds && acc >= total * (x + 1) / kXcds) {
+              sl.m.chunkn[x] = (int)(t - sl.m.chunk0[x]);
+              sl.m.chunk0[++x] = (int)t;
+            }
+            acc += m.weight;
+          }
+        }
+        sl.m.chunkn[x] = (int)(t - sl.m.chunk0[x]);
+        for (int y = x + 1; y < kXcds; ++y) {
+          sl.m.chunk0[y] = (int)t;
+          sl.m.chunkn[y] = 0;
+        }
+        int most = 0;
+        for (int y = 0; y < kXcds; ++y) most = std::max(most, sl.m.chunkn[y]);
+        sl.m.nblocks = kXcds * most;
         Fix fx{fi, f.ops.size(), 0, 0, SIZE_MAX};
         push_bytes(img, descs.data(), descs.size(), fx.desc);
         push_bytes(img, block0.data(), block0.size() * sizeof(int), fx.block0);

@@ -64,6 +64,10 @@ void push_bluestein(std::vector<unsigned char>& buf, int n, int m, int prec, siz
 const void* table_twiddle(int n, int prec);
 const void* table_post(int nz, int prec);
 bool table_bluestein(int n, int m, int prec, const void* out[3]);
+// Finite-difference factor table g[k] = 4 sin^2(pi k/n) (order 3) or
+// 4 sin^2(pi k/n) (1 + sin^2(pi k/n)/3) (order 5), k < n, FP64.
+const double* table_symbol(int n, int order);
+long double fd_factor_ld(long long k, long long n, int order);
 
 // exp(-2 pi i t / n) in long double with exact integer argument reduction.
 void twiddle_ld(long long t, long long n, long double& c, long double& s);

@@ -92,7 +92,10 @@ struct ColArgs {
   // boundary and no 32-byte sector is written in parts by two workgroups.
   // cshift_mask = 0: no shift.
   int cshift_mul, cshift_mask;
-  double ddi, ddj, ddk;
+  double ddi, ddj, ddk;  // filter constants (finite-difference symbol: 1/h^2 per axis)
+  // finite-difference symbol: per-axis factor tables g[k] (null: the
+  // spectral filter); the axis term is g[k] * dd
+  const double *symx, *symy, *symz;
   double* dc_out;  // FILTER / SYMBOL: receives the forward (0,0,0) term (sum of the input)
   const void* tw;
   LinePlan lp;
@@ -209,22 +212,41 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
     line_dft<E, +1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   } else {
     line_dft<E, -1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
-    // spectral filter of ParisPencil.cpp:269-284 at (i, j, kh) = (r, outer, col)
+    // spectral filter of ParisPencil.cpp:269-284 at (i, j, kh) = (r, outer, col),
+    // or the finite-difference symbol from the per-axis tables
     const int jj = outer, kk = col;
-    const double j2 = sqr(double(jj < a.ny - jj ? jj : a.ny - jj) * io.ddj);
-    const double dk = double(kk) * io.ddk;
     // the (0,0,0) term before the multiply is the sum of the input field
     if (io.dc_out && valid && r0 == 0 && jj == 0 && kk == 0) *glb<double>(io.dc_out) = double(v[0].x);
+    if (a.symx) {
+      // columns outside the plane and Bluestein positions past the line read
+      // entry 0 (their values are never stored)
+      const double tj = glb<const double>(a.symy)[jj] * io.ddj;
+      const double gk = glb<const double>(a.symz)[valid ? kk : 0];
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const int ii = r0 + i * L;
-      const double di = double(ii < a.nx - ii ? ii : a.nx - ii) * io.ddi;
-      const double s2 = fmadd(dk, dk, fmadd(di, di, j2));  // (i^2 + j^2) + k^2, as the reference sums
-      double d;
-      if constexpr (MODE == COL_FILTER) d = (ii | jj | kk) ? -1.0 / s2 : 0.0;
-      else d = -s2;
-      const F df = F(d);
-      v[i] = cx<F>{df * v[i].x, df * v[i].y};
+      for (int i = 0; i < E; ++i) {
+        const int ii = r0 + i * L;
+        const double gi = glb<const double>(a.symx)[ii < a.nx ? ii : 0];
+        const double s2 = fmadd(gk, io.ddk, fmadd(gi, io.ddi, tj));  // (i + j) + k
+        double d;
+        if constexpr (MODE == COL_FILTER) d = (ii | jj | kk) ? -1.0 / s2 : 0.0;
+        else d = -s2;
+        const F df = F(d);
+        v[i] = cx<F>{df * v[i].x, df * v[i].y};
+      }
+    } else {
+      const double j2 = sqr(double(jj < a.ny - jj ? jj : a.ny - jj) * io.ddj);
+      const double dk = double(kk) * io.ddk;
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int ii = r0 + i * L;
+        const double di = double(ii < a.nx - ii ? ii : a.nx - ii) * io.ddi;
+        const double s2 = fmadd(dk, dk, fmadd(di, di, j2));  // (i^2 + j^2) + k^2, as the reference sums
+        double d;
+        if constexpr (MODE == COL_FILTER) d = (ii | jj | kk) ? -1.0 / s2 : 0.0;
+        else d = -s2;
+        const F df = F(d);
+        v[i] = cx<F>{df * v[i].x, df * v[i].y};
+      }
     }
     if constexpr (SN > 0) {
       // With the plan compiled in, the inverse transform's twiddle loads and

@@ -15,7 +15,7 @@
 
 namespace ff {
 
-// Descriptor of workgroup b: the number of first workgroups <= b, minus one.
+// Descriptor of logical tile b: the number of first tiles <= b, minus one.
 // Lane l of the wave compares descriptor base + l; the count of a ballot is
 // wave-uniform, so the result lives in scalar registers.
 __device__ __forceinline__ int multi_find(const int* block0, int ndesc, int b) {
@@ -42,10 +42,17 @@ __device__ __forceinline__ void mcol_run(int v, const ColArgs& a, const PassInst
   }
 }
 
+// Logical tile of this workgroup, or -1 past its XCD chunk's end.
+__device__ __forceinline__ int multi_tile_of(const MultiArgs& m) {
+  const int x = blockIdx.x % kXcds, s = blockIdx.x / kXcds;
+  return s < m.chunkn[x] ? m.chunk0[x] + s : -1;
+}
+
 template <typename F, int MODE>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, kMultiThreads), amdgpu_waves_per_eu(2))) void k_mcol(
     const MultiArgs m) {
-  const int gb = xcd_remap(blockIdx.x, m.nblocks);
+  const int gb = multi_tile_of(m);
+  if (gb < 0) return;
   const int d = __builtin_amdgcn_readfirstlane(multi_find(m.block0, m.ndesc, gb));
   const FF_GLB MultiColDesc* D = glb<const MultiColDesc>(m.desc) + d;
   const ColArgs a = D->a;
@@ -79,10 +86,10 @@ __device__ __forceinline__ void mc2r_run(int v, const RowArgs& a, const PassInst
   }
 }
 
-// The row descriptor of workgroup blockIdx.x: (descriptor arguments, its
+// The row descriptor of logical tile gb: (descriptor arguments, its
 // per-instance fields, its row block).
-__device__ __forceinline__ void multi_row_desc(const MultiArgs& m, RowArgs& a, PassInst& io, int& v, int& b) {
-  const int gb = xcd_remap(blockIdx.x, m.nblocks);
+__device__ __forceinline__ void multi_row_desc(const MultiArgs& m, int gb, RowArgs& a, PassInst& io, int& v,
+                                               int& b) {
   const int d = __builtin_amdgcn_readfirstlane(multi_find(m.block0, m.ndesc, gb));
   const FF_GLB MultiRowDesc* D = glb<const MultiRowDesc>(m.desc) + d;
   a = D->a;
@@ -94,20 +101,24 @@ __device__ __forceinline__ void multi_row_desc(const MultiArgs& m, RowArgs& a, P
 template <typename F, typename IN>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, kMultiThreads), amdgpu_waves_per_eu(2))) void k_mr2c(
     const MultiArgs m) {
+  const int gb = multi_tile_of(m);
+  if (gb < 0) return;
   RowArgs a;
   PassInst io;
   int v, b;
-  multi_row_desc(m, a, io, v, b);
+  multi_row_desc(m, gb, a, io, v, b);
   mr2c_run<F, IN, 0>(v, a, io, b);
 }
 
 template <typename F, typename OUT, int EPI>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, kMultiThreads), amdgpu_waves_per_eu(2))) void k_mc2r(
     const MultiArgs m) {
+  const int gb = multi_tile_of(m);
+  if (gb < 0) return;
   RowArgs a;
   PassInst io;
   int v, b;
-  multi_row_desc(m, a, io, v, b);
+  multi_row_desc(m, gb, a, io, v, b);
   mc2r_run<F, OUT, EPI, 0>(v, a, io, b);
 }
 

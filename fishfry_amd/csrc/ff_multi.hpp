@@ -47,11 +47,21 @@ struct MultiRowDesc {
   int pad;
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b mod 8).  The launch's logical tiles are cut into 8 contiguous chunks
+// of about equal work (tiles of different variants differ in size), chunk x
+// running on XCD x in order, so neighbouring tiles of one solve share an L2
+// and no XCD idles while another still works; the grid is 8 times the largest
+// chunk, and workgroups past their chunk's end return at once.
+constexpr int kXcds = 8;
+
 struct MultiArgs {
   const void* desc;    // MultiColDesc[] or MultiRowDesc[]
-  const int* block0;   // first workgroup of each descriptor, ascending (block0[0] = 0)
+  const int* block0;   // first logical tile of each descriptor, ascending (block0[0] = 0)
   int ndesc;
-  int nblocks;         // workgroups of the launch
+  int nblocks;         // workgroups of the launch (kXcds * the largest chunk)
+  int chunk0[kXcds];   // first logical tile of each XCD's chunk
+  int chunkn[kXcds];   // tiles in each XCD's chunk
 };
 
 // The kernels are compiled in kMultiParts units (ff_multi.hip); unit PART

@@ -176,7 +176,8 @@ struct TableEntry {
 
 // Builds a table on the host, uploads it once per device and keeps it until
 // the process ends; kind 0: W_n[t], t < n; 1: exp(-2 pi i k/n), k < n/2 (at
-// least one entry); 2: the Bluestein tables of length n over m.
+// least one entry); 2: the Bluestein tables of length n over m; 3: the
+// finite-difference factors of order m, k < n (FP64).
 static const TableEntry* table_entry(int kind, int n, int m, int prec) {
   static std::mutex mu;
   static std::map<TableKey, TableEntry> cache;
@@ -190,6 +191,12 @@ static const TableEntry* table_entry(int kind, int n, int m, int prec) {
   TableEntry e;
   if (kind == 2) {
     push_bluestein(buf, n, m, prec, e.off[0], e.off[1], e.off[2]);
+  } else if (kind == 3) {  // finite-difference factors of order m
+    buf.resize((size_t)n * sizeof(double));
+    for (int t = 0; t < n; ++t) {
+      const double v = (double)fd_factor_ld(t, n, m);
+      memcpy(&buf[(size_t)t * sizeof(double)], &v, sizeof v);
+    }
   } else {
     const long long count = kind == 0 ? n : std::max(1, n / 2);
     const size_t elem = prec ? 8 : 16;
@@ -212,6 +219,24 @@ static const TableEntry* table_entry(int kind, int n, int m, int prec) {
     return nullptr;
   }
   return &(cache[key] = e);
+}
+
+// 4 s^2 (order 3) or 4 s^2 (1 + s^2/3) (order 5), s = sin(pi k/n): the
+// eigenvalue magnitude (times h^2) of the 3- / 5-point second difference on
+// exp(2 pi i k j/n).  sinl of pi min(k, n - k)/n keeps small angles accurate.
+long double fd_factor_ld(long long k, long long n, int order) {
+  long long kk = k % n;
+  if (kk < 0) kk += n;
+  if (n - kk < kk) kk = n - kk;
+  const long double pi = 3.14159265358979323846264338327950288L;
+  const long double sv = sinl(pi * (long double)kk / (long double)n);
+  const long double s2 = sv * sv;
+  return order == 5 ? 4.0L * s2 * (1.0L + s2 / 3.0L) : 4.0L * s2;
+}
+
+const double* table_symbol(int n, int order) {
+  const TableEntry* e = table_entry(3, n, order, 0);
+  return e ? static_cast<const double*>(e->mem) : nullptr;
 }
 
 const void* table_twiddle(int n, int prec) {

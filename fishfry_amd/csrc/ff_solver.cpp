@@ -58,8 +58,10 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   if (o.precision != FF_FP64 && o.precision != FF_FP32_REFINE)
     return set_error(FF_ERR_INVALID, "options.precision must be FF_FP64 or FF_FP32_REFINE");
   if (o.refine_steps < 0 || o.refine_steps > 8) return set_error(FF_ERR_INVALID, "refine_steps must be 0..8");
-  if (o.flags & ~(FF_PRESERVE_INPUT | FF_BLOCKED_LAYOUT | FF_SLAB_PIPELINE))
+  if (o.flags & ~(FF_PRESERVE_INPUT | FF_BLOCKED_LAYOUT | FF_SLAB_PIPELINE | FF_SYMBOL_3PT | FF_SYMBOL_5PT))
     return set_error(FF_ERR_INVALID, "unknown bits in options.flags");
+  if ((o.flags & FF_SYMBOL_3PT) && (o.flags & FF_SYMBOL_5PT))
+    return set_error(FF_ERR_INVALID, "options.flags: at most one of FF_SYMBOL_3PT and FF_SYMBOL_5PT");
 
   ff_plan* p = new (std::nothrow) ff_plan();
   if (!p) return set_error(FF_ERR_NOMEM, "host allocation failed");
@@ -73,6 +75,12 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   p->ddi = dd_const(n[0], lo[0], hi[0]);
   p->ddj = dd_const(n[1], lo[1], hi[1]);
   p->ddk = dd_const(n[2], lo[2], hi[2]);
+  p->sym = (o.flags & FF_SYMBOL_3PT) ? 3 : ((o.flags & FF_SYMBOL_5PT) ? 5 : 0);
+  for (int a = 0; a < 3; ++a) {
+    const double ih = double(n[a] - 1) / (hi[a] - lo[a]);  // 1/h, h = (hi - lo)/(n - 1)
+    p->fd[a] = ih * ih;
+    p->symtab[a] = nullptr;
+  }
   const int ones[3] = {1, 1, 1};
   p->bytes = reference_bytes(n, ones);
   // kh blocks of 8 complex doubles (one 128-byte line per block row)
@@ -140,6 +148,7 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     blue(p->cy[prec].line, prec, p->bl_y[prec]);
     blue(p->cx[prec].line, prec, p->bl_x[prec]);
   }
+  for (int a = 0; a < 3 && ok && p->sym; ++a) ok = (p->symtab[a] = table_symbol(n[a], p->sym)) != nullptr;
   if (!ok) {
     plan_destroy(p);
     return set_error(FF_ERR_NOMEM, "twiddle table allocation failed");
@@ -341,9 +350,18 @@ ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* sr
   a.ps = c.ps;
   a.nx = p->n[0];
   a.ny = p->n[1];
-  a.ddi = p->ddi;
-  a.ddj = p->ddj;
-  a.ddk = p->ddk;
+  if (p->sym) {
+    a.ddi = p->fd[0];
+    a.ddj = p->fd[1];
+    a.ddk = p->fd[2];
+    a.symx = p->symtab[0];
+    a.symy = p->symtab[1];
+    a.symz = p->symtab[2];
+  } else {
+    a.ddi = p->ddi;
+    a.ddj = p->ddj;
+    a.ddk = p->ddk;
+  }
   a.dc_out = dc_out;
   a.tw = axis == 1 ? p->tw_y[prec] : p->tw_x[prec];
   a.lp = c.line.lp;
@@ -367,15 +385,22 @@ static ff_status solve_fp32_refine(const ff_plan* p, const double* f, double* ph
   if ((st = run_col(p, 1, 1, COL_INV, spec, spec, s))) return st;
   if ((st = run_zinv(p, 1, 0, EPI_STORE, spec, phi, nullptr, nullptr, s))) return st;
   for (int it = 0; it < p->opt.refine_steps; ++it) {
-    // r = f - L_h phi in FP64, with the same spectral symbol the filter inverts;
-    // the x pass also captures sum(phi) for the FP64 mean removal
-    if ((st = run_zfwd(p, 0, 0, phi, spec, s))) return st;
-    if ((st = run_col(p, 0, 1, COL_FWD, spec, spec, s))) return st;
-    if ((st = run_col(p, 0, 0, COL_SYMBOL, spec, spec, s, p->dc))) return st;
-    if ((st = run_col(p, 0, 1, COL_INV, spec, spec, s))) return st;
-    // r = (f - mean f) - L_h phi: L_h phi has no mean, and removing f's mean
-    // keeps the FP32 rounding of r relative to the residual itself
-    if ((st = run_zinv(p, 0, 1, EPI_RESIDUAL, spec, p->res, f, p->dcf, s))) return st;
+    if (p->sym) {
+      // finite-difference symbol: r = (f - mean f) - L_h phi with the stencil
+      // in real space, one pass over f and phi; sum(phi) by a two-stage
+      // reduction through the (idle) spectrum area
+      if ((st = run_stencil(p, phi, f, p->res, false, p->dcf, static_cast<double*>(spec), p->dc, s))) return st;
+    } else {
+      // r = f - L_h phi in FP64, with the same spectral symbol the filter inverts;
+      // the x pass also captures sum(phi) for the FP64 mean removal
+      if ((st = run_zfwd(p, 0, 0, phi, spec, s))) return st;
+      if ((st = run_col(p, 0, 1, COL_FWD, spec, spec, s))) return st;
+      if ((st = run_col(p, 0, 0, COL_SYMBOL, spec, spec, s, p->dc))) return st;
+      if ((st = run_col(p, 0, 1, COL_INV, spec, spec, s))) return st;
+      // r = (f - mean f) - L_h phi: L_h phi has no mean, and removing f's mean
+      // keeps the FP32 rounding of r relative to the residual itself
+      if ((st = run_zinv(p, 0, 1, EPI_RESIDUAL, spec, p->res, f, p->dcf, s))) return st;
+    }
     // correction in FP32: phi += L_h^-1 r - mean(phi)
     if ((st = run_zfwd(p, 1, 1, p->res, spec, s))) return st;
     if ((st = run_col(p, 1, 1, COL_FWD, spec, spec, s))) return st;

@@ -12,6 +12,12 @@ struct ff_plan {
   long long N;   // nx*ny*nz
   long long Nh;  // nx*ny*nh
   double ddi, ddj, ddk;
+  // finite-difference symbol (FF_SYMBOL_3PT / _5PT): order 3 or 5, 0 for the
+  // spectral filter; 1/h^2 per axis and the per-axis factor tables
+  // 4 sin^2(pi k/n) [(1 + sin^2(pi k/n)/3)] (resident cache, FP64)
+  int sym;
+  double fd[3];
+  const double* symtab[3];
   size_t bytes;
   ff_options opt;
   int device;
@@ -79,5 +85,11 @@ void plan_destroy(ff_plan* p);
 size_t plan_scratch_bytes(const ff_plan* p);
 void plan_bind_scratch(ff_plan* p, void* base);
 size_t reference_bytes(const int n[3], const int m[3]);
+// The FP64 stencil residual of a finite-difference plan: r = (f - sum_f/N) -
+// L_h phi stored as float, and sum(phi) into *sum_phi (deterministic two-stage
+// reduction through `partial`, at least nx*ny doubles).  With r_f64 set, out
+// receives L_h phi in FP64 instead (f, sum_f unused).
+ff_status run_stencil(const ff_plan* p, const double* phi, const double* f, void* out, bool r_f64,
+                      const double* sum_f, double* partial, double* sum_phi, hipStream_t s);
 
 }  // namespace ff

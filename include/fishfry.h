@@ -60,10 +60,23 @@ typedef enum ff_precision {
                                just wrote while it is still in the 256 MB Infinity
                                Cache (plain spectrum layout only)                      */
 
+#define FF_SYMBOL_3PT 8     /* finite-difference symbol of the 3-point Laplacian on every
+                               axis in place of the spectral filter: for mode k of an
+                               n-point axis with spacing h = (hi - lo)/(n - 1) and
+                               s = sin(pi k/n) the axis term is (4/h^2) s^2; the solve
+                               inverts the 7-point stencil exactly.  Named, not
+                               implemented, by the reference (make-frontier.sh:9-10,
+                               make-summit.sh:7-8).  FP32 refinement computes its FP64
+                               residual with the stencil in real space (one pass over
+                               f and phi instead of an FP64 transform round trip)     */
+#define FF_SYMBOL_5PT 16    /* the same with the 5-point (fourth-order) Laplacian: axis
+                               term (4/h^2) s^2 (1 + s^2/3), 13-point stencil residual   */
+
 typedef struct ff_options {
   int precision;    /* ff_precision                                             */
   int refine_steps; /* FF_FP32_REFINE only: 0..8                                */
-  int flags;        /* 0 or FF_PRESERVE_INPUT | FF_BLOCKED_LAYOUT | FF_SLAB_PIPELINE */
+  int flags;        /* 0 or FF_PRESERVE_INPUT | FF_BLOCKED_LAYOUT | FF_SLAB_PIPELINE,
+                       plus at most one of FF_SYMBOL_3PT / FF_SYMBOL_5PT          */
   int slab_planes;  /* FF_SLAB_PIPELINE: x-planes per slab (0: about 64 MB of
                        half spectrum per slab)                                  */
 } ff_options;
@@ -181,6 +194,13 @@ void ff_tune_reset(void);
  * ncols is the column count of the pass (nh), used to cap T. */
 ff_status ff_tune_query(int kind, int n, int precision, int ncols, ff_tune* out);
 ff_status ff_solve_pass(const ff_plan* plan, int pass, const double* src, double* dst, void* stream);
+
+/* No counterpart in the reference.  out = L_h phi in FP64 with the plan's
+ * finite-difference stencil (FF_SYMBOL_3PT / FF_SYMBOL_5PT plans; periodic,
+ * [nx][ny][nz] arrays of nx*ny*nz doubles): the operator the solve inverts,
+ * so L_h(solve(f)) = f - mean(f).  FF_ERR_INVALID for spectral plans.
+ * Enqueued on stream. */
+ff_status ff_stencil_apply(const ff_plan* plan, const double* phi, double* out, void* stream);
 
 /* ----------------------------------------------------------------------- */
 /* Batched solves: many independent grids (uneven shapes, FP64 or FP32 +    */

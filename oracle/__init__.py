@@ -53,7 +53,14 @@ def lib():
         L.ora_init_field.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 6 + [_dp]
         L.ora_diff.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 6 + [_dp, _dp]
         L.ora_twiddle.argtypes = [ctypes.c_long, ctypes.c_long, _dp, _dp]
-        for f in ("ora_fft", "ora_rfft", "ora_irfft", "ora_solve"):
+        L.ora_fd_factor.argtypes = [ctypes.c_long, ctypes.c_long, ctypes.c_int]
+        L.ora_fd_factor.restype = ctypes.c_double
+        L.ora_filter_sym.argtypes = [_ip, _dp, _dp, ctypes.c_int, _dp]
+        L.ora_filter_sym.restype = None
+        L.ora_solve_sym.argtypes = [_ip, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
+        L.ora_stencil.argtypes = [_ip, _dp, _dp, ctypes.c_int, _dp, _dp]
+        L.ora_stencil.restype = None
+        for f in ("ora_fft", "ora_rfft", "ora_irfft", "ora_solve", "ora_solve_sym"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -118,8 +125,10 @@ def grid(n):
     return lo, hi, dx
 
 
-def solve(density: np.ndarray, lo=None, hi=None, nthreads: int = 0) -> np.ndarray:
-    """Full periodic Poisson solve, ParisPencil.cpp:95-425 at 1x1x1 tasks."""
+def solve(density: np.ndarray, lo=None, hi=None, nthreads: int = 0, symbol: int = 0) -> np.ndarray:
+    """Full periodic Poisson solve, ParisPencil.cpp:95-425 at 1x1x1 tasks.
+    symbol 3 or 5: the 3-point / 5-point finite-difference symbol in place of
+    the spectral filter (the options make-frontier.sh:9-10 names)."""
     d = np.ascontiguousarray(density, dtype=np.float64)
     n = d.shape
     if lo is None or hi is None:
@@ -127,20 +136,38 @@ def solve(density: np.ndarray, lo=None, hi=None, nthreads: int = 0) -> np.ndarra
     nh = n[2] // 2 + 1
     work = np.empty(2 * n[0] * n[1] * nh, dtype=np.float64)
     out = np.empty_like(d)
-    if lib().ora_solve(_ints(n), _dbls(lo), _dbls(hi), _ptr(d), _ptr(out), _ptr(work), int(nthreads)):
+    if lib().ora_solve_sym(_ints(n), _dbls(lo), _dbls(hi), int(symbol), _ptr(d), _ptr(out), _ptr(work),
+                           int(nthreads)):
         raise RuntimeError("ora_solve failed")
     return out
 
 
-def filter_half(spec: np.ndarray, nz: int, lo=None, hi=None) -> np.ndarray:
-    """Spectral filter of ParisPencil.cpp:263-285 applied to a copy of the
-    half spectrum spec[nx][ny][nz/2+1]."""
+def fd_factor(k: int, n: int, order: int) -> float:
+    """4 s^2 (3-point) or 4 s^2 (1 + s^2/3) (5-point), s = sin(pi k / n)."""
+    return lib().ora_fd_factor(int(k), int(n), int(order))
+
+
+def stencil(phi: np.ndarray, order: int, lo=None, hi=None) -> np.ndarray:
+    """L_h phi with the 3- or 5-point stencil on every axis (periodic)."""
+    p = np.ascontiguousarray(phi, dtype=np.float64)
+    n = p.shape
+    if lo is None or hi is None:
+        lo, hi, _ = grid(n)
+    out = np.empty_like(p)
+    lib().ora_stencil(_ints(n), _dbls(lo), _dbls(hi), int(order), _ptr(p), _ptr(out))
+    return out
+
+
+def filter_half(spec: np.ndarray, nz: int, lo=None, hi=None, symbol: int = 0) -> np.ndarray:
+    """Spectral filter of ParisPencil.cpp:263-285 (or, symbol 3 / 5, the
+    finite-difference one) applied to a copy of the half spectrum
+    spec[nx][ny][nz/2+1]."""
     a = np.ascontiguousarray(spec, dtype=np.complex128).copy()
     n = (a.shape[0], a.shape[1], nz)
     assert a.shape[2] == nz // 2 + 1
     if lo is None or hi is None:
         lo, hi, _ = grid(n)
-    lib().ora_filter(_ints(n), _dbls(lo), _dbls(hi), _ptr(a.view(np.float64)))
+    lib().ora_filter_sym(_ints(n), _dbls(lo), _dbls(hi), int(symbol), _ptr(a.view(np.float64)))
     return a
 
 

@@ -307,6 +307,97 @@ size_t ora_bytes(const int n[3], const int m[3])
 
 static inline double sqr(double x) { return x * x; }
 
+/* Finite-difference symbols, per axis, for mode k of an n-point periodic
+ * axis with spacing h = (hi - lo)/(n - 1) (the reference's hi is the last
+ * cell, FishFry.hpp:133-134).  With s = sin(pi k / n):
+ *   3-point  (u[j+1] - 2u[j] + u[j-1]) / h^2                     -> -(4/h^2) s^2
+ *   5-point  (-u[j+2] + 16u[j+1] - 30u[j] + 16u[j-1] - u[j-2]) / (12 h^2)
+ *                                                                 -> -(4/h^2) s^2 (1 + s^2/3)
+ * (the eigenvalues of the stencils on exp(2 pi i k j / n)).  The reference's
+ * build scripts name these symbols (PARIS_3PT / PARIS_5PT, make-frontier.sh:9-10,
+ * make-summit.sh:7-8) but its source implements only the spectral one.
+ * ora_fd_factor returns 4 s^2 [(1 + s^2/3)], in long double from sinl of
+ * pi min(k, n - k) / n, rounded to double. */
+double ora_fd_factor(long k, long n, int order)
+{
+  long kk = k % n;
+  if (kk < 0) kk += n;
+  if (n - kk < kk) kk = n - kk;
+  const long double sv = sinl(3.14159265358979323846264338327950288L * (long double)kk / (long double)n);
+  const long double s2 = sv * sv;
+  const long double g = order == 5 ? 4.0L * s2 * (1.0L + s2 / 3.0L) : 4.0L * s2;
+  return (double)g;
+}
+
+/* 1/h^2 of an axis, h = (hi - lo)/(n - 1). */
+static double fd_scale(int n, double lo, double hi)
+{
+  const double ih = (double)(n - 1) / (hi - lo);
+  return ih * ih;
+}
+
+void ora_filter_sym(const int n[3], const double lo[3], const double hi[3], int order, double *spec)
+{
+  if (order != 3 && order != 5) {
+    ora_filter(n, lo, hi, spec);
+    return;
+  }
+  const int ni = n[0], nj = n[1], nk = n[2];
+  const int nh = nk / 2 + 1;
+  const double ci = fd_scale(ni, lo[0], hi[0]);
+  const double cj = fd_scale(nj, lo[1], hi[1]);
+  const double ck = fd_scale(nk, lo[2], hi[2]);
+  cplx *w = (cplx *)spec;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < ni; i++) {
+    const double ti = ora_fd_factor(i, ni, order) * ci;
+    for (int j = 0; j < nj; j++) {
+      const double tj = ora_fd_factor(j, nj, order) * cj;
+      for (int k = 0; k < nh; k++) {
+        const long iab = ((long)i * nj + j) * nh + k;
+        if (i || j || k) {
+          const double tk = ora_fd_factor(k, nk, order) * ck;
+          const double d = -1.0 / ((ti + tj) + tk);
+          w[iab] = d * creal(w[iab]) + I * (d * cimag(w[iab]));
+        } else {
+          w[iab] = 0;
+        }
+      }
+    }
+  }
+}
+
+/* L_h phi with the 3- or 5-point stencil on every axis, periodic, in real
+ * space: the operator whose symbol ora_filter_sym inverts. */
+void ora_stencil(const int n[3], const double lo[3], const double hi[3], int order, const double *phi, double *out)
+{
+  const int ni = n[0], nj = n[1], nk = n[2];
+  const double c[3] = {fd_scale(ni, lo[0], hi[0]), fd_scale(nj, lo[1], hi[1]), fd_scale(nk, lo[2], hi[2])};
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < ni; i++) {
+    for (int j = 0; j < nj; j++) {
+      for (int k = 0; k < nk; k++) {
+        const int idx[3] = {i, j, k}, ext[3] = {ni, nj, nk};
+        const double center = phi[((long)i * nj + j) * nk + k];
+        double lap = 0.0;
+        for (int a = 0; a < 3; a++) {
+          double v[5];
+          for (int o = -2; o <= 2; o++) {
+            int q[3] = {idx[0], idx[1], idx[2]};
+            q[a] = ((q[a] + o) % ext[a] + ext[a]) % ext[a];
+            v[o + 2] = phi[((long)q[0] * nj + q[1]) * nk + q[2]];
+          }
+          double d2;
+          if (order == 5) d2 = (16.0 * (v[3] + v[1]) - (v[4] + v[0]) - 30.0 * center) / 12.0;
+          else d2 = (v[3] + v[1]) - 2.0 * center;
+          lap += c[a] * d2;
+        }
+        out[((long)i * nj + j) * nk + k] = lap;
+      }
+    }
+  }
+}
+
 void ora_filter(const int n[3], const double lo[3], const double hi[3], double *spec)
 {
   const int ni = n[0], nj = n[1], nk = n[2];
@@ -339,6 +430,12 @@ void ora_filter(const int n[3], const double lo[3], const double hi[3], double *
 int ora_solve(const int n[3], const double lo[3], const double hi[3],
               const double *density, double *potential, double *work, int nthreads)
 {
+  return ora_solve_sym(n, lo, hi, 0, density, potential, work, nthreads);
+}
+
+int ora_solve_sym(const int n[3], const double lo[3], const double hi[3], int order,
+                  const double *density, double *potential, double *work, int nthreads)
+{
   const int ni = n[0], nj = n[1], nk = n[2];
   if (ni < 1 || nj < 1 || nk < 1) return -1;
   const int nh = nk / 2 + 1;
@@ -362,7 +459,7 @@ int ora_solve(const int n[3], const double lo[3], const double hi[3],
   if (!err) err |= fft_lines(&pi_, w, (long)nj * nh, (long)nj * nh, 1, -1);
   if (err) { plan_free(&pi_); plan_free(&pj_); return -1; }
 
-  ora_filter(n, lo, hi, work);
+  ora_filter_sym(n, lo, hi, order, work);
 
   /* Z2Z inverse along x, then y  (ParisPencil.cpp:288,334) */
   err |= fft_lines(&pi_, w, (long)nj * nh, (long)nj * nh, 1, +1);

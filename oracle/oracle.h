@@ -18,6 +18,11 @@
  *   ora_init_field closed form evaluated in long double by numpy
  *   ora_diff       numpy long-double sums
  *   ora_dd         closed-form values for n in {32..1024} (1 ulp cases)
+ *   ora_solve_sym  the stencil (numpy, np.roll) applied to its phi gives
+ *                  f - mean f; errors against the analytic phi fall as h^2
+ *                  (3-point) and h^4 (5-point); single-mode closed forms
+ *   ora_stencil    numpy stencil on random fields
+ *   ora_fd_factor  small-angle limits and the 3-point / 5-point relation
  */
 #ifndef FISHFRY_ORACLE_H
 #define FISHFRY_ORACLE_H
@@ -63,6 +68,20 @@ void ora_filter(const int n[3], const double lo[3], const double hi[3], double *
  * the OpenMP default. */
 int ora_solve(const int n[3], const double lo[3], const double hi[3],
               const double *density, double *potential, double *work, int nthreads);
+
+/* Finite-difference symbols (order 3: the 3-point Laplacian, 5: the 5-point
+ * one, on every axis; any other order: the spectral filter above), named but
+ * not implemented by the reference (make-frontier.sh:9-10, make-summit.sh:7-8):
+ * for mode k of an n-point axis with spacing h = (hi - lo)/(n - 1) and
+ * s = sin(pi k/n), the axis term is (4/h^2) s^2 (3-point) or
+ * (4/h^2) s^2 (1 + s^2/3) (5-point); the filter is -1/(sum of the axis terms),
+ * the (0,0,0) mode set to 0.  ora_fd_factor: 4 s^2 [(1 + s^2/3)].
+ * ora_stencil applies the matching stencil in real space (periodic). */
+double ora_fd_factor(long k, long n, int order);
+void ora_filter_sym(const int n[3], const double lo[3], const double hi[3], int order, double *spec);
+int ora_solve_sym(const int n[3], const double lo[3], const double hi[3], int order,
+                  const double *density, double *potential, double *work, int nthreads);
+void ora_stencil(const int n[3], const double lo[3], const double hi[3], int order, const double *phi, double *out);
 
 /* The benchmark's right-hand side f = lap(phi), FishFry.hpp:14-45, at
  * x_i = xlo + i*dx (and likewise y, z).  Layout [ni][nj][nk]. */

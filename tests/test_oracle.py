@@ -212,3 +212,119 @@ def test_init_field_long_double(golden):
         f = oracle.init_field(shp)
         ref = golden[f"solve_field_in_{tag}"]
         assert np.abs(f - ref).max() < 1e-14 * np.abs(ref).max()
+
+
+def test_diff_direct_long_double():
+    """ora_diff against sums formed here in numpy long double from the
+    analytic potential phi = e^{sin x} cos x * e^{cos y} sin y * e^{cos z} cos z
+    (FishFry.hpp:57-101) on an arbitrary field: a wrong axis function, a
+    missing square or a wrong maximum fails it, and phi = 0 gives the scales."""
+    rng = np.random.default_rng(71)
+    for n in ((8, 12, 10), (16, 16, 16), (7, 5, 11)):
+        lo, _, dx = oracle.grid(n)
+        ld = np.longdouble
+        x = ld(lo[0]) + np.arange(n[0], dtype=ld) * ld(dx[0])
+        y = ld(lo[1]) + np.arange(n[1], dtype=ld) * ld(dx[1])
+        z = ld(lo[2]) + np.arange(n[2], dtype=ld) * ld(dx[2])
+        px = np.exp(np.sin(x)) * np.cos(x)
+        py = np.exp(np.cos(y)) * np.sin(y)
+        pz = np.exp(np.cos(z)) * np.cos(z)
+        exact = px[:, None, None] * py[None, :, None] * pz[None, None, :]
+        for phi in (rng.standard_normal(n), np.zeros(n), np.asarray(exact, dtype=np.float64) * 0.5):
+            e = np.abs(exact - phi.astype(ld))
+            want = (np.sum(e), np.sum(e * e), np.max(e))
+            got = oracle.diff(phi)
+            for a, b in zip(got, want):
+                assert abs(ld(a) - b) <= 1e-13 * abs(b), (n, got, want)
+
+
+def numpy_stencil(phi, order, lo, hi):
+    """L_h phi written independently with np.roll (periodic)."""
+    out = np.zeros_like(phi)
+    for ax in range(3):
+        n = phi.shape[ax]
+        h = (hi[ax] - lo[ax]) / (n - 1)
+        p1, m1 = np.roll(phi, -1, ax), np.roll(phi, 1, ax)
+        if order == 3:
+            d2 = p1 + m1 - 2.0 * phi
+        else:
+            p2, m2 = np.roll(phi, -2, ax), np.roll(phi, 2, ax)
+            d2 = (16.0 * (p1 + m1) - (p2 + m2) - 30.0 * phi) / 12.0
+        out += d2 / (h * h)
+    return out
+
+
+@pytest.mark.parametrize("order", [3, 5])
+@pytest.mark.parametrize("n", [(8, 10, 12), (9, 7, 11), (16, 16, 16), (5, 6, 4), (12, 3, 2)])
+def test_fd_solve_inverts_the_stencil(order, n):
+    """The finite-difference solve inverts the stencil written here with
+    np.roll: L_h phi = f - mean f, phi has no mean; and the oracle's own
+    real-space stencil agrees with the numpy one on a random field.  A wrong
+    symbol (sin for sin^2, a dropped 1/h^2, a wrong 5-point coefficient)
+    fails the first check."""
+    rng = np.random.default_rng(72)
+    f = rng.standard_normal(n)
+    lo, hi = [0.0, -1.0, 0.5], [2.0, 3.0, 4.5]
+    phi = oracle.solve(f, lo, hi, symbol=order)
+    lap = numpy_stencil(phi, order, lo, hi)
+    assert np.max(np.abs(lap - (f - f.mean()))) < 1e-12 * np.max(np.abs(f))
+    assert abs(phi.mean()) < 1e-14 * np.max(np.abs(phi))
+    g = rng.standard_normal(n)
+    assert np.max(np.abs(oracle.stencil(g, order, lo, hi) - numpy_stencil(g, order, lo, hi))) < 1e-12 * np.max(
+        np.abs(numpy_stencil(g, order, lo, hi)))
+
+
+def test_fd_convergence_orders():
+    """Against the benchmark's analytic potential the 3-point solve's error
+    falls as h^2 and the 5-point one's as h^4 (error ratios near 4 and 16 when
+    n doubles), while the spectral solve is exact to round-off."""
+    errs = {3: [], 5: []}
+    for m in (32, 64):
+        n = (m, m, m)
+        f = oracle.init_field(n)
+        for order in (3, 5):
+            errs[order].append(oracle.norms(oracle.solve(f, symbol=order))[2])
+    r3 = errs[3][0] / errs[3][1]
+    r5 = errs[5][0] / errs[5][1]
+    assert 3.6 < r3 < 4.4, errs
+    assert 14.0 < r5 < 18.0, errs
+    assert errs[5][1] < errs[3][1] / 50
+
+
+def test_fd_factor_limits():
+    """Small angles: with theta = 2 pi k / n the 3-point factor is theta^2 -
+    theta^4/12 + ..., the 5-point one theta^2 - theta^6/90 + ... (the Taylor
+    series of the two stencils' eigenvalues); symmetric in k -> n - k; zero at
+    k = 0; at the Nyquist mode 4 and 16/3."""
+    n = 4096
+    for k in (1, 2, 5, 9):
+        t = 2 * math.pi * k / n
+        g3, g5 = oracle.fd_factor(k, n, 3), oracle.fd_factor(k, n, 5)
+        assert abs((t * t - g3) / t ** 4 - 1 / 12) < 1e-3
+        assert abs((t * t - g5) / t ** 6 - 1 / 90) < 2e-2 or abs(t * t - g5) < 1e-15 * t * t
+        assert oracle.fd_factor(n - k, n, 3) == g3 and oracle.fd_factor(n - k, n, 5) == g5
+    assert oracle.fd_factor(0, 64, 3) == 0.0 and oracle.fd_factor(0, 64, 5) == 0.0
+    assert oracle.fd_factor(32, 64, 3) == 4.0
+    assert abs(oracle.fd_factor(32, 64, 5) - 16.0 / 3.0) < 1e-15
+
+
+@pytest.mark.parametrize("order", [3, 5])
+def test_fd_single_mode(order):
+    """One Fourier mode: the discrete solution is -f divided by the stencil's
+    eigenvalue, formed here from cosines (2 - 2 cos(theta), 5-point
+    (30 - 32 cos(theta) + 2 cos(2 theta))/12), a different route to the
+    same number than the oracle's sin^2 factor."""
+    n = (12, 10, 16)
+    lo, hi, dx = oracle.grid(n)
+    k = (2, 3, 5)
+    idx = np.meshgrid(*[np.arange(m) for m in n], indexing="ij")
+    f = np.cos(sum(2 * math.pi * k[a] * idx[a] / n[a] for a in range(3)))
+    lam = 0.0
+    for a in range(3):
+        th = 2 * math.pi * k[a] / n[a]
+        h = (hi[a] - lo[a]) / (n[a] - 1)
+        e = 2 - 2 * math.cos(th) if order == 3 else (30 - 32 * math.cos(th) + 2 * math.cos(2 * th)) / 12
+        lam += e / (h * h)
+    phi = oracle.solve(f, symbol=order)
+    # (the cosine forms cancel for small angles: a few ulp of the eigenvalue)
+    assert np.max(np.abs(phi + f / lam)) < 1e-13 * np.max(np.abs(f / lam))
