@@ -27,6 +27,7 @@ static const void* col_fn(int f32, int mode) {
       case COL_FWD: return (const void*)&k_col<float, E, COL_FWD, S, G, BL>;
       case COL_INV: return (const void*)&k_col<float, E, COL_INV, S, G, BL>;
       case COL_FILTER: return (const void*)&k_col<float, E, COL_FILTER, S, G, BL>;
+      case COL_FILTER_FD: return (const void*)&k_col<float, E, COL_FILTER_FD, S, G, BL>;
       default: return nullptr;
     }
   }
@@ -35,6 +36,7 @@ static const void* col_fn(int f32, int mode) {
     case COL_INV: return (const void*)&k_col<double, E, COL_INV, S, G, BL>;
     case COL_FILTER: return (const void*)&k_col<double, E, COL_FILTER, S, G, BL>;
     case COL_SYMBOL: return (const void*)&k_col<double, E, COL_SYMBOL, S, G, BL>;
+    case COL_FILTER_FD: return (const void*)&k_col<double, E, COL_FILTER_FD, S, G, BL>;
     default: return nullptr;
   }
 }
@@ -57,6 +59,9 @@ static const void* col_static_modes(int mode) {
       return nullptr;
     case COL_FILTER:
       if constexpr (x) return (const void*)&k_col<F, E, COL_FILTER, S, 1, false, N, T, W>;
+      return nullptr;
+    case COL_FILTER_FD:
+      if constexpr (x) return (const void*)&k_col<F, E, COL_FILTER_FD, S, 1, false, N, T, W>;
       return nullptr;
     case COL_SYMBOL:
       if constexpr (x && std::is_same<F, double>::value)

@@ -147,7 +147,10 @@ __device__ __forceinline__ PassInst inst_of(const ColArgs& a) {
 
 // MODE: 0 forward, 1 inverse, 2 forward * filter * inverse (the solve),
 //       3 forward * symbol * inverse (applies L_h for a residual).
-enum { COL_FWD = 0, COL_INV = 1, COL_FILTER = 2, COL_SYMBOL = 3 };
+//       4 forward * finite-difference filter * inverse (symbol from the
+//         per-axis tables symx, symy, symz; a separate instantiation so the
+//         spectral kernels carry none of its registers).
+enum { COL_FWD = 0, COL_INV = 1, COL_FILTER = 2, COL_SYMBOL = 3, COL_FILTER_FD = 4 };
 
 // Unnormalized DFT of direction DIR (-1 forward, +1 inverse) of a line of
 // length nline held at positions r0 + i*L.  When nline equals the engine
@@ -217,7 +220,7 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
     const int jj = outer, kk = col;
     // the (0,0,0) term before the multiply is the sum of the input field
     if (io.dc_out && valid && r0 == 0 && jj == 0 && kk == 0) *glb<double>(io.dc_out) = double(v[0].x);
-    if (a.symx) {
+    if constexpr (MODE == COL_FILTER_FD) {
       // columns outside the plane and Bluestein positions past the line read
       // entry 0 (their values are never stored)
       const double tj = glb<const double>(a.symy)[jj] * io.ddj;
@@ -227,9 +230,7 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
         const int ii = r0 + i * L;
         const double gi = glb<const double>(a.symx)[ii < a.nx ? ii : 0];
         const double s2 = fmadd(gk, io.ddk, fmadd(gi, io.ddi, tj));  // (i + j) + k
-        double d;
-        if constexpr (MODE == COL_FILTER) d = (ii | jj | kk) ? -1.0 / s2 : 0.0;
-        else d = -s2;
+        const double d = (ii | jj | kk) ? -1.0 / s2 : 0.0;
         const F df = F(d);
         v[i] = cx<F>{df * v[i].x, df * v[i].y};
       }

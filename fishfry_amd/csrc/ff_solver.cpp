@@ -307,6 +307,9 @@ ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* sr
                   double* dc_out, int lay_in, int lay_out, int outer0, int nouter) {
   const ColCfg& c = axis == 1 ? p->cy[prec] : p->cx[prec];
   if (src == dst && lay_in != lay_out) return set_error(FF_ERR_INVALID, "in-place column pass must keep its layout");
+  if (mode == COL_FILTER && p->sym) mode = COL_FILTER_FD;
+  if (mode == COL_SYMBOL && p->sym)
+    return set_error(FF_ERR_INVALID, "finite-difference plans apply L_h with the stencil, not the x pass");
   ColArgs a;
   memset(&a, 0, sizeof a);
   a.src = src;

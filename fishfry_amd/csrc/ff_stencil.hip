@@ -8,13 +8,17 @@
 // r = (f - mean f) - L_h phi here, in one pass over f and phi (20 bytes per
 // point with an FP32 r), instead of an FP64 transform round trip.
 //
-// A workgroup owns `rows` consecutive rows (i, j) of the [i][j][k] array;
-// thread t handles positions k = t mod TPR, k + TPR, ... of row t / TPR of
-// each group of 256 / TPR rows, so short rows still keep every lane busy.
-// Neighbour rows along x and y are read directly (they are the rows this and
-// nearby workgroups read as centres, so they come from the caches); the sum
-// of phi for the mean removal is reduced in a fixed order: per thread, per
-// workgroup (tree), then over workgroups (k_stencil_sum).
+// A workgroup owns a tile of TJ rows (y) by TK positions (z) and marches
+// along x through a chunk of planes.  Each thread keeps the x-line of its
+// (j, k) in a register queue phi[i - H .. i + H] (H = 1 or 2), so every
+// value of phi is read from memory once as it enters a queue; the tile of
+// plane i, with an H-wide halo in y and z, goes to LDS (two buffers,
+// alternating by plane, so one barrier per plane suffices) for the y and z
+// neighbours.  Tiles and halos wrap periodically; lanes past the grid's
+// edge load wrapped values so the LDS tile is consistent, and store nothing.
+// The sum of phi for the mean removal is reduced in a fixed order: per
+// thread (by plane), per workgroup (tree), then over workgroups
+// (k_stencil_sum).
 #include <algorithm>
 #include <climits>
 #include <cstring>
@@ -28,14 +32,14 @@ constexpr int kStencilThreads = 256;
 struct StencilArgs {
   const double* phi;
   const double* f;
-  void* out;               // float r (residual) or double L_h phi
+  void* out;           // float r (residual) or double L_h phi
   int nx, ny, nz;
-  int rows, nblocks;       // rows per workgroup, workgroups
-  int lgtpr;               // log2 of the threads per row
-  double cx, cy, cz;       // 1/h^2 per axis (5-point: divided by 12)
-  double scale;            // 1/(nx ny nz)
-  const double* sum_f;     // residual: sum of f
-  double* partial;         // residual: sum of phi per workgroup
+  int lgtk;            // log2 of TK (z positions per tile); TJ = 256 / TK rows
+  int tiles_j, tiles_k, chunks, chunk;  // workgroup grid: tiles_j * tiles_k * chunks; planes per chunk
+  double cx, cy, cz;   // 1/h^2 per axis (5-point: divided by 12)
+  double scale;        // 1/(nx ny nz)
+  const double* sum_f; // residual: sum of f
+  double* partial;     // residual: sum of phi per workgroup
 };
 
 __device__ __forceinline__ int wrap_idx(int v, int n) {
@@ -45,57 +49,87 @@ __device__ __forceinline__ int wrap_idx(int v, int n) {
 
 template <int ORDER, bool RESID>
 __global__ __launch_bounds__(kStencilThreads) void k_stencil(const StencilArgs a) {
+  constexpr int H = ORDER == 3 ? 1 : 2;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* lds = reinterpret_cast<double*>(smem_raw);
   __shared__ double red[kStencilThreads];
   const int tid = threadIdx.x;
-  const int tpr = 1 << a.lgtpr;
-  const int sub = tid >> a.lgtpr, k0 = tid & (tpr - 1);
-  const int rpi = kStencilThreads >> a.lgtpr;  // rows per iteration
-  const int b = xcd_remap(blockIdx.x, a.nblocks);
-  const long long nrows = (long long)a.nx * a.ny;
-  const long long nz = a.nz;
+  const int TK = 1 << a.lgtk, TJ = kStencilThreads >> a.lgtk;
+  const int W = TK + 2 * H;                 // LDS tile pitch
+  const int tile_cells = (TJ + 2 * H) * W;  // one buffer
+  const int b = xcd_remap(blockIdx.x, a.tiles_j * a.tiles_k * a.chunks);
+  const int c = b % a.chunks;
+  const int t = b / a.chunks;
+  const int tj = t / a.tiles_k, tk = t - (t / a.tiles_k) * a.tiles_k;
+  const int jl = tid >> a.lgtk, kl = tid & (TK - 1);
+  const int j0 = tj * TJ, k0 = tk * TK;
+  const int j = j0 + jl, k = k0 + kl;
+  const bool valid = j < a.ny && k < a.nz;
+  const int jw = wrap_idx(j, a.ny), kw = wrap_idx(k, a.nz);
+  const long long plane = (long long)a.ny * a.nz;
+  const long long own = (long long)jw * a.nz + kw;  // offset of (j, k) in a plane
+  // halo cells of this thread (the same every plane): plane offset and LDS index
+  const int halo = tile_cells - TJ * TK;
+  long long hoff[2];
+  int hidx[2];
+  int nh = 0;
+  for (int e = tid; e < halo && nh < 2; e += kStencilThreads) {
+    // enumerate the ring: rows -H..-1 and TJ..TJ+H-1 whole, then the side columns
+    int jj, kk;
+    const int top = 2 * H * W;
+    if (e < top) {
+      const int r = e / W;
+      jj = r < H ? r - H : TJ + (r - H);
+      kk = e - r * W - H;
+    } else {
+      const int e2 = e - top, r = e2 / (2 * H), q = e2 - r * 2 * H;
+      jj = r;
+      kk = q < H ? q - H : TK + (q - H);
+    }
+    hoff[nh] = (long long)wrap_idx(j0 + jj, a.ny) * a.nz + wrap_idx(k0 + kk, a.nz);
+    hidx[nh] = (jj + H) * W + (kk + H);
+    nh++;
+  }
   const FF_GLB double* phi = glb<const double>(a.phi);
   double meanf = 0.0;
   if constexpr (RESID) meanf = *glb<const double>(a.sum_f) * a.scale;
+  const int i0 = c * a.chunk, i1 = min(a.nx, i0 + a.chunk);
+  double q[2 * H + 1];
+#pragma unroll
+  for (int m = 0; m < 2 * H; ++m) q[m] = phi[(long long)wrap_idx(i0 - H + m, a.nx) * plane + own];
   double acc = 0.0;
-  for (int r0 = 0; r0 < a.rows; r0 += rpi) {
-    const long long row = (long long)b * a.rows + r0 + sub;
-    if (r0 + sub >= a.rows || row >= nrows) continue;
-    const int i = (int)(row / a.ny), j = (int)(row - (long long)i * a.ny);
-    const FF_GLB double* c0 = phi + row * nz;
-    const FF_GLB double* xm1 = phi + ((long long)wrap_idx(i - 1, a.nx) * a.ny + j) * nz;
-    const FF_GLB double* xp1 = phi + ((long long)wrap_idx(i + 1, a.nx) * a.ny + j) * nz;
-    const FF_GLB double* ym1 = phi + ((long long)i * a.ny + wrap_idx(j - 1, a.ny)) * nz;
-    const FF_GLB double* yp1 = phi + ((long long)i * a.ny + wrap_idx(j + 1, a.ny)) * nz;
-    const FF_GLB double *xm2 = xm1, *xp2 = xp1, *ym2 = ym1, *yp2 = yp1;
-    if constexpr (ORDER == 5) {
-      xm2 = phi + ((long long)wrap_idx(i - 2, a.nx) * a.ny + j) * nz;
-      xp2 = phi + ((long long)wrap_idx(i + 2, a.nx) * a.ny + j) * nz;
-      ym2 = phi + ((long long)i * a.ny + wrap_idx(j - 2, a.ny)) * nz;
-      yp2 = phi + ((long long)i * a.ny + wrap_idx(j + 2, a.ny)) * nz;
+  const int me = (jl + H) * W + (kl + H);
+  for (int i = i0; i < i1; ++i) {
+    q[2 * H] = phi[(long long)wrap_idx(i + H, a.nx) * plane + own];
+    double* buf = lds + (i & 1) * tile_cells;
+    const long long pb = (long long)i * plane;
+    buf[me] = q[H];
+    for (int h = 0; h < nh; ++h) buf[hidx[h]] = phi[pb + hoff[h]];
+    __syncthreads();
+    const double cv = q[H];
+    double dx, dy, dz;
+    if constexpr (ORDER == 3) {
+      dx = fmadd(-2.0, cv, q[2] + q[0]);
+      dy = fmadd(-2.0, cv, buf[me + W] + buf[me - W]);
+      dz = fmadd(-2.0, cv, buf[me + 1] + buf[me - 1]);
+    } else {
+      dx = fmadd(-30.0, cv, fmadd(16.0, q[3] + q[1], -(q[4] + q[0])));
+      dy = fmadd(-30.0, cv, fmadd(16.0, buf[me + W] + buf[me - W], -(buf[me + 2 * W] + buf[me - 2 * W])));
+      dz = fmadd(-30.0, cv, fmadd(16.0, buf[me + 1] + buf[me - 1], -(buf[me + 2] + buf[me - 2])));
     }
-    for (int k = k0; k < a.nz; k += tpr) {
-      const double c = c0[k];
-      const int km1 = wrap_idx(k - 1, a.nz), kp1 = wrap_idx(k + 1, a.nz);
-      double dx, dy, dz;
-      if constexpr (ORDER == 3) {
-        dx = fmadd(-2.0, c, xp1[k] + xm1[k]);
-        dy = fmadd(-2.0, c, yp1[k] + ym1[k]);
-        dz = fmadd(-2.0, c, c0[kp1] + c0[km1]);
-      } else {
-        const int km2 = wrap_idx(k - 2, a.nz), kp2 = wrap_idx(k + 2, a.nz);
-        dx = fmadd(-30.0, c, fmadd(16.0, xp1[k] + xm1[k], -(xp2[k] + xm2[k])));
-        dy = fmadd(-30.0, c, fmadd(16.0, yp1[k] + ym1[k], -(yp2[k] + ym2[k])));
-        dz = fmadd(-30.0, c, fmadd(16.0, c0[kp1] + c0[km1], -(c0[kp2] + c0[km2])));
-      }
-      const double lap = fmadd(a.cz, dz, fmadd(a.cy, dy, a.cx * dx));  // (x + y) + z
+    const double lap = fmadd(a.cz, dz, fmadd(a.cy, dy, a.cx * dx));  // (x + y) + z
+    if (valid) {
+      const long long o = pb + own;
       if constexpr (RESID) {
-        const double fv = glb<const double>(a.f)[row * nz + k];
-        glb<float>(a.out)[row * nz + k] = float((fv - meanf) - lap);
-        acc += c;
+        const double fv = glb<const double>(a.f)[o];
+        glb<float>(a.out)[o] = float((fv - meanf) - lap);
+        acc += cv;
       } else {
-        glb<double>(a.out)[row * nz + k] = lap;
+        glb<double>(a.out)[o] = lap;
       }
     }
+#pragma unroll
+    for (int m = 0; m < 2 * H; ++m) q[m] = q[m + 1];
   }
   if constexpr (RESID) {
     red[tid] = acc;
@@ -134,17 +168,23 @@ ff_status run_stencil(const ff_plan* p, const double* phi, const double* f, void
   a.nx = p->n[0];
   a.ny = p->n[1];
   a.nz = p->n[2];
-  int tpr = 1;
-  while (tpr < a.nz && tpr < kStencilThreads) tpr *= 2;
-  a.lgtpr = 0;
-  while ((1 << a.lgtpr) < tpr) a.lgtpr++;
-  // about 8 points per thread
-  const long long per = std::max(1LL, 8LL * kStencilThreads / a.nz);
-  a.rows = (int)std::min<long long>(per, 1 << 16);
-  const long long nrows = (long long)a.nx * a.ny;
-  const long long nb = (nrows + a.rows - 1) / a.rows;
+  // z positions per tile: the power of two covering nz, 8 .. 64
+  int tk = 8;
+  while (tk < a.nz && tk < 64) tk *= 2;
+  a.lgtk = 0;
+  while ((1 << a.lgtk) < tk) a.lgtk++;
+  const int tj = kStencilThreads / tk;
+  a.tiles_j = (a.ny + tj - 1) / tj;
+  a.tiles_k = (a.nz + tk - 1) / tk;
+  const long long tiles = (long long)a.tiles_j * a.tiles_k;
+  // split the x march into chunks until about 2048 workgroups run
+  long long chunks = std::max(1LL, std::min<long long>(a.nx, (2048 + tiles - 1) / tiles));
+  a.chunk = (int)((a.nx + chunks - 1) / chunks);
+  a.chunks = (a.nx + a.chunk - 1) / a.chunk;
+  const long long nb = tiles * a.chunks;
   if (nb > INT_MAX) return set_error(FF_ERR_UNSUPPORTED, "stencil: grid too large");
-  a.nblocks = (int)nb;
+  const int H = p->sym == 3 ? 1 : 2;
+  const size_t lds = 2 * (size_t)(tj + 2 * H) * (tk + 2 * H) * sizeof(double);
   const double c12 = p->sym == 5 ? 12.0 : 1.0;
   a.cx = p->fd[0] / c12;
   a.cy = p->fd[1] / c12;
@@ -156,9 +196,11 @@ ff_status run_stencil(const ff_plan* p, const double* phi, const double* f, void
   const void* fn;
   if (r_f64) fn = p->sym == 3 ? (const void*)&k_stencil<3, false> : (const void*)&k_stencil<5, false>;
   else fn = p->sym == 3 ? (const void*)&k_stencil<3, true> : (const void*)&k_stencil<5, true>;
-  ff_status st = hip_check(hipLaunchKernel(fn, dim3(a.nblocks), dim3(kStencilThreads), argv, 0, s), "stencil launch");
+  ff_status st =
+      hip_check(hipLaunchKernel(fn, dim3((unsigned)nb), dim3(kStencilThreads), argv, lds, s), "stencil launch");
   if (st != FF_OK || r_f64) return st;
-  const void* sargv[] = {&partial, &a.nblocks, &sum_phi};
+  int n = (int)nb;
+  const void* sargv[] = {&partial, &n, &sum_phi};
   return hip_check(hipLaunchKernel((const void*)&k_stencil_sum, dim3(1), dim3(kStencilThreads),
                                    const_cast<void**>(sargv), 0, s),
                    "stencil sum launch");

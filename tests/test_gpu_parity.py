@@ -422,6 +422,37 @@ def test_large_benchmark_norms(n):
     torch.cuda.empty_cache()
 
 
+def test_solve_beyond_2_31_points():
+    """1296^3 = 2,176,782,336 points, more than 2^31 (the reference indexes in
+    32-bit int and asserts its arrays stay below INT_MAX, ParisPencil.cpp:74):
+    every pass indexes with 64-bit offsets.  Checked on the whole array by the
+    analytic norms (the last planes included), and bitwise by exact scaling:
+    doubling f doubles every operation of the solve exactly, so
+    solve(2 f) = 2 solve(f) in every element."""
+    n = (1296, 1296, 1296)
+    N = n[0] * n[1] * n[2]
+    assert N > 2 ** 31
+    say("beyond 2^31", n)
+    plan, nb, a, b = make(n)
+    field = ff.field_default(n)
+    ff.field_init(field, a)
+    plan.solve(nb, a, b)
+    l1, l2, linf = ff.field_norms(field, b)
+    say("  norms", l1, l2, linf)
+    assert linf < 4e-15 and l2 < 4e-15 and l1 < 4e-15
+    phi1 = b[:N].clone()
+    ff.field_init(field, a)  # the solve clobbered f
+    a[:N].mul_(2.0)
+    plan.solve(nb, a, b)
+    assert torch.equal(b[:N], 2.0 * phi1)
+    # the last four planes lie past element 2^31 and are not zero
+    tail = b[N - 4 * n[1] * n[2]:N]
+    assert tail.abs().max().item() > 0.1
+    del phi1, tail
+    plan.close()
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("n", [(64, 64, 64), (48, 40, 36), (22, 14, 26)])
 def test_fp32_refinement_converges(n):
     say("refine", n)
