@@ -1,7 +1,7 @@
 """Python mirror of the reference's benchmark driver FishFry<P>
-(FishFry.hpp:103-246) over the C-ABI, used by tests and bench.py.  The C++
-driver (fishfry_amd/host/FishFry.hpp, binary ``fishfry``) is the drop-in
-equivalent of the reference's ``main``.
+(FishFry.hpp:103-246) over the C-ABI: the same header, per-run lines and
+phase table as the C++ driver (fishfry_amd/host/FishFry.hpp, binary
+``fishfry``), which is the drop-in equivalent of the reference's ``main``.
 """
 from __future__ import annotations
 

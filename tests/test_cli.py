@@ -75,3 +75,30 @@ def test_driver_output_and_norms(exe, args):
     assert [row.split("| ")[-1] for row in table] == PHASES
     assert "(100.0%) | Total" in table[0]
     assert any(l.startswith("Solve latency p50") for l in lines)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,iters,precision,steps", [((32, 32, 32), 2, 0, 0), ((24, 20, 30), 1, 1, 2)])
+def test_python_driver_matches_cli_format(n, iters, precision, steps):
+    """fishfry_amd.FishFry, the Python mirror of FishFry<ParisPencil>, prints
+    the same header, per-run lines and phase table as the C++ driver and the
+    reference (FishFry.hpp:198-227), with the analytic norms at round-off."""
+    import fishfry_amd as ff
+
+    out = []
+    drv = ff.FishFry((1, 1, 1), n, precision=precision, refine_steps=steps, printer=out.append)
+    res = drv.run(iters)
+    drv.close()
+    text = "\n".join(out)
+    print(text, flush=True)
+    lines, runs, table = parse(text)
+    assert lines[0] == f"FishFry<ParisPencil>: {n[0]}x{n[1]}x{n[2]} points over 1x1x1 MPI processes"
+    assert [int(m.group(1)) for m in runs] == list(range(iters + 1))
+    assert runs[0].group(7) and not any(m.group(7) for m in runs[1:])
+    assert len(res) == iters + 1 and res[0]["warmup"] and not any(r["warmup"] for r in res[1:])
+    for m, r in zip(runs, res):
+        assert abs(float(m.group(3)) / r["rate"] - 1) < 1e-3
+        assert max(r["l1"], r["l2"], r["linf"]) < 1e-12
+    assert [row.split("| ")[-1] for row in table] == PHASES
+    with pytest.raises(ff.FishFryError):
+        ff.FishFry((1, 2, 1), n)
