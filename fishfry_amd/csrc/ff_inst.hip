@@ -77,8 +77,12 @@ static const void* col_static_fn(int f32, int split, int mode, int n, int T) {
     constexpr StaticCol sc = kStaticCols[K];
     if constexpr (sc.E == FF_E) {
       if (sc.n == n && sc.T == T && sc.f32 == f32 && sc.split == split) {
-        if constexpr (sc.f32) return col_static_modes<float, sc.split != 0, sc.n, sc.T, sc.waves, sc.kinds>(mode);
-        else return col_static_modes<double, sc.split != 0, sc.n, sc.T, sc.waves, sc.kinds>(mode);
+        // entries may serve different passes of one shape: keep looking when
+        // this one has no kernel for the mode
+        const void* fn;
+        if constexpr (sc.f32) fn = col_static_modes<float, sc.split != 0, sc.n, sc.T, sc.waves, sc.kinds>(mode);
+        else fn = col_static_modes<double, sc.split != 0, sc.n, sc.T, sc.waves, sc.kinds>(mode);
+        if (fn) return fn;
       }
     }
     return col_static_fn<K + 1>(f32, split, mode, n, T);

@@ -626,7 +626,10 @@ struct Launch {
 };
 
 // Column kernels compiled with the line plan of (n, E) and the tile width T
-// fixed (LDS pad shift 4, one group).  At 1024 (E = 16, the tile of the
+// fixed (LDS pad shift 4, one group).  The FP64 x pass at 1024 runs the split
+// exchange (real and imaginary parts through half the LDS, 33 KB): at 166
+// VGPRs three workgroups then fit a CU instead of two, and the pass takes
+// 4.84 ms instead of 5.14 ms on MI355X (the y passes gain nothing from it).  At 1024 (E = 16, the tile of the
 // default configuration) the folded indices save 15-20 % of the x pass and a
 // few percent of the y passes on MI355X.  Measured and not kept: 512 (E = 8,
 // three waves per SIMD: the solve as fast as with the generic kernels), other
@@ -638,7 +641,7 @@ struct StaticCol {
   int waves;  // waves per SIMD the registers must allow
   int kinds;  // passes it serves, as 1 << kind: 2 the y pass (forward, inverse), 4 the x pass (filter, symbol)
 };
-constexpr StaticCol kStaticCols[] = {{1024, 16, 4, 0, 0, 2, 6}, {1024, 16, 8, 1, 0, 2, 6},
+constexpr StaticCol kStaticCols[] = {{1024, 16, 4, 0, 0, 2, 6}, {1024, 16, 8, 1, 0, 2, 6}, {1024, 16, 4, 0, 1, 3, 4},
                                      // x passes only at 512, 768 and 1000 (the y passes are as fast or
                                      // faster generic there); both column passes at 256 and below
                                      {768, 12, 4, 0, 0, 3, 4}, {1000, 20, 4, 0, 0, 2, 4},

@@ -507,23 +507,25 @@ bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T, int flags) 
 // fewest-passes heuristic picks a slower elements-per-thread value (the
 // 768^3 and 1000^3 grids); user overrides (ff_tune_set) take precedence.
 struct BuiltinCfg {
-  int kind, n, E, T;
+  int kind, n, E, T, flags;
 };
 static const BuiltinCfg kBuiltin[] = {
-    {0, 384, 12, 4},  // z forward, nz = 768
-    {3, 384, 12, 4},  // z inverse, nz = 768
-    {1, 768, 24, 4},  // y, ny = 768
-    {2, 768, 12, 4},  // x, nx = 768
-    {1, 1000, 20, 4}, // y, ny = 1000
-    {2, 1000, 20, 4}, // x, nx = 1000 (compiled-in plan)
+    {0, 384, 12, 4, 0},   // z forward, nz = 768
+    {3, 384, 12, 4, 0},   // z inverse, nz = 768
+    {1, 768, 24, 4, 0},   // y, ny = 768
+    {2, 768, 12, 4, 0},   // x, nx = 768
+    {1, 1000, 20, 4, 0},  // y, ny = 1000
+    {2, 1000, 20, 4, 0},  // x, nx = 1000 (compiled-in plan)
+    {2, 1024, 16, 4, FF_TUNE_SPLIT_LDS},  // x, nx = 1024: split exchange, three workgroups per CU
 };
 
-static bool builtin_lookup(int kind, int n, int prec, int& E, int& T) {
+static bool builtin_lookup(int kind, int n, int prec, int& E, int& T, int& flags) {
   if (prec != 0) return false;
   for (const BuiltinCfg& b : kBuiltin)
     if (b.kind == kind && b.n == n) {
       E = b.E;
       T = b.T;
+      flags = b.flags;
       return true;
     }
   return false;
@@ -533,7 +535,8 @@ bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int kind, int p
   int E = 0, T = 0, flags = 0;
   if (tune_lookup(kind, n, prec, E, T, flags) && col_cfg_with(n, ncols, elem_bytes, out, E, T, flags, kind))
     return true;
-  if (builtin_lookup(kind, n, prec, E, T) && col_cfg_with(n, ncols, elem_bytes, out, E, T, 0, kind)) return true;
+  if (builtin_lookup(kind, n, prec, E, T, flags) && col_cfg_with(n, ncols, elem_bytes, out, E, T, flags, kind))
+    return true;
   return col_cfg_with(n, ncols, elem_bytes, out, 0, 0, 0, kind);
 }
 
@@ -541,7 +544,7 @@ bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int kind, int prec) {
   const int M = (nz % 2 == 0) ? nz / 2 : nz;
   int E = 0, T = 0, flags = 0;
   if (tune_lookup(kind, M, prec, E, T, flags) && row_cfg_with(nz, elem_bytes, out, E, T, flags)) return true;
-  if (builtin_lookup(kind, M, prec, E, T) && row_cfg_with(nz, elem_bytes, out, E, T, 0)) return true;
+  if (builtin_lookup(kind, M, prec, E, T, flags) && row_cfg_with(nz, elem_bytes, out, E, T, flags)) return true;
   return row_cfg_with(nz, elem_bytes, out, 0, 0, 0);
 }
 

@@ -129,8 +129,9 @@ def test_tuning_host_logic_without_gpu_work():
     ff.tune_reset()
     try:
         for kind in (1, 2):
+            # the FP64 x pass at 1024 runs the split exchange by default (three workgroups per CU)
             t = ff.tune_query(kind, 1024, 0, 513)
-            assert (t.E, t.T, t.flags) == (16, 4, 0)
+            assert (t.E, t.T, t.flags) == (16, 4, ff.FF_TUNE_SPLIT_LDS if kind == 2 else 0)
             t = ff.tune_query(kind, 1024, 1, 513)
             assert (t.E, t.T, t.flags) == (16, 8, 0)
         assert ff.tune_query(1, 512, 0, 257).flags & ff.FF_TUNE_GENERIC
@@ -154,14 +155,16 @@ def test_tuning_host_logic_without_gpu_work():
         ff.tune_set(2, 1024, 0, 16, 4, ff.FF_TUNE_GENERIC)
         assert ff.tune_query(2, 1024, 0, 513).flags == ff.FF_TUNE_GENERIC
         assert ff.tune_query(1, 1024, 0, 513).flags == 0
-        ff.tune_set(2, 1024, 0, 16, 4, ff.FF_TUNE_SPLIT_LDS)
-        assert ff.tune_query(2, 1024, 0, 513).flags == ff.FF_TUNE_SPLIT_LDS | ff.FF_TUNE_GENERIC
+        ff.tune_set(1, 1024, 0, 16, 4, ff.FF_TUNE_SPLIT_LDS)  # no compiled-in split kernel for the y pass
+        assert ff.tune_query(1, 1024, 0, 513).flags == ff.FF_TUNE_SPLIT_LDS | ff.FF_TUNE_GENERIC
+        ff.tune_set(2, 1024, 0, 16, 4, 0)  # the non-split x kernel on request
+        assert ff.tune_query(2, 1024, 0, 513).flags == 0
         with pytest.raises(ff.FishFryError):
             ff.tune_set(2, 1024, 0, 16, 4, 8)  # unknown flag
         with pytest.raises(ff.FishFryError):
             ff.tune_set(0, 512, 0, 8, 2, ff.FF_TUNE_SPLIT_LDS)  # row passes take FF_TUNE_GENERIC only
         ff.tune_reset()
-        assert ff.tune_query(2, 1024, 0, 513).flags == 0
+        assert ff.tune_query(2, 1024, 0, 513).flags == ff.FF_TUNE_SPLIT_LDS
         # row passes: compiled-in plans at the default configurations of even
         # nz = 32 ... 1024 (length nz/2), generic otherwise or on request
         for kind in (0, 3):

@@ -78,7 +78,7 @@ def test_driver_output_and_norms(exe, args):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,iters,precision,steps", [((32, 32, 32), 2, 0, 0), ((24, 20, 30), 1, 1, 2)])
+@pytest.mark.parametrize("n,iters,precision,steps", [((32, 32, 32), 2, 0, 0), ((40, 32, 50), 1, 1, 2)])
 def test_python_driver_matches_cli_format(n, iters, precision, steps):
     """fishfry_amd.FishFry, the Python mirror of FishFry<ParisPencil>, prints
     the same header, per-run lines and phase table as the C++ driver and the
