@@ -1,6 +1,7 @@
-// Host side of the batched solve (see ff_batch.hpp): op chains per instance,
-// tile shapes, resident twiddle tables, the persistent launch (optionally
-// captured in a HIP graph) and the autotuner.
+// The batched solve: many independent grids of uneven shapes, each with its
+// own plan, launched stream-parallel, grouped by shape, or step by step
+// through the multi-shape kernels (ff_multi.hpp), optionally captured in a
+// HIP graph, and the autotuner.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -15,383 +16,9 @@
 #include <tuple>
 #include <vector>
 
-#include "ff_batch.hpp"
 #include "ff_internal.hpp"
 #include "ff_multi.hpp"
 #include "ff_solver.hpp"
-
-namespace ff {
-hipError_t batch_launch(const BatchDev& d, int grid, hipStream_t s);
-hipError_t batch_reset(unsigned int* done, int nops, unsigned long long* head, int nheads, unsigned int* err,
-                       hipStream_t s);
-hipError_t batch_occupancy(int lds_bytes, int* blocks_per_cu);
-hipError_t batch_prepare(int lds_bytes);
-
-namespace {
-
-// Radix sequence for the LDS engine.  strategy 0: fewest passes (largest
-// radices first); strategy 1: prefer radix 8 and 4 (shorter butterflies).
-bool engine_radices(int n, int strategy, std::vector<int>& seq) {
-  static const int big[] = {16, 12, 10, 8, 6, 5, 4, 3, 2};
-  static const int mid[] = {8, 4, 6, 5, 3, 2, 16, 12, 10};
-  seq.clear();
-  if (strategy == 0) {
-    std::vector<int> best(n + 1, 1 << 20), pick(n + 1, 0);
-    best[1] = 0;
-    for (int m = 2; m <= n; ++m) {
-      if (n % m) continue;
-      for (int R : big) {
-        if (m % R) continue;
-        if (best[m / R] + 1 < best[m]) {
-          best[m] = best[m / R] + 1;
-          pick[m] = R;
-        }
-      }
-    }
-    if (best[n] >= (1 << 20)) return false;
-    for (int m = n; m > 1; m /= pick[m]) seq.push_back(pick[m]);
-    std::sort(seq.begin(), seq.end(), [](int a, int b) { return a > b; });
-  } else {
-    int m = n;
-    while (m > 1) {
-      int R = 0;
-      for (int r : mid)
-        if (m % r == 0) {
-          R = r;
-          break;
-        }
-      if (!R) return false;
-      seq.push_back(R);
-      m /= R;
-    }
-  }
-  return (int)seq.size() <= FF_MAXP;
-}
-
-uint32_t inv32(int d) { return d > 1 ? (uint32_t)(((1ull << 32) + (uint64_t)d - 1) / (uint64_t)d) : 0u; }
-
-bool make_bline(int n, int strategy, BLine& lp) {
-  std::vector<int> seq;
-  memset(&lp, 0, sizeof lp);
-  lp.n = n;
-  if (n == 1) return true;
-  if (!engine_radices(n, strategy, seq)) return false;
-  lp.np = (int)seq.size();
-  int ns = 1;
-  for (int p = 0; p < lp.np; ++p) {
-    const int R = seq[p];
-    lp.radix[p] = R;
-    lp.nr[p] = n / R;
-    lp.nrinv[p] = inv32(n / R);
-    lp.ns[p] = ns;
-    lp.nsinv[p] = inv32(ns);
-    lp.twstep[p] = n / (ns * R);
-    ns *= R;
-  }
-  return true;
-}
-
-int floor_pow2(int v) {
-  int p = 1;
-  while (p * 2 <= v) p *= 2;
-  return p;
-}
-
-double dd_const(int n, double lo, double hi) { return 2.0 * M_PI * double(n - 1) / (double(n) * (hi - lo)); }
-
-size_t align256(size_t v) { return (v + 255) / 256 * 256; }
-
-}  // namespace
-
-const void* twiddle_table(int n, int prec) { return table_twiddle(n, prec); }
-const void* twiddle_post(int nz, int prec) { return table_post(nz, prec); }
-
-size_t batch_item_bytes(const ff_batch_item& it) {
-  const int ones[3] = {1, 1, 1};
-  return reference_bytes(it.n, ones);
-}
-
-size_t batch_scratch_bytes(const ff_batch_item& it) {
-  if (it.precision != FF_FP32_REFINE) return 0;
-  const size_t N = (size_t)it.n[0] * it.n[1] * it.n[2];
-  const size_t Nh = (size_t)it.n[0] * it.n[1] * (it.n[2] / 2 + 1);
-  return align256(Nh * 16) + align256(N * 4) + 256;
-}
-
-// ---- BatchCore -----------------------------------------------------------------
-
-ff_status BatchCore::init(int cnt, const ff_batch_item* its, int flg) {
-  if (cnt < 1 || !its) return set_error(FF_ERR_INVALID, "batch: need at least one item");
-  if (flg & ~(FF_BATCH_GRAPH | kBatchPerStep)) return set_error(FF_ERR_INVALID, "batch: bad flags");
-  count = cnt;
-  flags = flg;
-  items.assign(its, its + cnt);
-  N.clear();
-  Nh.clear();
-  bytes.clear();
-  scratch_off.clear();
-  scratch_bytes = 0;
-  for (int i = 0; i < cnt; ++i) {
-    const ff_batch_item& it = its[i];
-    for (int a = 0; a < 3; ++a)
-      if (it.n[a] < 2 || !(it.hi[a] > it.lo[a]))
-        return set_error(FF_ERR_INVALID, "batch: extents must be >= 2 with hi > lo");
-    if ((it.precision != FF_FP64 && it.precision != FF_FP32_REFINE) || it.refine_steps < 0 || it.refine_steps > 8)
-      return set_error(FF_ERR_INVALID, "batch: bad precision / refine_steps");
-    const int nz = it.n[2];
-    const int M = nz % 2 ? nz : nz / 2;
-    BLine tmp;
-    for (int n : {it.n[0], it.n[1], M}) {
-      if (n > 4096 || !make_bline(n, 0, tmp)) {
-        char buf[160];
-        snprintf(buf, sizeof buf, "batch: item %d (%dx%dx%d) has an extent with a prime factor above 5", i, it.n[0],
-                 it.n[1], it.n[2]);
-        return set_error(FF_ERR_UNSUPPORTED, buf);
-      }
-    }
-    const int nprec = it.precision == FF_FP64 ? 1 : 2;
-    for (int prec = 0; prec < nprec; ++prec) {
-      for (int n : {it.n[0], it.n[1], M})
-        if (!twiddle_table(n, prec)) return set_error(FF_ERR_NOMEM, "batch: twiddle table allocation failed");
-      if (nz % 2 == 0 && !twiddle_post(nz, prec))
-        return set_error(FF_ERR_NOMEM, "batch: twiddle table allocation failed");
-    }
-    bytes.push_back(batch_item_bytes(it));
-    N.push_back((long long)it.n[0] * it.n[1] * it.n[2]);
-    Nh.push_back((long long)it.n[0] * it.n[1] * (it.n[2] / 2 + 1));
-    scratch_off.push_back(scratch_bytes);
-    scratch_bytes += batch_scratch_bytes(it);
-  }
-  return FF_OK;
-}
-
-namespace {
-
-bool size_op(const BatchCore& b, BOp& o) {
-  const int eb = o.prec ? 8 : 16;
-  const int budget = b.budget_kib * 1024 / (2 * eb);  // complex elements per buffer
-  if (o.kind == BOP_COL) {
-    o.pitch = o.line.n + 1;
-    int T = floor_pow2(std::max(1, budget / o.pitch));
-    int cap = 1;
-    while (cap < o.nh) cap *= 2;
-    T = std::min(T, cap);
-    o.T = T;
-    o.lgT = 0;
-    while ((1 << o.lgT) < T) o.lgT++;
-    o.tiles_pp = (o.nh + T - 1) / T;
-    o.ntiles = (int)(o.nlines * o.tiles_pp);
-  } else {
-    o.pitch = o.line.n + 2;
-    o.T = std::max(1, std::min(budget / o.pitch, 64));
-    o.lgT = 0;
-    o.ntiles = (int)((o.nlines + o.T - 1) / o.T);
-  }
-  return (size_t)2 * o.T * o.pitch * eb <= 160 * 1024;
-}
-
-ff_status build_chain(const BatchCore& b, int i, double* density, double* potential, void* scratch,
-                      std::vector<BOp>& ops) {
-  const ff_batch_item& it = b.items[i];
-  const int nx = it.n[0], ny = it.n[1], nz = it.n[2];
-  const int nh = nz / 2 + 1;
-  const int odd = nz % 2;
-  const int M = odd ? nz : nz / 2;
-  BOp base;
-  memset(&base, 0, sizeof base);
-  base.nx = nx;
-  base.ny = ny;
-  base.nz = nz;
-  base.nh = nh;
-  base.odd = odd;
-  base.scale = 1.0 / (double(nx) * double(ny) * double(nz));  // ParisPencil.cpp:408
-  base.ddi = dd_const(nx, it.lo[0], it.hi[0]);               // ParisPencil.cpp:9-11
-  base.ddj = dd_const(ny, it.lo[1], it.hi[1]);
-  base.ddk = dd_const(nz, it.lo[2], it.hi[2]);
-  base.dep = -1;
-  auto zf = [&](int prec, int io32, const void* src, void* dst) {
-    BOp o = base;
-    o.kind = BOP_ZF;
-    o.prec = prec;
-    o.io32 = io32;
-    o.nlines = (long long)nx * ny;
-    make_bline(M, b.strategy, o.line);
-    o.tw = twiddle_table(M, prec);
-    o.twpost = odd ? nullptr : twiddle_post(nz, prec);
-    o.src = src;
-    o.dst = dst;
-    return o;
-  };
-  auto col = [&](int prec, int axis, int mode, const void* src, void* dst, double* dc) {
-    BOp o = base;
-    o.kind = BOP_COL;
-    o.prec = prec;
-    o.mode = mode;
-    const int n = axis == 1 ? ny : nx;
-    make_bline(n, b.strategy, o.line);
-    o.tw = twiddle_table(n, prec);
-    if (axis == 1) {  // planes i, positions j
-      o.nlines = nx;
-      o.s_outer = (long long)ny * nh;
-      o.s_r = nh;
-    } else {  // planes j, positions i
-      o.nlines = ny;
-      o.s_outer = nh;
-      o.s_r = (long long)ny * nh;
-    }
-    o.src = src;
-    o.dst = dst;
-    o.dc_out = dc;
-    return o;
-  };
-  auto zi = [&](int prec, int io32, int epi, const void* src, void* dst, const double* aux, const double* shiftp) {
-    BOp o = base;
-    o.kind = BOP_ZI;
-    o.prec = prec;
-    o.io32 = io32;
-    o.mode = epi;
-    o.nlines = (long long)nx * ny;
-    make_bline(M, b.strategy, o.line);
-    o.tw = twiddle_table(M, prec);
-    o.twpost = odd ? nullptr : twiddle_post(nz, prec);
-    o.src = src;
-    o.dst = dst;
-    o.aux = aux;
-    o.shiftp = shiftp;
-    return o;
-  };
-  ops.clear();
-  if (it.precision == FF_FP64) {
-    // ping-pong density <-> potential as in the single-grid solve
-    ops.push_back(zf(0, 0, density, potential));
-    ops.push_back(col(0, 1, COL_FWD, potential, density, nullptr));
-    ops.push_back(col(0, 0, COL_FILTER, density, potential, nullptr));
-    ops.push_back(col(0, 1, COL_INV, potential, density, nullptr));
-    ops.push_back(zi(0, 0, EPI_STORE, density, potential, nullptr, nullptr));
-  } else {
-    unsigned char* w = static_cast<unsigned char*>(scratch) + b.scratch_off[i];
-    void* spec = w;
-    float* res = reinterpret_cast<float*>(w + align256((size_t)b.Nh[i] * 16));
-    double* dc = reinterpret_cast<double*>(w + align256((size_t)b.Nh[i] * 16) + align256((size_t)b.N[i] * 4));
-    ops.push_back(zf(1, 0, density, spec));
-    ops.push_back(col(1, 1, COL_FWD, spec, spec, nullptr));
-    ops.push_back(col(1, 0, COL_FILTER, spec, spec, nullptr));
-    ops.push_back(col(1, 1, COL_INV, spec, spec, nullptr));
-    ops.push_back(zi(1, 0, EPI_STORE, spec, potential, nullptr, nullptr));
-    for (int s = 0; s < it.refine_steps; ++s) {
-      ops.push_back(zf(0, 0, potential, spec));
-      ops.push_back(col(0, 1, COL_FWD, spec, spec, nullptr));
-      ops.push_back(col(0, 0, COL_SYMBOL, spec, spec, dc));
-      ops.push_back(col(0, 1, COL_INV, spec, spec, nullptr));
-      ops.push_back(zi(0, 1, EPI_RESIDUAL, spec, res, density, nullptr));
-      ops.push_back(zf(1, 1, res, spec));
-      ops.push_back(col(1, 1, COL_FWD, spec, spec, nullptr));
-      ops.push_back(col(1, 0, COL_FILTER, spec, spec, nullptr));
-      ops.push_back(col(1, 1, COL_INV, spec, spec, nullptr));
-      ops.push_back(zi(1, 0, EPI_ACCUM, spec, potential, nullptr, dc));
-    }
-  }
-  for (BOp& o : ops) {
-    if (!o.tw || (!o.odd && o.kind != BOP_COL && !o.twpost))
-      return set_error(FF_ERR_NOMEM, "batch: twiddle table missing");
-    if (!size_op(b, o)) return set_error(FF_ERR_UNSUPPORTED, "batch: tile does not fit in LDS");
-  }
-  return FF_OK;
-}
-
-}  // namespace
-
-ff_status BatchCore::build(double* const* density, double* const* potential, void* scratch) {
-  std::vector<std::vector<BOp>> chains(count);
-  size_t maxlen = 0;
-  for (int i = 0; i < count; ++i) {
-    ff_status st = build_chain(*this, i, density[i], potential[i], scratch, chains[i]);
-    if (st != FF_OK) return st;
-    maxlen = std::max(maxlen, chains[i].size());
-  }
-  // step-major order; each op waits for the same instance's previous op
-  ops.clear();
-  item_start.assign(1, 0);
-  step_start.assign(1, 0);
-  std::vector<int> last(count, -1);
-  lds_bytes = 0;
-  for (size_t s = 0; s < maxlen; ++s) {
-    for (int i = 0; i < count; ++i) {
-      if (s >= chains[i].size()) continue;
-      BOp o = chains[i][s];
-      o.dep = last[i];
-      last[i] = (int)ops.size();
-      ops.push_back(o);
-      item_start.push_back(item_start.back() + o.ntiles);
-      const int eb = o.prec ? 8 : 16;
-      lds_bytes = std::max(lds_bytes, 2 * o.T * o.pitch * eb);
-    }
-    step_start.push_back(item_start.back());
-  }
-  nsteps = (int)maxlen;
-  const int nops = (int)ops.size();
-  // device block: ops | item_start | done | heads | err
-  o_is = align256(sizeof(BOp) * nops);
-  o_done = align256(o_is + sizeof(long long) * (nops + 1));
-  o_head = align256(o_done + sizeof(unsigned) * nops);
-  o_err = align256(o_head + sizeof(unsigned long long) * (nsteps + 1));
-  desc_bytes = o_err + 256;
-  image.assign(o_done, 0);
-  memcpy(&image[0], ops.data(), sizeof(BOp) * nops);
-  memcpy(&image[o_is], item_start.data(), sizeof(long long) * (nops + 1));
-  int per_cu = 0, dev_id = 0, ncu = 0;
-  ff_status st = hip_check(batch_prepare(lds_bytes), "batch kernel LDS attribute");
-  if (st != FF_OK) return st;
-  st = hip_check(batch_occupancy(lds_bytes, &per_cu), "batch occupancy");
-  if (st != FF_OK) return st;
-  (void)hipGetDevice(&dev_id);
-  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_id);
-  const long long nitems = item_start.back();
-  grid = (int)std::max(1LL, std::min<long long>(nitems, (long long)std::max(1, per_cu) * std::max(1, ncu)));
-  return FF_OK;
-}
-
-ff_status BatchCore::upload(void* d, hipStream_t s) {
-  dev = d;
-  return hip_check(hipMemcpyAsync(d, image.data(), image.size(), hipMemcpyHostToDevice, s), "batch upload");
-}
-
-BatchDev BatchCore::view() const {
-  unsigned char* base = static_cast<unsigned char*>(dev);
-  BatchDev d;
-  d.ops = reinterpret_cast<const BOp*>(base);
-  d.item_start = reinterpret_cast<const long long*>(base + o_is);
-  d.nops = (int)ops.size();
-  d.done = reinterpret_cast<unsigned*>(base + o_done);
-  d.head = reinterpret_cast<unsigned long long*>(base + o_head);
-  d.err = reinterpret_cast<unsigned*>(base + o_err);
-  d.item_begin = 0;
-  d.item_end = item_start.back();
-  d.spin_limit = 200000000LL;  // 2 s
-  d.lds_bytes = lds_bytes;
-  return d;
-}
-
-ff_status BatchCore::enqueue(hipStream_t s) const {
-  const BatchDev d = view();
-  ff_status st = hip_check(batch_reset(d.done, d.nops, d.head, nsteps + 1, d.err, s), "batch reset launch");
-  if (st != FF_OK) return st;
-  if (flags & kBatchPerStep) {
-    for (int k = 0; k < nsteps; ++k) {
-      BatchDev dk = d;
-      dk.head = d.head + 1 + k;
-      dk.item_begin = step_start[k];
-      dk.item_end = step_start[k + 1];
-      st = hip_check(batch_launch(dk, (int)std::min<long long>(dk.item_end - dk.item_begin, grid), s),
-                     "batch step launch");
-      if (st != FF_OK) return st;
-    }
-    return FF_OK;
-  }
-  return hip_check(batch_launch(d, grid, s), "batch launch");
-}
-
-}  // namespace ff
 
 using namespace ff;
 
@@ -410,9 +37,11 @@ using namespace ff;
 // launchers' recorder), so a group runs exactly the kernels and arithmetic of
 // its instances' single solves.
 //
-// The single persistent launch over every instance (BatchCore, k_batch) is
-// switched off: its kernel did not complete on MI355X and the cause is not
-// established, so FF_BATCH_PERSISTENT is rejected.
+// FF_BATCH_PERSISTENT, a single persistent launch whose workgroups waited on
+// each other's completion counters, is rejected: its kernel did not complete
+// on MI355X, the cause was never established, and it was removed.  The
+// step-batched mode below reaches one launch per pass step with no
+// workgroup waiting on another.
 //
 // With FF_BATCH_STEPS, pass step s of every instance of a family (instances
 // whose solves launch the same kernel roles step by step: same precision and
@@ -939,8 +568,8 @@ ff_status ff_batch_create(int count, const ff_batch_item* items, int flags, ff_b
   if (count < 1 || !items) return set_error(FF_ERR_INVALID, "ff_batch_create: need at least one item");
   if (flags & FF_BATCH_PERSISTENT)
     return set_error(FF_ERR_UNSUPPORTED,
-                     "ff_batch_create: the single persistent launch is disabled in this build (its kernel did not "
-                     "complete on MI355X); use the default stream-parallel batch");
+                     "ff_batch_create: the single persistent launch is not available (its kernel did not "
+                     "complete on MI355X and was removed); FF_BATCH_STEPS runs one launch per pass step");
   if (flags & ~(FF_BATCH_GRAPH | FF_BATCH_GROUPED | FF_BATCH_STEPS))
     return set_error(FF_ERR_INVALID, "ff_batch_create: bad flags");
   if ((flags & FF_BATCH_GROUPED) && (flags & FF_BATCH_STEPS))

@@ -218,8 +218,9 @@ typedef struct ff_batch_item {
 
 #define FF_BATCH_GRAPH 1      /* capture the fork/join launch sequence in a HIP graph on
                                  first solve and replay it while buffers are unchanged */
-#define FF_BATCH_PERSISTENT 4 /* one persistent kernel over every instance: disabled in
-                                 this build, rejected with FF_ERR_UNSUPPORTED           */
+#define FF_BATCH_PERSISTENT 4 /* one persistent kernel over every instance: removed (its
+                                 kernel did not complete on MI355X), rejected with
+                                 FF_ERR_UNSUPPORTED; FF_BATCH_STEPS replaces it         */
 #define FF_BATCH_GROUPED 8    /* instances of one shape, precision and refinement count
                                  run each pass as one launch (one grid row per instance,
                                  per-instance buffers and domains from a device table);

@@ -1,11 +1,13 @@
 """The single persistent launch over every batch instance (FF_BATCH_PERSISTENT).
 
-Its kernel (k_batch: an atomic (op, tile) queue in step-major order with
-per-op completion counters) did not complete on MI355X: the first run's
-dependency polls invalidated the XCD L2 on every poll; after polls were made
-relaxed and every wait bounded in wall time, the run still did not finish in
-120 s, so the cause is not established.  The path is switched off (rejected
-with FF_ERR_UNSUPPORTED) and these tests stay skipped until it is understood.
+Its kernel (an atomic (op, tile) queue in step-major order with per-op
+completion counters) did not complete on MI355X: the first run's dependency
+polls invalidated the XCD L2 on every poll; after polls were made relaxed and
+every wait bounded in wall time, the run still did not finish in 120 s, so the
+cause is not established.  The kernel was removed (FF_BATCH_PERSISTENT is
+rejected with FF_ERR_UNSUPPORTED); FF_BATCH_STEPS (tests/test_gpu_batch_steps.py)
+runs one launch per pass step with no workgroup waiting on another.  These
+tests stay skipped.
 """
 import numpy as np
 import pytest
@@ -13,7 +15,7 @@ import pytest
 import fishfry_amd as ff
 
 pytestmark = [pytest.mark.gpu,
-              pytest.mark.skip(reason="persistent batch kernel did not complete on MI355X (twice); path disabled")]
+              pytest.mark.skip(reason="persistent batch kernel did not complete on MI355X (twice); removed")]
 
 
 def test_persistent_batch_fp64_matches_oracle():
