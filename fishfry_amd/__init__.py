@@ -45,6 +45,7 @@ FF_SYMBOL_5PT = 16
 FF_TUNE_SPLIT_LDS = 1
 FF_TUNE_TWO_GROUPS = 2
 FF_TUNE_GENERIC = 4
+FF_TUNE_LDS_TWIDDLES = 8
 FF_SOLVE_STAMPS = 14
 FF_MAX_MODES = 32
 
@@ -336,7 +337,7 @@ class ParisPencil:
 
 
 def tune_set(kind: int, n: int, precision: int, E: int, T: int, flags: int = 0) -> None:
-    """Override E / T (and FF_TUNE_SPLIT_LDS / FF_TUNE_TWO_GROUPS for column passes, FF_TUNE_GENERIC for any) for passes of
+    """Override E / T (and FF_TUNE_SPLIT_LDS / FF_TUNE_TWO_GROUPS / FF_TUNE_LDS_TWIDDLES for column passes, FF_TUNE_GENERIC for any) for passes of
     kind (0 z forward rows, 1 y, 2 x, 3 z inverse rows) and length n."""
     t = Tune(kind, n, precision, E, T, flags)
     check(lib().ff_tune_set(ctypes.byref(t)), "tune_set")
@@ -525,5 +526,5 @@ from .driver import FishFry  # noqa: E402  (needs the names above)
 __all__ = [
     "ParisPencil", "FishFry", "Batch", "Server", "FF_BATCH_GRAPH", "FF_BATCH_PERSISTENT", "FF_BATCH_GROUPED", "FF_BATCH_STEPS", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
     "field_norms", "reference_bytes", "benchmark_grid", "FishFryError", "lib", "LIB_PATH",
-    "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_BLOCKED_LAYOUT", "FF_SLAB_PIPELINE", "FF_SYMBOL_3PT", "FF_SYMBOL_5PT", "FF_TUNE_SPLIT_LDS", "FF_TUNE_TWO_GROUPS", "FF_TUNE_GENERIC",
+    "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_BLOCKED_LAYOUT", "FF_SLAB_PIPELINE", "FF_SYMBOL_3PT", "FF_SYMBOL_5PT", "FF_TUNE_SPLIT_LDS", "FF_TUNE_TWO_GROUPS", "FF_TUNE_GENERIC", "FF_TUNE_LDS_TWIDDLES",
 ]

@@ -237,7 +237,7 @@ ff_status ff_tune_query(int kind, int n, int precision, int ncols, ff_tune* out)
     E = c.line.E;
     T = c.T;
     flags = (c.split ? FF_TUNE_SPLIT_LDS : 0) | (c.groups == 2 ? FF_TUNE_TWO_GROUPS : 0) |
-            (c.stat ? 0 : FF_TUNE_GENERIC);
+            (c.stat ? 0 : FF_TUNE_GENERIC) | (c.stat == 2 ? FF_TUNE_LDS_TWIDDLES : 0);
   }
   *out = ff_tune{kind, n, precision, E, T, flags};
   return FF_OK;

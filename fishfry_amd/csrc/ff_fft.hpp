@@ -247,10 +247,34 @@ constexpr bool radix_ok() {
                                   R == 10 || R == 12 || R == 16);
 }
 
+// Twiddle sources for the passes: tw_at(tw, t) = W[t].
+// A table in global memory:
+template <typename F>
+__device__ __forceinline__ cx<F> tw_at(const cx<F>* __restrict__ tw, int t) {
+  return tw[t];
+}
+// The first half of a table of even length 2H, held in LDS by the kernel:
+// W[t + H] = -W[t] exactly in the host's tables (a half turn adds two quarter
+// turns to the same reduced angle, so only the signs differ), so the negated
+// entry is bit-identical to the stored one.  Entry t is stored at
+// tw_swizzle(t): a permutation within blocks of 16 entries that spreads the
+// strided indices of a wave's lanes (t = m*k*twstep) over the LDS banks.
+__device__ __forceinline__ int tw_swizzle(int t) { return t ^ (((t >> 4) ^ (t >> 8)) & 15); }
+template <typename F, int H>
+struct TwHalf {
+  const cx<F>* p;
+};
+template <typename F, int H>
+__device__ __forceinline__ cx<F> tw_at(const TwHalf<F, H>& tw, int t) {
+  const bool hi = t >= H;
+  const cx<F> w = tw.p[tw_swizzle(hi ? t - H : t)];
+  return hi ? cx<F>{-w.x, -w.y} : w;
+}
+
 // Butterflies of one pass on registers (twiddles applied to the inputs).
-template <int E, int R, int DIR, typename F>
+template <int E, int R, int DIR, typename F, typename TW>
 __device__ __forceinline__ void pass_regs(cx<F> (&v)[E], int r0, int L, int ns, uint32_t nsinv, int twstep,
-                                          const cx<F>* __restrict__ tw) {
+                                          const TW& tw) {
   constexpr int Q = E / R;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -261,7 +285,7 @@ __device__ __forceinline__ void pass_regs(cx<F> (&v)[E], int r0, int L, int ns, 
       const int j = r0 + q * L;
       const int e = fast_mod(j, ns, nsinv) * twstep;
 #pragma unroll
-      for (int m = 1; m < R; ++m) a[m] = twmul<DIR>(a[m], tw[m * e]);
+      for (int m = 1; m < R; ++m) a[m] = twmul<DIR>(a[m], tw_at(tw, m * e));
     }
     Dft<R, DIR, F>::run(a);
 #pragma unroll
@@ -325,8 +349,8 @@ __device__ __forceinline__ void gather_part(cx<F> (&v)[E], const F* lds, const I
 
 // One pass with compile-time radix R: butterflies, then (unless last) the LDS
 // exchange to natural order for the next pass.
-template <int E, int R, int DIR, typename F, bool SPLIT, typename Idx>
-__device__ __forceinline__ void run_pass(cx<F> (&v)[E], const LinePlan& lp, int p, const cx<F>* __restrict__ tw,
+template <int E, int R, int DIR, typename F, bool SPLIT, typename Idx, typename TW>
+__device__ __forceinline__ void run_pass(cx<F> (&v)[E], const LinePlan& lp, int p, const TW& tw,
                                          cx<F>* lds, const Idx& idx, int r0, bool last) {
   if constexpr (radix_ok<E, R>()) {
     pass_regs<E, R, DIR, F>(v, r0, lp.L, lp.ns[p], lp.nsinv[p], lp.twstep[p], tw);
@@ -443,10 +467,10 @@ constexpr LinePlan static_plan() {
   return lp;
 }
 
-// fft_line with the plan of (N, E) fixed at compile time.
-template <int N, int E, int DIR, typename F, typename Idx, bool SPLIT, int P = 0>
-__device__ __forceinline__ void fft_line_static(cx<F> (&v)[E], const cx<F>* __restrict__ tw, cx<F>* lds,
-                                                const Idx& idx, int r0) {
+// fft_line with the plan of (N, E) fixed at compile time; TW: a twiddle
+// source (tw_at).
+template <int N, int E, int DIR, typename F, typename Idx, bool SPLIT, int P = 0, typename TW>
+__device__ __forceinline__ void fft_line_static(cx<F> (&v)[E], const TW& tw, cx<F>* lds, const Idx& idx, int r0) {
   constexpr LinePlan lp = static_plan<N, E>();
   static_assert(lp.np > 0 && N % E == 0, "no compile-time plan for this length");
   if constexpr (P < lp.np) {

@@ -45,47 +45,56 @@ static const void* col_fn(int f32, int mode) {
 // values that long column transforms use.
 constexpr bool kTwoGroups = FF_E == 8 || FF_E == 16;
 
-// Kernels with the line plan compiled in: the kStaticCols entries of this E.
-template <typename F, bool S, int N, int T, int W, int KINDS>
+// Kernels with the line plan compiled in: the kStaticCols entries of this E,
+// with twiddles from the table in memory or (TWL) from LDS.
+template <typename F, int MODE, bool S, int N, int T, int W, bool TWL>
+static const void* col_static_kernel() {
+  return (const void*)&k_col<F, FF_E, MODE, S, 1, false, N, T, W, TWL>;
+}
+
+template <typename F, bool S, int N, int T, int W, int KINDS, bool TWL>
 static const void* col_static_modes(int mode) {
-  constexpr int E = FF_E;
   constexpr bool y = (KINDS & 2) != 0, x = (KINDS & 4) != 0;
   switch (mode) {
     case COL_FWD:
-      if constexpr (y) return (const void*)&k_col<F, E, COL_FWD, S, 1, false, N, T, W>;
+      if constexpr (y) return col_static_kernel<F, COL_FWD, S, N, T, W, TWL>();
       return nullptr;
     case COL_INV:
-      if constexpr (y) return (const void*)&k_col<F, E, COL_INV, S, 1, false, N, T, W>;
+      if constexpr (y) return col_static_kernel<F, COL_INV, S, N, T, W, TWL>();
       return nullptr;
     case COL_FILTER:
-      if constexpr (x) return (const void*)&k_col<F, E, COL_FILTER, S, 1, false, N, T, W>;
+      if constexpr (x) return col_static_kernel<F, COL_FILTER, S, N, T, W, TWL>();
       return nullptr;
     case COL_FILTER_FD:
-      if constexpr (x) return (const void*)&k_col<F, E, COL_FILTER_FD, S, 1, false, N, T, W>;
+      if constexpr (x) return col_static_kernel<F, COL_FILTER_FD, S, N, T, W, TWL>();
       return nullptr;
     case COL_SYMBOL:
-      if constexpr (x && std::is_same<F, double>::value)
-        return (const void*)&k_col<F, E, COL_SYMBOL, S, 1, false, N, T, W>;
+      if constexpr (x && std::is_same<F, double>::value) return col_static_kernel<F, COL_SYMBOL, S, N, T, W, TWL>();
       return nullptr;
     default: return nullptr;
   }
 }
 
+// stat: 1 twiddles from the table in memory, 2 from LDS.
 template <int K>
-static const void* col_static_fn(int f32, int split, int mode, int n, int T) {
+static const void* col_static_fn(int f32, int split, int mode, int n, int T, int stat) {
   if constexpr (K < kNumStaticCols) {
     constexpr StaticCol sc = kStaticCols[K];
     if constexpr (sc.E == FF_E) {
       if (sc.n == n && sc.T == T && sc.f32 == f32 && sc.split == split) {
         // entries may serve different passes of one shape: keep looking when
         // this one has no kernel for the mode
-        const void* fn;
-        if constexpr (sc.f32) fn = col_static_modes<float, sc.split != 0, sc.n, sc.T, sc.waves, sc.kinds>(mode);
-        else fn = col_static_modes<double, sc.split != 0, sc.n, sc.T, sc.waves, sc.kinds>(mode);
+        using F = typename std::conditional<sc.f32 != 0, float, double>::type;
+        const void* fn = nullptr;
+        if (stat == 1) {
+          fn = col_static_modes<F, sc.split != 0, sc.n, sc.T, sc.waves, sc.kinds, false>(mode);
+        } else if constexpr (sc.twl_waves > 0) {
+          fn = col_static_modes<F, sc.split != 0, sc.n, sc.T, sc.twl_waves, sc.kinds, true>(mode);
+        }
         if (fn) return fn;
       }
     }
-    return col_static_fn<K + 1>(f32, split, mode, n, T);
+    return col_static_fn<K + 1>(f32, split, mode, n, T, stat);
   } else {
     return nullptr;
   }
@@ -97,7 +106,7 @@ hipError_t PassLaunchers<FF_E>::col(int f32, int mode, int split, int groups, in
                                     const ColArgs& a) {
   const void* fn = nullptr;
   if (stat) {
-    fn = col_static_fn<0>(f32, split, mode, a.lp.n, a.T);
+    fn = col_static_fn<0>(f32, split, mode, a.lp.n, a.T, stat);
   } else if (blue) {
     if (groups == 1 && !split) fn = col_fn<FF_E, false, 1, true>(f32, mode);
   } else if (groups == 1) {

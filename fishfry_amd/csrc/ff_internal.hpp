@@ -77,7 +77,8 @@ struct ColCfg {
   int T = 1, lgT = 0, ps = 30;
   int split = 0;   // FF_TUNE_SPLIT_LDS: one component per LDS slot
   int groups = 1;  // FF_TUNE_TWO_GROUPS: column groups of T per workgroup
-  int stat = 0;    // a kStaticCols kernel runs this configuration (FF_TUNE_GENERIC clears it)
+  int stat = 0;    // a kStaticCols kernel runs this configuration (FF_TUNE_GENERIC clears it):
+                   // 1 twiddles from the table in memory, 2 from LDS (FF_TUNE_LDS_TWIDDLES)
   size_t lds = 0;
   int threads = 0;
 };

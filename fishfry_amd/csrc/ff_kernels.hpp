@@ -163,51 +163,50 @@ enum { COL_FWD = 0, COL_INV = 1, COL_FILTER = 2, COL_SYMBOL = 3, COL_FILTER_FD =
 // inverse transform.  Positions >= nline are zero on entry to the
 // convolution and undefined on exit.
 // SN > 0: the plan of length SN is fixed at compile time (lp is not read).
-template <int E, int DIR, typename F, typename Idx, bool SPLIT, bool BL, int SN = 0>
-__device__ __forceinline__ void line_dft(cx<F> (&v)[E], const LinePlan& lp, int nline, const cx<F>* __restrict__ tw,
+// TW: the twiddle source (tw_at); a table in global memory except where SN > 0.
+template <int E, int DIR, typename F, typename Idx, bool SPLIT, bool BL, int SN = 0, typename TW>
+__device__ __forceinline__ void line_dft(cx<F> (&v)[E], const LinePlan& lp, int nline, const TW& tw,
                                          const void* chirp_p, const void* bsf_p, const void* bsi_p, cx<F>* lds,
                                          const Idx& idx, int r0) {
   static_assert(SN == 0 || !BL, "compile-time plans are for direct lengths");
   if constexpr (SN > 0) {
     fft_line_static<SN, E, DIR, F, Idx, SPLIT>(v, tw, lds, idx, r0);
-    return;
   } else if constexpr (!BL) {
     fft_line<E, DIR, F, Idx, SPLIT>(v, lp, tw, lds, idx, r0);
-    return;
-  }
-  const cx<F>* __restrict__ chirp = reinterpret_cast<const cx<F>*>(chirp_p);
-  const cx<F>* __restrict__ bs = reinterpret_cast<const cx<F>*>(DIR < 0 ? bsf_p : bsi_p);
-  const int L = lp.L;
+  } else {
+    const cx<F>* __restrict__ chirp = reinterpret_cast<const cx<F>*>(chirp_p);
+    const cx<F>* __restrict__ bs = reinterpret_cast<const cx<F>*>(DIR < 0 ? bsf_p : bsi_p);
+    const int L = lp.L;
 #pragma unroll
-  for (int i = 0; i < E; ++i) {
-    const int r = r0 + i * L;
-    if (r < nline) {
-      const cx<F> w = chirp[r];
-      v[i] = twmul<DIR>(v[i], w);
-    } else {
-      v[i] = cx<F>{F(0), F(0)};
+    for (int i = 0; i < E; ++i) {
+      const int r = r0 + i * L;
+      if (r < nline) {
+        const cx<F> w = chirp[r];
+        v[i] = twmul<DIR>(v[i], w);
+      } else {
+        v[i] = cx<F>{F(0), F(0)};
+      }
     }
-  }
-  fft_line<E, -1, F, Idx, SPLIT>(v, lp, tw, lds, idx, r0);
+    fft_line<E, -1, F, Idx, SPLIT>(v, lp, tw, lds, idx, r0);
 #pragma unroll
-  for (int i = 0; i < E; ++i) v[i] = twmul<-1>(v[i], bs[r0 + i * L]);
-  fft_line<E, +1, F, Idx, SPLIT>(v, lp, tw, lds, idx, r0);
+    for (int i = 0; i < E; ++i) v[i] = twmul<-1>(v[i], bs[r0 + i * L]);
+    fft_line<E, +1, F, Idx, SPLIT>(v, lp, tw, lds, idx, r0);
 #pragma unroll
-  for (int i = 0; i < E; ++i) {
-    const int r = r0 + i * L;
-    if (r < nline) {
-      const cx<F> w = chirp[r];
-      v[i] = twmul<DIR>(v[i], w);
+    for (int i = 0; i < E; ++i) {
+      const int r = r0 + i * L;
+      if (r < nline) {
+        const cx<F> w = chirp[r];
+        v[i] = twmul<DIR>(v[i], w);
+      }
     }
   }
 }
 
 // One column line held in registers through the transform, filter or symbol
 // and the inverse transform of MODE.
-template <typename F, int E, int MODE, bool SPLIT, bool BL, int SN>
-__device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const PassInst& io,
-                                         const cx<F>* __restrict__ tw, cx<F>* lds, const ColIdx& idx, int r0,
-                                         int outer, int col, bool valid) {
+template <typename F, int E, int MODE, bool SPLIT, bool BL, int SN, typename TW>
+__device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const PassInst& io, const TW& tw,
+                                         cx<F>* lds, const ColIdx& idx, int r0, int outer, int col, bool valid) {
   const int L = SN > 0 ? SN / E : a.lp.L;
   if constexpr (MODE == COL_FWD) {
     line_dft<E, -1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
@@ -276,45 +275,66 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
 // (LDS pad shift 4), for the lengths that dominate the solve; the workgroup is
 // then exactly (SN/E)*ST threads and SW the waves per SIMD the register
 // allocation must allow (the LDS-limited occupancy of the tile).
-// col_tile: the work of one workgroup, tile b of the pass (b < a.nblocks)
-// with the per-instance fields io; k_col runs one pass, the multi-shape
-// kernels (ff_multi.hip) run the tiles of many instances' passes.
-template <typename F, int E, int MODE, bool SPLIT, int G, bool BL, int SN, int ST>
-__device__ __forceinline__ void col_tile(const ColArgs& a, const PassInst& io, int b) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
-  static_assert(SN == 0 || (ST >= 4 && (ST & (ST - 1)) == 0), "static column tile: 4, 8, ... columns");
-  const int T = SN > 0 ? ST : a.T;
-  const int tid = threadIdx.x;
-  const int c = tid & (T - 1);
-  const int r0 = SN > 0 ? tid / ST : tid >> a.lgT;
-  const int L = SN > 0 ? SN / E : a.lp.L;
-  int outer, tile;
+//
+// A tile's work is split into load, transform and store (col_load,
+// col_transform, col_store); col_tile does one tile (k_col, and the
+// multi-shape kernels of ff_multi.hip).
+
+// Tile b of a pass (b < a.nblocks): its plane, its index within the plane
+// and the plane's column shift.
+struct ColTile {
+  int outer, tile, sh;
+};
+
+template <typename F, int SN>
+__device__ __forceinline__ ColTile col_tile_at(const ColArgs& a, const PassInst& io, int b) {
+  ColTile t;
   if (a.outer_fast) {
-    tile = b / a.nouter;
-    outer = b - tile * a.nouter;
+    t.tile = b / a.nouter;
+    t.outer = b - t.tile * a.nouter;
   } else {
-    outer = b / a.tiles;
-    tile = b - outer * a.tiles;
+    t.outer = b / a.tiles;
+    t.tile = b - t.outer * a.tiles;
   }
   // (the generic kernels never shift: the extra live column index costs
   // them an occupancy step)
-  int sh = 0;
+  t.sh = 0;
   if constexpr (SN > 0)
-    sh = (int)(((unsigned long long)(uintptr_t)io.dst / sizeof(cx<F>) + (unsigned long long)outer * a.cshift_mul) &
-               (unsigned long long)a.cshift_mask);
-  constexpr int kMinCol = SN > 0 ? 0 : INT_MIN;  // columns below 0 exist only when shifted
-  const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
-  // Threads outside the columns load the nearest column (in bounds, never
-  // stored), so the loads need no predicate.  src may equal dst only when
-  // both sides use the same layout (in-place pass): every thread stores
-  // exactly the elements it loaded, after all of its loads.
-  cx<F> v[G][E];
+    t.sh = (int)(((unsigned long long)(uintptr_t)io.dst / sizeof(cx<F>) + (unsigned long long)t.outer * a.cshift_mul) &
+                 (unsigned long long)a.cshift_mask);
+  return t;
+}
+
+// This thread's column within a group and first position.
+template <int E, int SN, int ST>
+__device__ __forceinline__ void col_thread(const ColArgs& a, int& c, int& r0) {
+  const int T = SN > 0 ? ST : a.T;
+  c = threadIdx.x & (T - 1);
+  r0 = SN > 0 ? (int)threadIdx.x / ST : (int)threadIdx.x >> a.lgT;
+}
+
+// Columns below 0 exist only when shifted.
+template <int SN>
+constexpr int col_min() {
+  return SN > 0 ? 0 : INT_MIN;
+}
+
+// Threads outside the columns load the nearest column (in bounds, never
+// stored), so the loads need no predicate.  src may equal dst only when both
+// sides use the same layout (in-place pass): every thread stores exactly the
+// elements it loaded, after all of its loads.
+template <typename F, int E, int G, bool BL, int SN, int ST>
+__device__ __forceinline__ void col_load(cx<F> (&v)[G][E], const ColArgs& a, const PassInst& io, const ColTile& t) {
+  constexpr int kMinCol = col_min<SN>();
+  const int T = SN > 0 ? ST : a.T;
+  const int L = SN > 0 ? SN / E : a.lp.L;
+  int c, r0;
+  col_thread<E, SN, ST>(a, c, r0);
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    const int col = (tile * G + g) * T + c - sh;
+    const int col = (t.tile * G + g) * T + c - t.sh;
     long long ib, is;
-    side_addr(a.in, outer, col < kMinCol ? kMinCol : (col < a.ncols ? col : a.ncols - 1), r0, L, ib, is);
+    side_addr(a.in, t.outer, col < kMinCol ? kMinCol : (col < a.ncols ? col : a.ncols - 1), r0, L, ib, is);
     const FF_GLB cx<F>* src = glb<const cx<F>>(io.src) + ib;
     if constexpr (!BL) {
 #pragma unroll
@@ -324,18 +344,61 @@ __device__ __forceinline__ void col_tile(const ColArgs& a, const PassInst& io, i
       for (int i = 0; i < E; ++i) v[g][i] = r0 + i * L < a.nline ? cx<F>(src[(long long)i * is]) : cx<F>{F(0), F(0)};
     }
   }
-  const ColIdx idx{T, c, SN > 0 ? 4 : a.ps};
+}
+
+// LDS bytes of the exchange of a compiled-in column tile (SN positions, ST
+// columns, a pad slot every 16 positions), rounded up to 16.
+template <typename F, bool SPLIT, int SN, int ST>
+constexpr int col_exchange_bytes() {
+  return (int)(((SN * ST + ((SN - 1) >> 4) + 1) * (SPLIT ? sizeof(F) : sizeof(cx<F>)) + 15) / 16 * 16);
+}
+
+// TWL: the twiddles come from the first half of the table, which the kernel
+// has copied to LDS after the exchange area (col_twiddles_to_lds).
+template <typename F, int E, int MODE, bool SPLIT, int G, bool BL, int SN, int ST, bool TWL = false>
+__device__ __forceinline__ void col_transform(cx<F> (&v)[G][E], const ColArgs& a, const PassInst& io,
+                                              const ColTile& t) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
+  static_assert(SN == 0 || (ST >= 4 && (ST & (ST - 1)) == 0), "static column tile: 4, 8, ... columns");
+  constexpr int kMinCol = col_min<SN>();
+  const int T = SN > 0 ? ST : a.T;
+  int c, r0;
+  col_thread<E, SN, ST>(a, c, r0);
+  if constexpr (TWL) {
+    static_assert(SN > 0 && SN % 32 == 0, "LDS twiddles: compiled-in plans, length a multiple of 32");
+    const ColIdx idx{T, c, 4};
+    const TwHalf<F, SN / 2> tw{reinterpret_cast<const cx<F>*>(smem_raw + col_exchange_bytes<F, SPLIT, SN, ST>())};
 #pragma unroll
-  for (int g = 0; g < G; ++g) {
-    const int col = (tile * G + g) * T + c - sh;
-    col_line<F, E, MODE, SPLIT, BL, SN>(v[g], a, io, tw, lds, idx, r0, outer, col, col >= kMinCol && col < a.ncols);
+    for (int g = 0; g < G; ++g) {
+      const int col = (t.tile * G + g) * T + c - t.sh;
+      col_line<F, E, MODE, SPLIT, BL, SN>(v[g], a, io, tw, lds, idx, r0, t.outer, col, col >= kMinCol && col < a.ncols);
+    }
+    return;
   }
+  const ColIdx idx{T, c, SN > 0 ? 4 : a.ps};
+  const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    const int col = (tile * G + g) * T + c - sh;
+    const int col = (t.tile * G + g) * T + c - t.sh;
+    col_line<F, E, MODE, SPLIT, BL, SN>(v[g], a, io, tw, lds, idx, r0, t.outer, col, col >= kMinCol && col < a.ncols);
+  }
+}
+
+template <typename F, int E, int G, bool BL, int SN, int ST>
+__device__ __forceinline__ void col_store(const cx<F> (&v)[G][E], const ColArgs& a, const PassInst& io,
+                                          const ColTile& t) {
+  constexpr int kMinCol = col_min<SN>();
+  const int T = SN > 0 ? ST : a.T;
+  const int L = SN > 0 ? SN / E : a.lp.L;
+  int c, r0;
+  col_thread<E, SN, ST>(a, c, r0);
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int col = (t.tile * G + g) * T + c - t.sh;
     if (col >= kMinCol && col < a.ncols) {
       long long ob, os;
-      side_addr(a.out, outer, col, r0, L, ob, os);
+      side_addr(a.out, t.outer, col, r0, L, ob, os);
       FF_GLB cx<F>* dst = glb<cx<F>>(io.dst) + ob;
 #pragma unroll
       for (int i = 0; i < E; ++i)
@@ -344,10 +407,48 @@ __device__ __forceinline__ void col_tile(const ColArgs& a, const PassInst& io, i
   }
 }
 
-template <typename F, int E, int MODE, bool SPLIT, int G, bool BL, int SN = 0, int ST = 0, int SW = 1>
+template <typename F, int E, int MODE, bool SPLIT, int G, bool BL, int SN, int ST, bool TWL = false>
+__device__ __forceinline__ void col_tile(const ColArgs& a, const PassInst& io, int b) {
+  const ColTile t = col_tile_at<F, SN>(a, io, b);
+  cx<F> v[G][E];
+  col_load<F, E, G, BL, SN, ST>(v, a, io, t);
+  col_transform<F, E, MODE, SPLIT, G, BL, SN, ST, TWL>(v, a, io, t);
+  col_store<F, E, G, BL, SN, ST>(v, a, io, t);
+}
+
+// Copies the first half of the twiddle table (SN/2 entries, swizzled) to LDS
+// after the exchange area, for col_transform<..., TWL = true>.  The tile's
+// loads are issued first, so the copy overlaps them.  The twiddle loads of
+// the transforms otherwise go through the vector memory path that the tile's
+// own loads and stores keep busy: from LDS the FP64 x pass at 1024 takes 7 %
+// less time on MI355X.  Launch LDS: col_exchange_bytes + (SN/2) complex values.
+template <typename F, bool SPLIT, int SN, int ST, int THREADS>
+__device__ __forceinline__ void col_twiddles_to_lds(const ColArgs& a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<F>* twl = reinterpret_cast<cx<F>*>(smem_raw + col_exchange_bytes<F, SPLIT, SN, ST>());
+  const FF_GLB cx<F>* twg = glb<const cx<F>>(a.tw);
+  for (int i = threadIdx.x; i < SN / 2; i += THREADS) twl[tw_swizzle(i)] = twg[i];
+  __syncthreads();
+}
+
+// TWL (compiled-in plans): twiddles from LDS.
+template <typename F, int E, int MODE, bool SPLIT, int G, bool BL, int SN = 0, int ST = 0, int SW = 1,
+          bool TWL = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, SN > 0 ? SN / E * ST : max_threads_for(E)),
                           amdgpu_waves_per_eu(SN > 0 ? SW : 1))) void k_col(const ColArgs a) {
-  col_tile<F, E, MODE, SPLIT, G, BL, SN, ST>(a, inst_of(a), xcd_remap(blockIdx.x, a.nblocks));
+  const PassInst io = inst_of(a);
+  const int b = xcd_remap(blockIdx.x, a.nblocks);
+  if constexpr (TWL) {
+    static_assert(SN > 0 && G == 1 && !BL, "LDS twiddles: compiled-in plans");
+    const ColTile t = col_tile_at<F, SN>(a, io, b);
+    cx<F> v[1][E];
+    col_load<F, E, 1, false, SN, ST>(v, a, io, t);
+    col_twiddles_to_lds<F, SPLIT, SN, ST, SN / E * ST>(a);
+    col_transform<F, E, MODE, SPLIT, 1, false, SN, ST, true>(v, a, io, t);
+    col_store<F, E, 1, false, SN, ST>(v, a, io, t);
+  } else {
+    col_tile<F, E, MODE, SPLIT, G, BL, SN, ST>(a, io, b);
+  }
 }
 
 struct RowArgs {
@@ -637,11 +738,13 @@ struct Launch {
 // waves per SIMD (slower).
 struct StaticCol {
   int n, E, T, f32;
-  int split;  // FF_TUNE_SPLIT_LDS exchange
-  int waves;  // waves per SIMD the registers must allow
-  int kinds;  // passes it serves, as 1 << kind: 2 the y pass (forward, inverse), 4 the x pass (filter, symbol)
+  int split;     // FF_TUNE_SPLIT_LDS exchange
+  int waves;     // waves per SIMD the registers must allow
+  int kinds;     // passes it serves, as 1 << kind: 2 the y pass (forward, inverse), 4 the x pass (filter, symbol)
+  int twl_waves;  // > 0: the kernel with LDS twiddles (FF_TUNE_LDS_TWIDDLES) is compiled too, for this
+                  // many waves per SIMD
 };
-constexpr StaticCol kStaticCols[] = {{1024, 16, 4, 0, 0, 2, 6}, {1024, 16, 8, 1, 0, 2, 6}, {1024, 16, 4, 0, 1, 3, 4},
+constexpr StaticCol kStaticCols[] = {{1024, 16, 4, 0, 0, 2, 6, 2}, {1024, 16, 8, 1, 0, 2, 6, 2}, {1024, 16, 4, 0, 1, 3, 4, 3},
                                      // x passes only at 512, 768 and 1000 (the y passes are as fast or
                                      // faster generic there); both column passes at 256 and below
                                      {768, 12, 4, 0, 0, 3, 4}, {1000, 20, 4, 0, 0, 2, 4},

@@ -446,14 +446,19 @@ bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, i
   if (out.groups == 2 && out.line.E != 8 && out.line.E != 16) return false;
   if (out.line.bluestein() && (out.split || out.groups != 1)) return false;  // Bluestein: default shape only
   out.stat = 0;
+  const bool twl = (flags & FF_TUNE_LDS_TWIDDLES) != 0;
   if (!(flags & FF_TUNE_GENERIC) && out.groups == 1 && !out.line.bluestein() && out.ps == 4)
     for (const StaticCol& sc : kStaticCols)
       if (sc.n == n && sc.E == out.line.E && sc.T == T && sc.f32 == (elem_bytes == 8 ? 1 : 0) &&
-          sc.split == out.split && (sc.kinds & (1 << kind)))
-        out.stat = static_plan_matches<0>(n, out.line.E, out.line.lp) ? 1 : 0;
+          sc.split == out.split && (sc.kinds & (1 << kind)) && (!twl || sc.twl_waves > 0))
+        out.stat = static_plan_matches<0>(n, out.line.E, out.line.lp) ? (twl ? 2 : 1) : 0;
+  if (twl && out.stat != 2) return false;  // only kernels with a compiled-in plan
   const size_t ne = (size_t)out.line.lp.n;  // engine length (Bluestein: m >= 2n - 1)
   const size_t elems = ne * T + ((ne - 1) >> out.ps) + 1;
   out.lds = elems * (size_t)(out.split ? elem_bytes / 2 : elem_bytes);
+  // LDS twiddles: the first half of the table after the exchange (16-byte
+  // aligned)
+  if (out.stat == 2) out.lds = (out.lds + 15) / 16 * 16 + ne / 2 * (size_t)elem_bytes;
   return out.lds <= 160 * 1024;
 }
 
@@ -503,26 +508,31 @@ bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T, int flags) 
   return out.lds <= 160 * 1024;
 }
 
-// FP64 configurations measured fastest on MI355X for lengths where the
-// fewest-passes heuristic picks a slower elements-per-thread value (the
-// 768^3 and 1000^3 grids); user overrides (ff_tune_set) take precedence.
+// Configurations measured fastest on MI355X where the default heuristic
+// picks a slower one: elements per thread at 768^3 and 1000^3 (FP64), and
+// the kernels with LDS twiddles at 1024 (FP64: 19.7 -> 18.4 ms per 1024^3
+// solve, FP32: 12.3 -> 11.0 ms); user overrides (ff_tune_set) take
+// precedence.
 struct BuiltinCfg {
-  int kind, n, E, T, flags;
+  int kind, n, prec, E, T, flags;
 };
 static const BuiltinCfg kBuiltin[] = {
-    {0, 384, 12, 4, 0},   // z forward, nz = 768
-    {3, 384, 12, 4, 0},   // z inverse, nz = 768
-    {1, 768, 24, 4, 0},   // y, ny = 768
-    {2, 768, 12, 4, 0},   // x, nx = 768
-    {1, 1000, 20, 4, 0},  // y, ny = 1000
-    {2, 1000, 20, 4, 0},  // x, nx = 1000 (compiled-in plan)
-    {2, 1024, 16, 4, FF_TUNE_SPLIT_LDS},  // x, nx = 1024: split exchange, three workgroups per CU
+    {0, 384, 0, 12, 4, 0},   // z forward, nz = 768
+    {3, 384, 0, 12, 4, 0},   // z inverse, nz = 768
+    {1, 768, 0, 24, 4, 0},   // y, ny = 768
+    {2, 768, 0, 12, 4, 0},   // x, nx = 768
+    {1, 1000, 0, 20, 4, 0},  // y, ny = 1000
+    {2, 1000, 0, 20, 4, 0},  // x, nx = 1000 (compiled-in plan)
+    {1, 1024, 0, 16, 4, FF_TUNE_LDS_TWIDDLES},  // y, ny = 1024
+    // x, nx = 1024: split exchange, three workgroups per CU
+    {2, 1024, 0, 16, 4, FF_TUNE_SPLIT_LDS | FF_TUNE_LDS_TWIDDLES},
+    {1, 1024, 1, 16, 8, FF_TUNE_LDS_TWIDDLES},  // FP32 y, ny = 1024
+    {2, 1024, 1, 16, 8, FF_TUNE_LDS_TWIDDLES},  // FP32 x, nx = 1024
 };
 
 static bool builtin_lookup(int kind, int n, int prec, int& E, int& T, int& flags) {
-  if (prec != 0) return false;
   for (const BuiltinCfg& b : kBuiltin)
-    if (b.kind == kind && b.n == n) {
+    if (b.kind == kind && b.n == n && b.prec == prec) {
       E = b.E;
       T = b.T;
       flags = b.flags;
@@ -558,7 +568,7 @@ ff_status tune_set(const ff_tune* t) {
     return FF_OK;
   }
   const int eb = t->precision ? 8 : 16;
-  if (t->flags & ~(FF_TUNE_SPLIT_LDS | FF_TUNE_TWO_GROUPS | FF_TUNE_GENERIC))
+  if (t->flags & ~(FF_TUNE_SPLIT_LDS | FF_TUNE_TWO_GROUPS | FF_TUNE_GENERIC | FF_TUNE_LDS_TWIDDLES))
     return set_error(FF_ERR_INVALID, "ff_tune_set: unknown flags");
   bool ok;
   if (t->kind == 0 || t->kind == 3) {

@@ -125,15 +125,17 @@ def test_tuning_host_logic_without_gpu_work():
     of 512, 768 and 1000, run the kernels with the line plan compiled in
     (flags without FF_TUNE_GENERIC -- which also needs the compile-time plan
     to equal the planner's); the y passes of 512, 768 and 1000 and other
-    lengths run the generic kernel, and the overrides round-trip."""
+    lengths run the generic kernel, and the overrides round-trip.  At 1024 the
+    column passes take their twiddles from LDS by default."""
+    TWL = ff.FF_TUNE_LDS_TWIDDLES
     ff.tune_reset()
     try:
         for kind in (1, 2):
             # the FP64 x pass at 1024 runs the split exchange by default (three workgroups per CU)
             t = ff.tune_query(kind, 1024, 0, 513)
-            assert (t.E, t.T, t.flags) == (16, 4, ff.FF_TUNE_SPLIT_LDS if kind == 2 else 0)
+            assert (t.E, t.T, t.flags) == (16, 4, (ff.FF_TUNE_SPLIT_LDS if kind == 2 else 0) | TWL)
             t = ff.tune_query(kind, 1024, 1, 513)
-            assert (t.E, t.T, t.flags) == (16, 8, 0)
+            assert (t.E, t.T, t.flags) == (16, 8, TWL)
         assert ff.tune_query(1, 512, 0, 257).flags & ff.FF_TUNE_GENERIC
         assert ff.tune_query(2, 600, 0, 301).flags & ff.FF_TUNE_GENERIC
         for kind in (1, 2):
@@ -154,17 +156,23 @@ def test_tuning_host_logic_without_gpu_work():
         ff.tune_reset()
         ff.tune_set(2, 1024, 0, 16, 4, ff.FF_TUNE_GENERIC)
         assert ff.tune_query(2, 1024, 0, 513).flags == ff.FF_TUNE_GENERIC
-        assert ff.tune_query(1, 1024, 0, 513).flags == 0
+        assert ff.tune_query(1, 1024, 0, 513).flags == TWL
         ff.tune_set(1, 1024, 0, 16, 4, ff.FF_TUNE_SPLIT_LDS)  # no compiled-in split kernel for the y pass
         assert ff.tune_query(1, 1024, 0, 513).flags == ff.FF_TUNE_SPLIT_LDS | ff.FF_TUNE_GENERIC
-        ff.tune_set(2, 1024, 0, 16, 4, 0)  # the non-split x kernel on request
+        ff.tune_set(2, 1024, 0, 16, 4, 0)  # the non-split x kernel with twiddles from memory, on request
         assert ff.tune_query(2, 1024, 0, 513).flags == 0
+        ff.tune_set(2, 1024, 0, 16, 4, TWL)
+        assert ff.tune_query(2, 1024, 0, 513).flags == TWL
         with pytest.raises(ff.FishFryError):
-            ff.tune_set(2, 1024, 0, 16, 4, 8)  # unknown flag
+            ff.tune_set(2, 1024, 0, 16, 4, 16)  # unknown flag
+        with pytest.raises(ff.FishFryError):
+            ff.tune_set(1, 512, 0, 8, 4, TWL)  # LDS twiddles need a kernel with the plan compiled in
+        with pytest.raises(ff.FishFryError):
+            ff.tune_set(1, 1024, 0, 16, 4, TWL | ff.FF_TUNE_GENERIC)
         with pytest.raises(ff.FishFryError):
             ff.tune_set(0, 512, 0, 8, 2, ff.FF_TUNE_SPLIT_LDS)  # row passes take FF_TUNE_GENERIC only
         ff.tune_reset()
-        assert ff.tune_query(2, 1024, 0, 513).flags == ff.FF_TUNE_SPLIT_LDS
+        assert ff.tune_query(2, 1024, 0, 513).flags == ff.FF_TUNE_SPLIT_LDS | TWL
         # row passes: compiled-in plans at the default configurations of even
         # nz = 32 ... 1024 (length nz/2), generic otherwise or on request
         for kind in (0, 3):

@@ -205,7 +205,8 @@ def test_layouts_agree_bitwise(n):
 
 
 @pytest.mark.parametrize("variant", [ff.FF_TUNE_SPLIT_LDS, ff.FF_TUNE_TWO_GROUPS,
-                                     ff.FF_TUNE_SPLIT_LDS | ff.FF_TUNE_TWO_GROUPS, ff.FF_TUNE_GENERIC])
+                                     ff.FF_TUNE_SPLIT_LDS | ff.FF_TUNE_TWO_GROUPS, ff.FF_TUNE_GENERIC,
+                                     ff.FF_TUNE_LDS_TWIDDLES, ff.FF_TUNE_SPLIT_LDS | ff.FF_TUNE_LDS_TWIDDLES])
 @pytest.mark.parametrize("n,E", [((64, 48, 40), 0), ((30, 25, 27), 0), ((96, 1000, 64), 10), ((1024, 16, 8), 16),
                                  ((768, 12, 6), 12), ((128, 64, 96), 8), ((256, 1024, 20), 16),
                                  ((512, 512, 12), 0), ((1024, 1024, 4), 0), ((64, 1000, 10), 20),
@@ -213,9 +214,10 @@ def test_layouts_agree_bitwise(n):
                                  ((96, 16, 128), 0), ((48, 8, 64), 0), ((160, 8, 64), 0), ((32, 8, 40), 0)])
 def test_column_variants_agree(n, E, variant):
     """Column-pass variants -- the split real/imaginary LDS exchange, two
-    column groups per workgroup, and the generic kernel in place of the one
-    with the line plan compiled in (length 1024) -- move the same values
-    as the default kernel.  The pass kernels are compiled without
+    column groups per workgroup, the generic kernel in place of the one
+    with the line plan compiled in, and twiddles read from a (half-table,
+    swizzled) LDS copy instead of memory (length 1024) -- move the same
+    values as the kernel with flags 0.  The pass kernels are compiled without
     floating-point contraction and write their multiply-adds out, so every
     variant performs the same rounded operations: results are bit-identical,
     and all match the oracle.  Two groups apply where E is 8 or 16.
@@ -232,7 +234,8 @@ def test_column_variants_agree(n, E, variant):
                 for kind, length in ((1, n[1]), (2, n[0])):
                     for p in sorted({0, pi}):  # FP32 refinement also runs FP64 residual passes
                         cur = ff.tune_query(kind, length, p, n[2] // 2 + 1)
-                        for fl in (flags, flags & ff.FF_TUNE_SPLIT_LDS, 0):  # drop what this E cannot do
+                        # drop what this E / length cannot do
+                        for fl in (flags, flags & ff.FF_TUNE_SPLIT_LDS, 0):
                             try:
                                 ff.tune_set(kind, length, p, E or cur.E, 0 if E else cur.T, fl)
                                 break
