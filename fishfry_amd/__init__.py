@@ -42,6 +42,8 @@ FF_BLOCKED_LAYOUT = 2
 FF_SLAB_PIPELINE = 4
 FF_SYMBOL_3PT = 8
 FF_SYMBOL_5PT = 16
+FF_PENCIL_LAYOUT = 32
+FF_PLAIN_LAYOUT = 64
 FF_TUNE_SPLIT_LDS = 1
 FF_TUNE_TWO_GROUPS = 2
 FF_TUNE_GENERIC = 4
@@ -526,5 +528,5 @@ from .driver import FishFry  # noqa: E402  (needs the names above)
 __all__ = [
     "ParisPencil", "FishFry", "Batch", "Server", "FF_BATCH_GRAPH", "FF_BATCH_PERSISTENT", "FF_BATCH_GROUPED", "FF_BATCH_STEPS", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
     "field_norms", "reference_bytes", "benchmark_grid", "FishFryError", "lib", "LIB_PATH",
-    "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_BLOCKED_LAYOUT", "FF_SLAB_PIPELINE", "FF_SYMBOL_3PT", "FF_SYMBOL_5PT", "FF_TUNE_SPLIT_LDS", "FF_TUNE_TWO_GROUPS", "FF_TUNE_GENERIC", "FF_TUNE_LDS_TWIDDLES",
+    "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_BLOCKED_LAYOUT", "FF_SLAB_PIPELINE", "FF_SYMBOL_3PT", "FF_SYMBOL_5PT", "FF_PENCIL_LAYOUT", "FF_PLAIN_LAYOUT", "FF_TUNE_SPLIT_LDS", "FF_TUNE_TWO_GROUPS", "FF_TUNE_GENERIC", "FF_TUNE_LDS_TWIDDLES",
 ]

@@ -258,7 +258,7 @@ bool col_desc(const PassLaunch& r, MultiColDesc& out, size_t& lds) {
   a.T = T;
   a.lgT = 0;
   while ((1 << a.lgT) < T) a.lgT++;
-  a.ps = 4;
+  a.ps = kColPadShift;
   // rows of a plane 64-byte aligned but planes not: shift the tiles onto the
   // 64-byte grid, as the single-solve kernels with compiled-in plans do
   const long long mask = r.f32 ? 7 : 3;
@@ -275,7 +275,7 @@ bool col_desc(const PassLaunch& r, MultiColDesc& out, size_t& lds) {
   memset(&out, 0, sizeof out);
   out.a = a;
   out.variant = v;
-  lds = ((size_t)mv.n * T + ((mv.n - 1) >> 4) + 1) * (r.f32 ? 8 : 16);
+  lds = ((size_t)mv.n * T + ((mv.n - 1) >> kColPadShift) + 1) * (r.f32 ? 8 : 16);
   return true;
 }
 

@@ -166,8 +166,8 @@ ff_status ff_plan_describe(const ff_plan* p, char* buf, size_t len) {
   snprintf(tmp, sizeof tmp, "\"n\":[%d,%d,%d],\"nh\":%d,\"bytes\":%zu,\"precision\":%d,\"refine_steps\":%d,", p->n[0],
            p->n[1], p->n[2], p->nh, p->bytes, p->opt.precision, p->opt.refine_steps);
   s += tmp;
-  snprintf(tmp, sizeof tmp, "\"dd\":[%.17g,%.17g,%.17g],\"kt\":%d,\"slab\":%d,\"symbol\":%d,", p->ddi, p->ddj,
-           p->ddk, p->lgKT ? 1 << p->lgKT : 0, p->slab, p->sym);
+  snprintf(tmp, sizeof tmp, "\"dd\":[%.17g,%.17g,%.17g],\"kt\":%d,\"slab\":%d,\"symbol\":%d,\"pencil\":%d,", p->ddi,
+           p->ddj, p->ddk, p->lgKT ? 1 << p->lgKT : 0, p->slab, p->sym, p->pencil);
   s += tmp;
   auto line = [&](const char* key, const LineCfg& c, int T, size_t lds, int threads) {
     std::string r = "\"";

@@ -481,6 +481,11 @@ __device__ __forceinline__ void fft_line_static(cx<F> (&v)[E], const TW& tw, cx<
 
 // LDS layouts.  Column tiles: element (line c, position r) at r*T + c plus one
 // pad slot every 2^PS positions.  Row tiles: line-major with a padded pitch.
+// The column pad shift of every column kernel:
+#ifndef FF_COL_PS
+#define FF_COL_PS 4
+#endif
+constexpr int kColPadShift = FF_COL_PS;
 struct ColIdx {
   int T, c, ps;
   __device__ __forceinline__ int operator()(int r) const { return r * T + c + (r >> ps); }

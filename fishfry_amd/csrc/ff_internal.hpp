@@ -95,7 +95,9 @@ struct RowCfg {
 // kind: 0 z forward rows, 1 y columns, 2 x columns, 3 z inverse rows;
 // prec: 0 FP64, 1 FP32.  Tuning
 // overrides (ff_tune_set) take precedence over the built-in heuristic.
-bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int kind, int prec);
+// pencil: the plan runs the pencil spectrum layout (the FP64 column passes
+// transpose as they transform), which has its own measured defaults
+bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int kind, int prec, bool pencil = false);
 bool make_row_cfg(int nz, int elem_bytes, RowCfg& out, int kind, int prec);
 bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, int flags, int kind);
 bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T, int flags);

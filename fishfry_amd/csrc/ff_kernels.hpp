@@ -41,7 +41,12 @@ __device__ __forceinline__ double sqr(double x) { return x * x; }
 // `outer` and transform positions r:
 //   offset(outer, r, col) = kb*KS + (outer*fo + r*fr)*W + kt,
 //   kb = col >> lgKT, kt = col & (KT-1).
-// The plain layout is the single-block case (KT >= nh, W = nh).
+// The plain layout is the single-block case (KT >= nh, W = nh).  With
+// lgKT = 0 (blocks of one column, W = 1) it is any strided layout,
+//   offset = col*KS + outer*fo + r*fr,
+// which the pencil layouts use (FF_PENCIL_LAYOUT): [i][kh][j] after the y
+// passes and [kh][j][i] after the x pass, so every column pass writes its
+// lines contiguously.
 struct SpecSide {
   long long KS;
   long long fo, fr;
@@ -84,6 +89,7 @@ struct ColArgs {
   int nblocks;        // nouter * tiles
   int ps;             // LDS pad shift
   int nx, ny;         // extents for the spectral symbol (x pass)
+  int cols_j;         // x pass of the pencil layout: columns are j and planes kh (else planes j, columns kh)
   // Alignment of the written row segments (kernels with a compiled-in plan).
   // When all rows of a plane start at one offset within a 64-byte block but
   // planes start at different ones (plain x pass, odd nh), a plane's tiles
@@ -214,16 +220,18 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
     line_dft<E, +1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   } else {
     line_dft<E, -1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
-    // spectral filter of ParisPencil.cpp:269-284 at (i, j, kh) = (r, outer, col),
-    // or the finite-difference symbol from the per-axis tables
-    const int jj = outer, kk = col;
+    // spectral filter of ParisPencil.cpp:269-284 at (i, j, kh) = (r, outer, col)
+    // (pencil layout: (r, col, outer)), or the finite-difference symbol from
+    // the per-axis tables
+    const int jj = a.cols_j ? col : outer, kk = a.cols_j ? outer : col;
+    const bool jv = a.cols_j ? valid : true, kv = a.cols_j ? true : valid;  // index in range
     // the (0,0,0) term before the multiply is the sum of the input field
     if (io.dc_out && valid && r0 == 0 && jj == 0 && kk == 0) *glb<double>(io.dc_out) = double(v[0].x);
     if constexpr (MODE == COL_FILTER_FD) {
       // columns outside the plane and Bluestein positions past the line read
       // entry 0 (their values are never stored)
-      const double tj = glb<const double>(a.symy)[jj] * io.ddj;
-      const double gk = glb<const double>(a.symz)[valid ? kk : 0];
+      const double tj = glb<const double>(a.symy)[jv ? jj : 0] * io.ddj;
+      const double gk = glb<const double>(a.symz)[kv ? kk : 0];
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const int ii = r0 + i * L;
@@ -350,7 +358,7 @@ __device__ __forceinline__ void col_load(cx<F> (&v)[G][E], const ColArgs& a, con
 // columns, a pad slot every 16 positions), rounded up to 16.
 template <typename F, bool SPLIT, int SN, int ST>
 constexpr int col_exchange_bytes() {
-  return (int)(((SN * ST + ((SN - 1) >> 4) + 1) * (SPLIT ? sizeof(F) : sizeof(cx<F>)) + 15) / 16 * 16);
+  return (int)(((SN * ST + ((SN - 1) >> kColPadShift) + 1) * (SPLIT ? sizeof(F) : sizeof(cx<F>)) + 15) / 16 * 16);
 }
 
 // TWL: the twiddles come from the first half of the table, which the kernel
@@ -367,7 +375,7 @@ __device__ __forceinline__ void col_transform(cx<F> (&v)[G][E], const ColArgs& a
   col_thread<E, SN, ST>(a, c, r0);
   if constexpr (TWL) {
     static_assert(SN > 0 && SN % 32 == 0, "LDS twiddles: compiled-in plans, length a multiple of 32");
-    const ColIdx idx{T, c, 4};
+    const ColIdx idx{T, c, kColPadShift};
     const TwHalf<F, SN / 2> tw{reinterpret_cast<const cx<F>*>(smem_raw + col_exchange_bytes<F, SPLIT, SN, ST>())};
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -376,7 +384,7 @@ __device__ __forceinline__ void col_transform(cx<F> (&v)[G][E], const ColArgs& a
     }
     return;
   }
-  const ColIdx idx{T, c, SN > 0 ? 4 : a.ps};
+  const ColIdx idx{T, c, SN > 0 ? kColPadShift : a.ps};
   const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
 #pragma unroll
   for (int g = 0; g < G; ++g) {
@@ -457,7 +465,9 @@ struct RowArgs {
   const void* aux;  // f for the residual epilogue
   long long nrows;
   int nz, nh;
-  SpecSide spec;  // k_c2r: layout of the half spectrum read (lgKT >= 30: plain rows)
+  SpecSide spec;  // k_c2r: layout of the half spectrum read (lgKT >= 30: plain rows; lgKT = 0:
+                  // strided, element (i, j, kh) at i*fo + j*fr + kh*KS, row = i*ny + j)
+  int ny;
   int T;          // rows per workgroup
   uint32_t Linv;  // ceil(2^32 / L)
   int nblocks;
@@ -633,6 +643,36 @@ __device__ __forceinline__ void c2r_tile(const RowArgs& a, const PassInst& io, i
       if ((!ODD && !BL) || k < a.nh) lds[idx(k)] = src[k];
     }
     if (!ODD && r0 == 0) lds[idx(M)] = src[M];
+  } else if (a.spec.lgKT == 0) {
+    // strided spectrum (pencil layout [i][kh][j]): the workgroup's T rows are
+    // T consecutive j, contiguous for each kh.  Thread t copies row t % T at
+    // kh = t / T + i*L (i < E, all loads issued first) and the last entry kh
+    // = M: a wave reads 64/T segments of T elements.  Rows past the end load
+    // the last row (in bounds, never stored).
+    const int rr = tid % a.T;
+    const int m = tid / a.T;
+    long long r = (long long)blk * a.T + rr;
+    if (r >= a.nrows) r = a.nrows - 1;
+    const unsigned ii = (unsigned)r / (unsigned)a.ny;  // nx*ny < 2^32
+    const unsigned jj = (unsigned)r - ii * (unsigned)a.ny;
+    const FF_GLB cx<F>* __restrict__ src =
+        glb<const cx<F>>(io.src) + ((long long)ii * a.spec.fo + (long long)jj * a.spec.fr);
+    const RowIdx sidx{rr * a.pitch, a.ps};
+    cx<F> t[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int k = m + i * L;
+      // even nz without Bluestein: k < M < nh always
+      if ((!ODD && !BL) || k < a.nh) t[i] = src[(long long)k * a.spec.KS];
+    }
+    cx<F> last{};
+    if (!ODD && m == 0) last = src[(long long)M * a.spec.KS];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int k = m + i * L;
+      if ((!ODD && !BL) || k < a.nh) lds[sidx(k)] = t[i];
+    }
+    if (!ODD && m == 0) lds[sidx(M)] = last;
   } else {
     // y-blocked spectrum: the workgroup's T rows are contiguous inside each
     // kh block (T*KT elements), so all threads copy element e of the tile in
@@ -744,7 +784,7 @@ struct StaticCol {
   int twl_waves;  // > 0: the kernel with LDS twiddles (FF_TUNE_LDS_TWIDDLES) is compiled too, for this
                   // many waves per SIMD
 };
-constexpr StaticCol kStaticCols[] = {{1024, 16, 4, 0, 0, 2, 6, 2}, {1024, 16, 8, 1, 0, 2, 6, 2}, {1024, 16, 4, 0, 1, 3, 4, 3},
+constexpr StaticCol kStaticCols[] = {{1024, 16, 4, 0, 0, 2, 6, 2}, {1024, 16, 8, 1, 0, 2, 6, 2}, {1024, 16, 4, 0, 1, 3, 6, 3},
                                      // x passes only at 512, 768 and 1000 (the y passes are as fast or
                                      // faster generic there); both column passes at 256 and below
                                      {768, 12, 4, 0, 0, 3, 4}, {1000, 20, 4, 0, 0, 2, 4},

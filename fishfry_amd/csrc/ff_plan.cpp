@@ -439,7 +439,7 @@ bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, i
   out.T = T;
   out.lgT = 0;
   while ((1 << out.lgT) < T) out.lgT++;
-  out.ps = 4;
+  out.ps = kColPadShift;
   out.threads = T * L;
   out.split = (flags & FF_TUNE_SPLIT_LDS) ? 1 : 0;
   out.groups = (flags & FF_TUNE_TWO_GROUPS) ? 2 : 1;
@@ -447,7 +447,7 @@ bool col_cfg_with(int n, int ncols, int elem_bytes, ColCfg& out, int E, int T, i
   if (out.line.bluestein() && (out.split || out.groups != 1)) return false;  // Bluestein: default shape only
   out.stat = 0;
   const bool twl = (flags & FF_TUNE_LDS_TWIDDLES) != 0;
-  if (!(flags & FF_TUNE_GENERIC) && out.groups == 1 && !out.line.bluestein() && out.ps == 4)
+  if (!(flags & FF_TUNE_GENERIC) && out.groups == 1 && !out.line.bluestein() && out.ps == kColPadShift)
     for (const StaticCol& sc : kStaticCols)
       if (sc.n == n && sc.E == out.line.E && sc.T == T && sc.f32 == (elem_bytes == 8 ? 1 : 0) &&
           sc.split == out.split && (sc.kinds & (1 << kind)) && (!twl || sc.twl_waves > 0))
@@ -511,28 +511,31 @@ bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T, int flags) 
 // Configurations measured fastest on MI355X where the default heuristic
 // picks a slower one: elements per thread at 768^3 and 1000^3 (FP64), and
 // the kernels with LDS twiddles at 1024 (FP64: 19.7 -> 18.4 ms per 1024^3
-// solve, FP32: 12.3 -> 11.0 ms); user overrides (ff_tune_set) take
-// precedence.
+// solve, FP32: 12.3 -> 11.0 ms); with the pencil layout the y passes at 1024
+// also take the split exchange (three workgroups per CU: 18.4 -> 17.3 ms;
+// in the plain layout the same kernel is slower).  lay: -1 any layout, 0
+// plain, 1 pencil.  User overrides (ff_tune_set) take precedence.
 struct BuiltinCfg {
-  int kind, n, prec, E, T, flags;
+  int kind, n, prec, lay, E, T, flags;
 };
 static const BuiltinCfg kBuiltin[] = {
-    {0, 384, 0, 12, 4, 0},   // z forward, nz = 768
-    {3, 384, 0, 12, 4, 0},   // z inverse, nz = 768
-    {1, 768, 0, 24, 4, 0},   // y, ny = 768
-    {2, 768, 0, 12, 4, 0},   // x, nx = 768
-    {1, 1000, 0, 20, 4, 0},  // y, ny = 1000
-    {2, 1000, 0, 20, 4, 0},  // x, nx = 1000 (compiled-in plan)
-    {1, 1024, 0, 16, 4, FF_TUNE_LDS_TWIDDLES},  // y, ny = 1024
+    {0, 384, 0, -1, 12, 4, 0},   // z forward, nz = 768
+    {3, 384, 0, -1, 12, 4, 0},   // z inverse, nz = 768
+    {1, 768, 0, -1, 24, 4, 0},   // y, ny = 768
+    {2, 768, 0, -1, 12, 4, 0},   // x, nx = 768
+    {1, 1000, 0, -1, 20, 4, 0},  // y, ny = 1000
+    {2, 1000, 0, -1, 20, 4, 0},  // x, nx = 1000 (compiled-in plan)
+    {1, 1024, 0, 0, 16, 4, FF_TUNE_LDS_TWIDDLES},                      // y, ny = 1024
+    {1, 1024, 0, 1, 16, 4, FF_TUNE_SPLIT_LDS | FF_TUNE_LDS_TWIDDLES},  // y, ny = 1024, pencil
     // x, nx = 1024: split exchange, three workgroups per CU
-    {2, 1024, 0, 16, 4, FF_TUNE_SPLIT_LDS | FF_TUNE_LDS_TWIDDLES},
-    {1, 1024, 1, 16, 8, FF_TUNE_LDS_TWIDDLES},  // FP32 y, ny = 1024
-    {2, 1024, 1, 16, 8, FF_TUNE_LDS_TWIDDLES},  // FP32 x, nx = 1024
+    {2, 1024, 0, -1, 16, 4, FF_TUNE_SPLIT_LDS | FF_TUNE_LDS_TWIDDLES},
+    {1, 1024, 1, -1, 16, 8, FF_TUNE_LDS_TWIDDLES},  // FP32 y, ny = 1024
+    {2, 1024, 1, -1, 16, 8, FF_TUNE_LDS_TWIDDLES},  // FP32 x, nx = 1024
 };
 
-static bool builtin_lookup(int kind, int n, int prec, int& E, int& T, int& flags) {
+static bool builtin_lookup(int kind, int n, int prec, int& E, int& T, int& flags, bool pencil = false) {
   for (const BuiltinCfg& b : kBuiltin)
-    if (b.kind == kind && b.n == n && b.prec == prec) {
+    if (b.kind == kind && b.n == n && b.prec == prec && (b.lay < 0 || b.lay == (pencil ? 1 : 0))) {
       E = b.E;
       T = b.T;
       flags = b.flags;
@@ -541,11 +544,11 @@ static bool builtin_lookup(int kind, int n, int prec, int& E, int& T, int& flags
   return false;
 }
 
-bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int kind, int prec) {
+bool make_col_cfg(int n, int ncols, int elem_bytes, ColCfg& out, int kind, int prec, bool pencil) {
   int E = 0, T = 0, flags = 0;
   if (tune_lookup(kind, n, prec, E, T, flags) && col_cfg_with(n, ncols, elem_bytes, out, E, T, flags, kind))
     return true;
-  if (builtin_lookup(kind, n, prec, E, T, flags) && col_cfg_with(n, ncols, elem_bytes, out, E, T, flags, kind))
+  if (builtin_lookup(kind, n, prec, E, T, flags, pencil) && col_cfg_with(n, ncols, elem_bytes, out, E, T, flags, kind))
     return true;
   return col_cfg_with(n, ncols, elem_bytes, out, 0, 0, 0, kind);
 }

@@ -58,8 +58,12 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   if (o.precision != FF_FP64 && o.precision != FF_FP32_REFINE)
     return set_error(FF_ERR_INVALID, "options.precision must be FF_FP64 or FF_FP32_REFINE");
   if (o.refine_steps < 0 || o.refine_steps > 8) return set_error(FF_ERR_INVALID, "refine_steps must be 0..8");
-  if (o.flags & ~(FF_PRESERVE_INPUT | FF_BLOCKED_LAYOUT | FF_SLAB_PIPELINE | FF_SYMBOL_3PT | FF_SYMBOL_5PT))
+  if (o.flags & ~(FF_PRESERVE_INPUT | FF_BLOCKED_LAYOUT | FF_SLAB_PIPELINE | FF_SYMBOL_3PT | FF_SYMBOL_5PT |
+                  FF_PENCIL_LAYOUT | FF_PLAIN_LAYOUT))
     return set_error(FF_ERR_INVALID, "unknown bits in options.flags");
+  if ((o.flags & FF_PENCIL_LAYOUT) && (o.flags & (FF_BLOCKED_LAYOUT | FF_SLAB_PIPELINE | FF_PLAIN_LAYOUT)))
+    return set_error(FF_ERR_INVALID,
+                     "options.flags: FF_PENCIL_LAYOUT excludes FF_BLOCKED_LAYOUT, FF_SLAB_PIPELINE and FF_PLAIN_LAYOUT");
   if ((o.flags & FF_SYMBOL_3PT) && (o.flags & FF_SYMBOL_5PT))
     return set_error(FF_ERR_INVALID, "options.flags: at most one of FF_SYMBOL_3PT and FF_SYMBOL_5PT");
 
@@ -85,6 +89,14 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   p->bytes = reference_bytes(n, ones);
   // kh blocks of 8 complex doubles (one 128-byte line per block row)
   p->lgKT = (o.flags & FF_BLOCKED_LAYOUT) ? 3 : 0;
+  // pencil layout: on request, or by default for FP64 grids of 2^25 spectrum
+  // points and more with nz <= 1024 (measured on MI355X: 512^3 and 768^3
+  // 12-13 % faster, 1024^3 8 %, 2048x1024x256 36 %, 1000^3 even; 128^3 and
+  // 256^3 slower, and 1296^3, whose z inverse rows do not fit eight to a
+  // workgroup in half the LDS)
+  const bool other_layout = (o.flags & (FF_PLAIN_LAYOUT | FF_BLOCKED_LAYOUT | FF_SLAB_PIPELINE)) != 0;
+  p->pencil = o.precision == FF_FP64 &&
+              ((o.flags & FF_PENCIL_LAYOUT) || (!other_layout && p->Nh >= (1LL << 25) && n[2] <= 1024));
   p->slab = 0;
   if ((o.flags & FF_SLAB_PIPELINE) && !p->lgKT) {
     const long long plane = (long long)n[1] * p->nh * 16;  // bytes of one x-plane of the spectrum
@@ -101,7 +113,8 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   for (int prec = 0; prec < nprec; ++prec) {
     const int eb = prec ? 8 : 16;
     if (!make_row_cfg(n[2], eb, p->zr[prec], 0, prec) || !make_row_cfg(n[2], eb, p->zi[prec], 3, prec) ||
-        !make_col_cfg(n[1], p->nh, eb, p->cy[prec], 1, prec) || !make_col_cfg(n[0], p->nh, eb, p->cx[prec], 2, prec)) {
+        !make_col_cfg(n[1], p->nh, eb, p->cy[prec], 1, prec, p->pencil && prec == 0) ||
+        !make_col_cfg(n[0], p->nh, eb, p->cx[prec], 2, prec, p->pencil && prec == 0)) {
       delete p;
       char buf[160];
       snprintf(buf, sizeof buf,
@@ -109,6 +122,22 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
                "workgroup's line transform",
                n[0], n[1], n[2]);
       return set_error(FF_ERR_UNSUPPORTED, buf);
+    }
+  }
+
+  // pencil layout: the z inverse reads its rows from [i][kh][j], T
+  // consecutive j per kh, so it takes 8 rows per workgroup (128-byte row
+  // segments) where two such workgroups fit a CU's LDS, else 4 (unless
+  // tuned: ff_tune_set takes precedence)
+  int tE, tT, tF;
+  if (p->pencil && !tune_lookup(3, n[2] % 2 ? n[2] : n[2] / 2, 0, tE, tT, tF)) {
+    for (int T : {8, 4}) {
+      RowCfg c;
+      if (row_cfg_with(n[2], 16, c, p->zi[0].line.E, T, p->zi[0].stat ? 0 : FF_TUNE_GENERIC) &&
+          (T == 4 || c.lds <= 80 * 1024)) {
+        p->zi[0] = c;
+        break;
+      }
     }
   }
 
@@ -121,7 +150,12 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
       const long long tiles = (p->nh + 7 + (long long)c.groups * c.T - 1) / ((long long)c.groups * c.T);
       return (long long)nouter * tiles;
     };
-    if (rb > INT_MAX || col_blocks(p->cy[prec], n[0]) > INT_MAX || col_blocks(p->cx[prec], n[1]) > INT_MAX) {
+    // (pencil layout: the x and y inverse passes run nh planes of ny and nx columns)
+    auto pencil_blocks = [&](const ColCfg& c, int ncols) {
+      return p->pencil ? (long long)p->nh * ((ncols + c.T - 1) / c.T) : 0LL;
+    };
+    if (rb > INT_MAX || col_blocks(p->cy[prec], n[0]) > INT_MAX || col_blocks(p->cx[prec], n[1]) > INT_MAX ||
+        pencil_blocks(p->cx[prec], n[1]) > INT_MAX || pencil_blocks(p->cy[prec], n[0]) > INT_MAX) {
       delete p;
       return set_error(FF_ERR_UNSUPPORTED, "grid too large: a pass would need more than 2^31-1 workgroups");
     }
@@ -200,11 +234,31 @@ void plan_destroy(ff_plan* p) {
 // ---- passes -------------------------------------------------------------------
 
 // A layout as seen by a column pass along axis (1 = y: planes i, positions j;
-// 0 = x: planes j, positions i), or by the row passes (axis 1).
+// 0 = x: planes j, positions i), or by the row passes (axis 1).  The pencil
+// layouts address (col, outer, r) by strides, with the roles of the pass:
+// y forward (col kh, outer i, r j), x (col j, outer kh, r i), y inverse
+// (col i, outer kh, r j); the z inverse reads (row i*ny + j, kh).
+enum { ROLE_Y = 1, ROLE_X = 0, ROLE_XT = 2, ROLE_YT = 3 };
 static SpecSide spec_side(const ff_plan* p, int axis, int lay) {
   SpecSide s;
   memset(&s, 0, sizeof s);
-  const long long nx = p->n[0], ny = p->n[1];
+  const long long nx = p->n[0], ny = p->n[1], nh = p->nh;
+  if (lay == LAY_A || lay == LAY_B || lay == LAY_D) {
+    // element strides of (i, j, kh)
+    long long si, sj, sk;
+    if (lay == LAY_A) si = nh * ny, sj = 1, sk = ny;        // [i][kh][j]
+    else if (lay == LAY_B) si = 1, sj = nx, sk = ny * nx;   // [kh][j][i]
+    else si = ny, sj = 1, sk = nx * ny;                     // [kh][i][j]
+    s.lgKT = 0;  // blocks of one column: offset = col*KS + outer*fo + r*fr
+    s.Wlast = 1;
+    switch (axis) {
+      case ROLE_Y: s.KS = sk, s.fo = si, s.fr = sj, s.nb = (int)nh; break;
+      case ROLE_XT: s.KS = sj, s.fo = sk, s.fr = si, s.nb = (int)ny; break;
+      case ROLE_YT: s.KS = si, s.fo = sk, s.fr = sj, s.nb = (int)nx; break;
+      default: break;
+    }
+    return s;
+  }
   if (lay == LAY_PLAIN) {
     s.lgKT = 30;  // one block of width nh
     s.nb = 1;
@@ -228,6 +282,16 @@ static SpecSide spec_side(const ff_plan* p, int axis, int lay) {
 
 void pass_layouts(const ff_plan* p, int pass, int& lay_in, int& lay_out) {
   lay_in = lay_out = LAY_PLAIN;
+  if (p->pencil) {
+    switch (pass) {
+      case 1: lay_out = LAY_D; break;                  // y forward: [i][j][kh] -> [kh][i][j]
+      case 2: lay_in = LAY_D; lay_out = LAY_B; break;  // x: [kh][i][j] -> [kh][j][i]
+      case 3: lay_in = LAY_B; lay_out = LAY_A; break;  // y inverse: [kh][j][i] -> [i][kh][j]
+      case 4: lay_in = LAY_A; break;                   // z inverse reads [i][kh][j]
+      default: break;
+    }
+    return;
+  }
   if (!p->lgKT) return;
   switch (pass) {
     case 1: lay_out = LAY_Y; break;                  // y forward: plain -> y-blocked
@@ -245,6 +309,7 @@ static RowArgs row_args(const ff_plan* p, int prec, bool inverse) {
   a.nrows = (long long)p->n[0] * p->n[1];
   a.nz = p->n[2];
   a.nh = p->nh;
+  a.ny = p->n[1];
   a.spec = spec_side(p, 1, LAY_PLAIN);
   a.T = c.T;
   a.Linv = c.Linv;
@@ -318,8 +383,26 @@ ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* sr
   // y: planes i, positions j (ParisPencil.cpp:79 plan, batch nx*nh);
   // x: planes j, positions i (ParisPencil.cpp:78 plan, batch ny*nh)
   a.nouter = axis == 1 ? p->n[0] : p->n[1];
-  a.in = spec_side(p, axis, lay_in);
-  a.out = spec_side(p, axis, lay_out);
+  // pencil layout: the x pass reads [i][kh][j] (columns j, planes kh) and the
+  // y inverse reads [kh][j][i] (columns i, planes kh)
+  int role = axis;
+  if (axis == 0 && (lay_in == LAY_A || lay_in == LAY_D)) role = ROLE_XT;
+  if (axis == 1 && lay_in == LAY_B) role = ROLE_YT;
+  auto pencil = [](int l) { return l == LAY_A || l == LAY_B || l == LAY_D; };
+  if ((pencil(lay_in) || pencil(lay_out)) &&
+      !((role == ROLE_Y && lay_in == LAY_PLAIN && (lay_out == LAY_A || lay_out == LAY_D)) ||
+        (role == ROLE_XT && lay_out == LAY_B) || (role == ROLE_YT && lay_out == LAY_A)))
+    return set_error(FF_ERR_INVALID, "column pass: unsupported pencil layout pair");
+  if (role == ROLE_XT) {
+    a.ncols = p->n[1];
+    a.nouter = p->nh;
+    a.cols_j = 1;
+  } else if (role == ROLE_YT) {
+    a.ncols = p->n[0];
+    a.nouter = p->nh;
+  }
+  a.in = spec_side(p, role, lay_in);
+  a.out = spec_side(p, role, lay_out);
   if (nouter >= 0) {
     // planes [outer0, outer0 + nouter) of a y pass over the plain layout
     if (axis != 1 || lay_in != LAY_PLAIN || lay_out != LAY_PLAIN)

@@ -8,6 +8,7 @@ struct ff_plan {
   double lo[3], hi[3];
   int nh;
   int lgKT;      // FP64 spectrum between the y passes: 0 plain, else kh blocks of 2^lgKT
+  int pencil;    // FF_PENCIL_LAYOUT (FP64 passes)
   int slab;      // FF_SLAB_PIPELINE: x-planes per slab (0: whole passes)
   long long N;   // nx*ny*nz
   long long Nh;  // nx*ny*nh
@@ -59,7 +60,9 @@ ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void
                    long long row0 = 0, long long nrows = -1);
 // Half-spectrum layouts (see SpecSide): plain [i][j][kh], y-blocked
 // [kb][i][j][kt], x-blocked [kb][j][i][kt].
-enum { LAY_PLAIN = 0, LAY_Y = 1, LAY_X = 2 };
+// spectrum layouts: plain [i][j][kh]; blocked y [kb][i][j][kt] and x
+// [kb][j][i][kt]; pencil A [i][kh][j], B [kh][j][i] and D [kh][i][j]
+enum { LAY_PLAIN = 0, LAY_Y = 1, LAY_X = 2, LAY_A = 3, LAY_B = 4, LAY_D = 5 };
 ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* src, void* dst, hipStream_t s,
                   double* dc_out = nullptr, int lay_in = LAY_PLAIN, int lay_out = LAY_PLAIN, int outer0 = 0,
                   int nouter = -1);

@@ -64,6 +64,12 @@ def test_plan_create_rejects_without_gpu_work():
     with pytest.raises(ff.FishFryError) as e:
         ff.ParisPencil((1, 8, 8), lo1, hi1)
     assert e.value.status == ff.FF_ERR_INVALID
+    # option flags: unknown bits, and layouts that exclude each other
+    for flags in (128, ff.FF_PENCIL_LAYOUT | ff.FF_PLAIN_LAYOUT, ff.FF_PENCIL_LAYOUT | ff.FF_BLOCKED_LAYOUT,
+                  ff.FF_PENCIL_LAYOUT | ff.FF_SLAB_PIPELINE, ff.FF_SYMBOL_3PT | ff.FF_SYMBOL_5PT):
+        with pytest.raises(ff.FishFryError) as e:
+            ff.ParisPencil((16, 16, 16), lo, hi, flags=flags)
+        assert e.value.status == ff.FF_ERR_INVALID, flags
     L = ff.lib()
     h = ctypes.c_void_p()
     rc = L.ff_plan_create(None, None, None, None, None, None, ctypes.byref(h))
@@ -157,8 +163,10 @@ def test_tuning_host_logic_without_gpu_work():
         ff.tune_set(2, 1024, 0, 16, 4, ff.FF_TUNE_GENERIC)
         assert ff.tune_query(2, 1024, 0, 513).flags == ff.FF_TUNE_GENERIC
         assert ff.tune_query(1, 1024, 0, 513).flags == TWL
-        ff.tune_set(1, 1024, 0, 16, 4, ff.FF_TUNE_SPLIT_LDS)  # no compiled-in split kernel for the y pass
-        assert ff.tune_query(1, 1024, 0, 513).flags == ff.FF_TUNE_SPLIT_LDS | ff.FF_TUNE_GENERIC
+        ff.tune_set(1, 1024, 0, 16, 4, ff.FF_TUNE_SPLIT_LDS)  # the compiled-in split kernel (pencil default)
+        assert ff.tune_query(1, 1024, 0, 513).flags == ff.FF_TUNE_SPLIT_LDS
+        ff.tune_set(1, 1024, 1, 16, 8, ff.FF_TUNE_SPLIT_LDS)  # no compiled-in FP32 split kernel
+        assert ff.tune_query(1, 1024, 1, 513).flags == ff.FF_TUNE_SPLIT_LDS | ff.FF_TUNE_GENERIC
         ff.tune_set(2, 1024, 0, 16, 4, 0)  # the non-split x kernel with twiddles from memory, on request
         assert ff.tune_query(2, 1024, 0, 513).flags == 0
         ff.tune_set(2, 1024, 0, 16, 4, TWL)

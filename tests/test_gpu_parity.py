@@ -186,13 +186,48 @@ def test_x_pass_tile_shift_bitwise(n):
     assert rel(outs[0], want) < TOL
 
 
-@pytest.mark.parametrize("n", [(32, 1024, 512), (48, 40, 36), (30, 25, 27), (64, 64, 2), (96, 160, 48)])
+@pytest.mark.parametrize("n", [(512, 512, 512), (2048, 1024, 256), (512, 256, 514)])
+def test_pencil_default_matches_plain(n):
+    """FP64 plans of 2^25 spectrum points and more take the pencil layout by
+    default; FF_PLAIN_LAYOUT keeps the reference's [i][j][kh].  The two give
+    bit-identical potentials, and the default one meets the analytic norms."""
+    field = ff.field_default(n)
+    lo, hi, _ = ff.benchmark_grid(n)
+    outs = []
+    for flags in (0, ff.FF_PLAIN_LAYOUT):
+        plan = ff.ParisPencil(n, lo, hi, flags=flags)
+        info = json.loads(plan.describe())
+        say("pencil default", n, flags, "pencil", info["pencil"])
+        assert info["pencil"] == (0 if flags else 1)
+        nb = plan.bytes()
+        a = torch.empty(nb // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty(nb // 8, dtype=torch.float64, device="cuda")
+        ff.field_init(field, a)
+        plan.solve(nb, a, b)
+        torch.cuda.synchronize()
+        N = n[0] * n[1] * n[2]
+        outs.append(b[:N].clone())
+        if not flags:
+            assert ff.field_norms(field, b)[2] < 4e-15
+        plan.close()
+        del a, b
+    assert torch.equal(outs[0], outs[1])
+    del outs
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("n", [(32, 1024, 512), (48, 40, 36), (30, 25, 27), (64, 64, 2), (96, 160, 48),
+                               (9, 7, 12), (5, 3, 9), (1024, 24, 64), (24, 1024, 30), (36, 1000, 20)])
 def test_layouts_agree_bitwise(n):
-    """The blocked spectrum layouts change addresses only, not arithmetic."""
+    """The blocked and pencil spectrum layouts change addresses only, not
+    arithmetic.  The pencil shapes include ny below the z inverse's rows per
+    workgroup (rows of one workgroup in several x-planes), odd nz, Bluestein
+    and 1024-point columns."""
     rng = np.random.default_rng(4)
     f = rng.standard_normal(n)
     outs = []
-    for flags in (0, ff.FF_BLOCKED_LAYOUT, ff.FF_BLOCKED_LAYOUT | ff.FF_PRESERVE_INPUT):
+    P = ff.FF_PENCIL_LAYOUT
+    for flags in (0, ff.FF_BLOCKED_LAYOUT, ff.FF_BLOCKED_LAYOUT | ff.FF_PRESERVE_INPUT, P, P | ff.FF_PRESERVE_INPUT):
         plan, nb, a, b = make(n, flags=flags)
         say("layout", n, flags, "kt", kt_of(plan))
         put(a, f)
@@ -200,7 +235,8 @@ def test_layouts_agree_bitwise(n):
         torch.cuda.synchronize()
         outs.append(get_real(b, n))
         plan.close()
-    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    for k in range(1, len(outs)):
+        assert np.array_equal(outs[0], outs[k]), k
     assert rel(outs[0], oracle.solve(f)) < TOL
 
 
