@@ -94,9 +94,16 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   // 12-13 % faster, 1024^3 8 %, 2048x1024x256 36 %, 1000^3 even; 128^3 and
   // 256^3 slower, and 1296^3, whose z inverse rows do not fit eight to a
   // workgroup in half the LDS)
+  // FP32 plans run their FP32 passes in the pencil layout on request only,
+  // between the two halves of the refinement work area (at 1024^3 their
+  // FP32 passes are 4 % slower than in place); their FP64 residual passes
+  // stay in place in the plain layout.
   const bool other_layout = (o.flags & (FF_PLAIN_LAYOUT | FF_BLOCKED_LAYOUT | FF_SLAB_PIPELINE)) != 0;
-  p->pencil = o.precision == FF_FP64 &&
-              ((o.flags & FF_PENCIL_LAYOUT) || (!other_layout && p->Nh >= (1LL << 25) && n[2] <= 1024));
+  p->pencil = (o.flags & FF_PENCIL_LAYOUT) ||
+                      (o.precision == FF_FP64 && !other_layout && p->Nh >= (1LL << 25) && n[2] <= 1024)
+                  ? 1
+                  : 0;
+  const int mainp = o.precision == FF_FP64 ? 0 : 1;  // precision of the passes the layout applies to
   p->slab = 0;
   if ((o.flags & FF_SLAB_PIPELINE) && !p->lgKT) {
     const long long plane = (long long)n[1] * p->nh * 16;  // bytes of one x-plane of the spectrum
@@ -113,8 +120,8 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   for (int prec = 0; prec < nprec; ++prec) {
     const int eb = prec ? 8 : 16;
     if (!make_row_cfg(n[2], eb, p->zr[prec], 0, prec) || !make_row_cfg(n[2], eb, p->zi[prec], 3, prec) ||
-        !make_col_cfg(n[1], p->nh, eb, p->cy[prec], 1, prec, p->pencil && prec == 0) ||
-        !make_col_cfg(n[0], p->nh, eb, p->cx[prec], 2, prec, p->pencil && prec == 0)) {
+        !make_col_cfg(n[1], p->nh, eb, p->cy[prec], 1, prec, p->pencil && prec == mainp) ||
+        !make_col_cfg(n[0], p->nh, eb, p->cx[prec], 2, prec, p->pencil && prec == mainp)) {
       delete p;
       char buf[160];
       snprintf(buf, sizeof buf,
@@ -126,16 +133,18 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   }
 
   // pencil layout: the z inverse reads its rows from [i][kh][j], T
-  // consecutive j per kh, so it takes 8 rows per workgroup (128-byte row
-  // segments) where two such workgroups fit a CU's LDS, else 4 (unless
-  // tuned: ff_tune_set takes precedence)
+  // consecutive j per kh, so it takes 128 bytes of rows per workgroup (8
+  // FP64 or 16 FP32 rows) where two such workgroups fit a CU's LDS, else 64
+  // (unless tuned: ff_tune_set takes precedence)
   int tE, tT, tF;
-  if (p->pencil && !tune_lookup(3, n[2] % 2 ? n[2] : n[2] / 2, 0, tE, tT, tF)) {
-    for (int T : {8, 4}) {
+  if (p->pencil && !tune_lookup(3, n[2] % 2 ? n[2] : n[2] / 2, mainp, tE, tT, tF)) {
+    RowCfg& zi = p->zi[mainp];
+    const int eb = mainp ? 8 : 16;
+    for (int T : {128 / eb, 64 / eb}) {
       RowCfg c;
-      if (row_cfg_with(n[2], 16, c, p->zi[0].line.E, T, p->zi[0].stat ? 0 : FF_TUNE_GENERIC) &&
-          (T == 4 || c.lds <= 80 * 1024)) {
-        p->zi[0] = c;
+      if (row_cfg_with(n[2], eb, c, zi.line.E, T, zi.stat ? 0 : FF_TUNE_GENERIC) &&
+          (T == 64 / eb || c.lds <= 80 * 1024)) {
+        zi = c;
         break;
       }
     }
@@ -461,15 +470,33 @@ ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* sr
                    "column pass launch");
 }
 
+// The FP32 transform round trip of the solve and of each correction, from
+// src (FP64 f or the FP32 residual r) to phi (stored or accumulated): in
+// place in the work area (plain layout), or, with the pencil layout, between
+// its two halves (each holds one FP32 spectrum).
+static ff_status fp32_round_trip(const ff_plan* p, int in_f32, const void* src, double* phi, int epi, double* dc_col,
+                                 const double* shiftp, hipStream_t s) {
+  void* a = p->spec;
+  void* b = p->pencil ? static_cast<void*>(static_cast<char*>(p->spec) + p->Nh * 8) : p->spec;
+  ff_status st;
+  if ((st = run_zfwd(p, 1, in_f32, src, a, s))) return st;
+  if (p->pencil) {
+    if ((st = run_col(p, 1, 1, COL_FWD, a, b, s, nullptr, LAY_PLAIN, LAY_D))) return st;
+    if ((st = run_col(p, 1, 0, COL_FILTER, b, a, s, dc_col, LAY_D, LAY_B))) return st;
+    if ((st = run_col(p, 1, 1, COL_INV, a, b, s, nullptr, LAY_B, LAY_A))) return st;
+    return run_zinv(p, 1, 0, epi, b, phi, nullptr, shiftp, s, LAY_A);
+  }
+  if ((st = run_col(p, 1, 1, COL_FWD, a, a, s))) return st;
+  if ((st = run_col(p, 1, 0, COL_FILTER, a, a, s, dc_col))) return st;
+  if ((st = run_col(p, 1, 1, COL_INV, a, a, s))) return st;
+  return run_zinv(p, 1, 0, epi, a, phi, nullptr, shiftp, s);
+}
+
 static ff_status solve_fp32_refine(const ff_plan* p, const double* f, double* phi, hipStream_t s) {
   void* spec = p->spec;
   ff_status st;
   // phi0: FP32 passes, FP64 in and out
-  if ((st = run_zfwd(p, 1, 0, f, spec, s))) return st;
-  if ((st = run_col(p, 1, 1, COL_FWD, spec, spec, s))) return st;
-  if ((st = run_col(p, 1, 0, COL_FILTER, spec, spec, s, p->dcf))) return st;
-  if ((st = run_col(p, 1, 1, COL_INV, spec, spec, s))) return st;
-  if ((st = run_zinv(p, 1, 0, EPI_STORE, spec, phi, nullptr, nullptr, s))) return st;
+  if ((st = fp32_round_trip(p, 0, f, phi, EPI_STORE, p->dcf, nullptr, s))) return st;
   for (int it = 0; it < p->opt.refine_steps; ++it) {
     if (p->sym) {
       // finite-difference symbol: r = (f - mean f) - L_h phi with the stencil
@@ -488,11 +515,7 @@ static ff_status solve_fp32_refine(const ff_plan* p, const double* f, double* ph
       if ((st = run_zinv(p, 0, 1, EPI_RESIDUAL, spec, p->res, f, p->dcf, s))) return st;
     }
     // correction in FP32: phi += L_h^-1 r - mean(phi)
-    if ((st = run_zfwd(p, 1, 1, p->res, spec, s))) return st;
-    if ((st = run_col(p, 1, 1, COL_FWD, spec, spec, s))) return st;
-    if ((st = run_col(p, 1, 0, COL_FILTER, spec, spec, s))) return st;
-    if ((st = run_col(p, 1, 1, COL_INV, spec, spec, s))) return st;
-    if ((st = run_zinv(p, 1, 0, EPI_ACCUM, spec, phi, nullptr, p->dc, s))) return st;
+    if ((st = fp32_round_trip(p, 1, p->res, phi, EPI_ACCUM, nullptr, p->dc, s))) return st;
   }
   return FF_OK;
 }

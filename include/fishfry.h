@@ -71,16 +71,18 @@ typedef enum ff_precision {
                                f and phi instead of an FP64 transform round trip)     */
 #define FF_SYMBOL_5PT 16    /* the same with the 5-point (fourth-order) Laplacian: axis
                                term (4/h^2) s^2 (1 + s^2/3), 13-point stencil residual   */
-#define FF_PENCIL_LAYOUT 32 /* FP64: every column pass transposes as it transforms, so it
+#define FF_PENCIL_LAYOUT 32 /* every column pass transposes as it transforms, so it
                                reads strided columns and writes its lines contiguously:
                                [kh][i][j] after the y forward pass, [kh][j][i] after the
                                x pass, [i][kh][j] after the y inverse pass (which the z
                                inverse reads); the reference's pencil transposes
                                (ParisPencil.cpp:167-377) folded into the passes.
-                               Default for FP64 plans of nx*ny*(nz/2+1) >= 2^25 points
-                               and nz <= 1024 (ff_plan_describe "pencil": 1)         */
-#define FF_PLAIN_LAYOUT 64  /* FP64: the reference's [i][j][kh] between all passes even
-                               where FF_PENCIL_LAYOUT would be the default              */
+                               FP32 plans (on request): their FP32 passes, between the
+                               two halves of the work area (FP64 residual passes stay
+                               in place).  Default for FP64 plans of nx*ny*(nz/2+1) >=
+                               2^25 points and nz <= 1024 (ff_plan_describe "pencil")  */
+#define FF_PLAIN_LAYOUT 64  /* the reference's [i][j][kh] between all passes even where
+                               FF_PENCIL_LAYOUT would be the default                    */
 
 typedef struct ff_options {
   int precision;    /* ff_precision                                             */

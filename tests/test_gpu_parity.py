@@ -216,6 +216,30 @@ def test_pencil_default_matches_plain(n):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("n", [(48, 40, 36), (30, 25, 27), (9, 7, 12), (1024, 24, 64), (36, 1000, 20)])
+@pytest.mark.parametrize("flags", [0, ff.FF_SYMBOL_3PT, ff.FF_SYMBOL_5PT])
+def test_pencil_layout_fp32_and_symbols_bitwise(n, flags):
+    """FP32 refinement plans run their FP32 passes in the pencil layout
+    between the two halves of the work area, and the finite-difference
+    symbols index the pencil x pass by (i, j, kh) as the plain one does: the
+    potentials are bit-identical to the plain layout's (FP64, FP32 + 0 and +
+    2 steps), and match the oracle."""
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal(n)
+    f -= f.mean()
+    for prec, steps in ((ff.FF_FP64, 0), (ff.FF_FP32_REFINE, 0), (ff.FF_FP32_REFINE, 2)):
+        outs = []
+        for lay in (ff.FF_PLAIN_LAYOUT, ff.FF_PENCIL_LAYOUT):
+            say("pencil", n, flags, prec, steps, lay)
+            outs.append(gpu_solve(f, precision=prec, refine_steps=steps, flags=flags | lay))
+        assert np.array_equal(outs[0], outs[1])
+        want = oracle.solve(f, symbol={0: 0, ff.FF_SYMBOL_3PT: 3, ff.FF_SYMBOL_5PT: 5}[flags])
+        # FP32 + 2 steps: the residual floor (1.03e-13 for the 5-point symbol
+        # at 1024x24x64); FP32 without refinement: FP32 rounding
+        tol = TOL if prec == ff.FF_FP64 else (1e-12 if steps else 1e-5)
+        assert rel(outs[1], want) < tol
+
+
 @pytest.mark.parametrize("n", [(32, 1024, 512), (48, 40, 36), (30, 25, 27), (64, 64, 2), (96, 160, 48),
                                (9, 7, 12), (5, 3, 9), (1024, 24, 64), (24, 1024, 30), (36, 1000, 20)])
 def test_layouts_agree_bitwise(n):
