@@ -247,6 +247,7 @@ bool groups_current(const ff_batch* b, double* const* d, double* const* p) {
 bool col_desc(const PassLaunch& r, MultiColDesc& out, size_t& lds) {
   const ColArgs& c = r.col;
   if (r.blue || r.groups != 1 || r.split || c.outer_fast) return false;
+  if (c.in.nlo || c.xtw || c.imap_n2 || c.jmap_n1) return false;  // four-step lines
   if (c.in.nb != 1 || c.out.nb != 1 || c.in.lgKT < 30 || c.out.lgKT < 30 || c.nline != c.lp.n) return false;
   if (!multi_kernel_exists(MULTI_COL, r.f32, r.sel, 0)) return false;
   const int v = multi_col_variant(c.lp.n, r.E, c.lp);
@@ -689,8 +690,11 @@ ff_status ff_batch_autotune(ff_batch* b, double* const* density, double* const* 
     for (int prec = 0; prec < nprec; ++prec) {
       const int M = p->n[2] % 2 ? p->n[2] : p->n[2] / 2;
       const int keys[4][2] = {{0, M}, {1, p->n[1]}, {2, p->n[0]}, {3, M}};
-      for (auto& kv : keys)
+      for (auto& kv : keys) {
+        // four-step column lines have no single configuration to tune
+        if ((kv[0] == 1 && p->fy.n) || (kv[0] == 2 && p->fx.n)) continue;
         if (!seen.count({kv[0], kv[1], prec})) seen[{kv[0], kv[1], prec}] = i;
+      }
     }
   }
   for (auto& kv : seen) {

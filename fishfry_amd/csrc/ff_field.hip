@@ -9,6 +9,8 @@
 // so they are bit-reproducible run to run.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "../../include/fishfry.h"
 #include "ff_internal.hpp"
 
@@ -25,11 +27,8 @@ struct FieldArgs {
   double minv[FF_MAX_MODES];  // -1/lambda_k of each mode (exact potential)
 };
 
-__global__ __launch_bounds__(256) void k_field_init(const FieldArgs a, double* __restrict__ f) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-  const int i = blockIdx.z;
-  if (k >= a.nk) return;
+// One element of f at (i, j, k).
+__device__ __forceinline__ void field_point(const FieldArgs& a, double* __restrict__ f, int i, int j, int k) {
   // FishFry.hpp:22-44, same expressions and order
   const double x = a.xlo + i * a.dx + a.sx;
   const double cx = cos(x);
@@ -60,6 +59,17 @@ __global__ __launch_bounds__(256) void k_field_init(const FieldArgs a, double* _
     v += a.ma[m] * c + a.mb[m] * s;
   }
   f[((long long)i * a.nj + j) * a.nk + k] = v;
+}
+
+// Grid (ceil(nk/256), rows): block row y runs rows (i, j) = y, y + gridDim.y, ...
+__global__ __launch_bounds__(256) void k_field_init(const FieldArgs a, double* __restrict__ f) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.nk) return;
+  const long long rows = (long long)a.ni * a.nj;
+  for (long long row = blockIdx.y; row < rows; row += gridDim.y) {
+    const int i = (int)(row / a.nj);
+    field_point(a, f, i, (int)(row - (long long)i * a.nj), k);
+  }
 }
 
 // Exact potential at (i, j, k), FishFry.hpp:71-86 plus the extra modes.
@@ -195,7 +205,6 @@ ff_status field_check(const ff_field* f) {
     if (f->n[c] < 1) return set_error(FF_ERR_INVALID, "ff_field: extents must be >= 1");
   if (f->nmodes < 0 || f->nmodes > FF_MAX_MODES) return set_error(FF_ERR_INVALID, "ff_field: nmodes out of range");
   if ((long long)f->n[0] * f->n[1] >= (1LL << 31)) return set_error(FF_ERR_INVALID, "ff_field: grid too large");
-  if (f->n[1] > 65535 || f->n[0] > 65535) return set_error(FF_ERR_INVALID, "ff_field: ni, nj must be <= 65535");
   return FF_OK;
 }
 
@@ -205,7 +214,8 @@ ff_status field_init(const ff_field* f, double* out, hipStream_t s) {
   if (!out) return set_error(FF_ERR_INVALID, "ff_field_init: null output");
   FieldArgs a;
   field_args(*f, a);
-  dim3 grid((a.nk + 255) / 256, a.nj, a.ni);
+  const long long rows = (long long)a.ni * a.nj;
+  dim3 grid((a.nk + 255) / 256, (unsigned)std::min<long long>(rows, 65535));
   hipLaunchKernelGGL(k_field_init, grid, dim3(256), 0, s, a, out);
   return hip_check(hipGetLastError(), "k_field_init launch");
 }

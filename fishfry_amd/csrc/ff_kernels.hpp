@@ -47,20 +47,28 @@ __device__ __forceinline__ double sqr(double x) { return x * x; }
 // which the pencil layouts use (FF_PENCIL_LAYOUT): [i][kh][j] after the y
 // passes and [kh][j][i] after the x pass, so every column pass writes its
 // lines contiguously.
+// Four-step sub-lines (generic kernels only): with nlo > 0 a plane index is
+// the pair (outer % nlo, outer / nlo) with strides fo and fo2.
 struct SpecSide {
   long long KS;
   long long fo, fr;
   int lgKT;
   int nb, Wlast;
+  int nlo;
+  long long fo2;
 };
 
 // Offset of (outer, r0, col) and the offset step between positions r and r+L.
+template <bool FS = false>
 __device__ __forceinline__ void side_addr(const SpecSide& s, int outer, int col, int r0, int L, long long& base,
                                           long long& step) {
   const int kb = col >> s.lgKT;
   const int kt = col & ((1 << s.lgKT) - 1);
   const long long W = kb == s.nb - 1 ? (long long)s.Wlast : (1LL << s.lgKT);
-  base = (long long)kb * s.KS + ((long long)outer * s.fo + (long long)r0 * s.fr) * W + kt;
+  long long ot = (long long)outer * s.fo;
+  if constexpr (FS)
+    if (s.nlo) ot = (long long)(outer % s.nlo) * s.fo + (long long)(outer / s.nlo) * s.fo2;
+  base = (long long)kb * s.KS + (ot + (long long)r0 * s.fr) * W + kt;
   step = (long long)L * s.fr * W;
 }
 
@@ -90,6 +98,15 @@ struct ColArgs {
   int ps;             // LDS pad shift
   int nx, ny;         // extents for the spectral symbol (x pass)
   int cols_j;         // x pass of the pencil layout: columns are j and planes kh (else planes j, columns kh)
+  // Four-step lines (generic kernels; the planes are (lo, hi) pairs, in.nlo > 0):
+  // xtw: W_n, n = line length; a forward transform multiplies position r of
+  // plane lo by W_n[lo*r] after the transform, an inverse one by its
+  // conjugate before (0: no twiddle).  The x filter's i index is lo +
+  // imap_n2*r where imap_n2 > 0, and its j index p (the plane, or the hi
+  // part of it) is p / jmap_n1 + jmap_n2 * (p % jmap_n1) where jmap_n1 > 0
+  // (y lines held in four-step order).
+  const void* xtw;
+  int imap_n2, jmap_n1, jmap_n2;
   // Alignment of the written row segments (kernels with a compiled-in plan).
   // When all rows of a plane start at one offset within a 64-byte block but
   // planes start at different ones (plain x pass, odd nh), a plane's tiles
@@ -214,6 +231,16 @@ template <typename F, int E, int MODE, bool SPLIT, bool BL, int SN, typename TW>
 __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const PassInst& io, const TW& tw,
                                          cx<F>* lds, const ColIdx& idx, int r0, int outer, int col, bool valid) {
   const int L = SN > 0 ? SN / E : a.lp.L;
+  // four-step lines (generic kernels): plane (lo, hi), the twiddle table
+  int plo = 0, phi = outer;
+  if constexpr (SN == 0) {
+    if (a.in.nlo) {
+      plo = outer % a.in.nlo;
+      phi = outer / a.in.nlo;
+    }
+  }
+  // (the four-step twiddles of forward and inverse sub-lines are applied by
+  // col_store and col_load, where the transform's registers are free)
   if constexpr (MODE == COL_FWD) {
     line_dft<E, -1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   } else if constexpr (MODE == COL_INV) {
@@ -222,11 +249,18 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
     line_dft<E, -1, F, ColIdx, SPLIT, BL, SN>(v, a.lp, a.nline, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
     // spectral filter of ParisPencil.cpp:269-284 at (i, j, kh) = (r, outer, col)
     // (pencil layout: (r, col, outer)), or the finite-difference symbol from
-    // the per-axis tables
-    const int jj = a.cols_j ? col : outer, kk = a.cols_j ? outer : col;
+    // the per-axis tables.  Four-step lines: i = lo + imap_n2*r, and j
+    // planes in four-step order map to their frequency.
+    int jj = a.cols_j ? col : phi;
+    const int kk = a.cols_j ? outer : col;
+    int istep = 1;
+    if constexpr (SN == 0) {
+      if (a.jmap_n1) jj = jj / a.jmap_n1 + a.jmap_n2 * (jj % a.jmap_n1);
+      if (a.imap_n2) istep = a.imap_n2;
+    }
     const bool jv = a.cols_j ? valid : true, kv = a.cols_j ? true : valid;  // index in range
     // the (0,0,0) term before the multiply is the sum of the input field
-    if (io.dc_out && valid && r0 == 0 && jj == 0 && kk == 0) *glb<double>(io.dc_out) = double(v[0].x);
+    if (io.dc_out && valid && r0 == 0 && plo == 0 && jj == 0 && kk == 0) *glb<double>(io.dc_out) = double(v[0].x);
     if constexpr (MODE == COL_FILTER_FD) {
       // columns outside the plane and Bluestein positions past the line read
       // entry 0 (their values are never stored)
@@ -234,7 +268,7 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
       const double gk = glb<const double>(a.symz)[kv ? kk : 0];
 #pragma unroll
       for (int i = 0; i < E; ++i) {
-        const int ii = r0 + i * L;
+        const int ii = plo + istep * (r0 + i * L);
         const double gi = glb<const double>(a.symx)[ii < a.nx ? ii : 0];
         const double s2 = fmadd(gk, io.ddk, fmadd(gi, io.ddi, tj));  // (i + j) + k
         const double d = (ii | jj | kk) ? -1.0 / s2 : 0.0;
@@ -246,7 +280,7 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
       const double dk = double(kk) * io.ddk;
 #pragma unroll
       for (int i = 0; i < E; ++i) {
-        const int ii = r0 + i * L;
+        const int ii = plo + istep * (r0 + i * L);
         const double di = double(ii < a.nx - ii ? ii : a.nx - ii) * io.ddi;
         const double s2 = fmadd(dk, dk, fmadd(di, di, j2));  // (i^2 + j^2) + k^2, as the reference sums
         double d;
@@ -331,7 +365,23 @@ constexpr int col_min() {
 // stored), so the loads need no predicate.  src may equal dst only when both
 // sides use the same layout (in-place pass): every thread stores exactly the
 // elements it loaded, after all of its loads.
-template <typename F, int E, int G, bool BL, int SN, int ST>
+// Four-step sub-lines (generic kernels): position r of plane lo times W_n[lo*r]
+// (DIR < 0, after a forward transform) or its conjugate (before an inverse).
+template <int DIR, typename F, int E, int G>
+__device__ __forceinline__ void four_step_twiddle(cx<F> (&v)[G][E], const ColArgs& a, int outer, int r0, int L) {
+  if (!a.xtw) return;
+  const int lo = outer % a.in.nlo;
+  const FF_GLB cx<F>* w = glb<const cx<F>>(a.xtw);
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int r = r0 + i * L;
+      if (r < a.nline) v[g][i] = twmul<DIR>(v[g][i], cx<F>(w[lo * r]));
+    }
+}
+
+template <typename F, int E, int G, bool BL, int SN, int ST, int MODE = COL_FWD>
 __device__ __forceinline__ void col_load(cx<F> (&v)[G][E], const ColArgs& a, const PassInst& io, const ColTile& t) {
   constexpr int kMinCol = col_min<SN>();
   const int T = SN > 0 ? ST : a.T;
@@ -342,7 +392,7 @@ __device__ __forceinline__ void col_load(cx<F> (&v)[G][E], const ColArgs& a, con
   for (int g = 0; g < G; ++g) {
     const int col = (t.tile * G + g) * T + c - t.sh;
     long long ib, is;
-    side_addr(a.in, t.outer, col < kMinCol ? kMinCol : (col < a.ncols ? col : a.ncols - 1), r0, L, ib, is);
+    side_addr<SN == 0>(a.in, t.outer, col < kMinCol ? kMinCol : (col < a.ncols ? col : a.ncols - 1), r0, L, ib, is);
     const FF_GLB cx<F>* src = glb<const cx<F>>(io.src) + ib;
     if constexpr (!BL) {
 #pragma unroll
@@ -352,6 +402,7 @@ __device__ __forceinline__ void col_load(cx<F> (&v)[G][E], const ColArgs& a, con
       for (int i = 0; i < E; ++i) v[g][i] = r0 + i * L < a.nline ? cx<F>(src[(long long)i * is]) : cx<F>{F(0), F(0)};
     }
   }
+  if constexpr (SN == 0 && MODE == COL_INV) four_step_twiddle<+1>(v, a, t.outer, r0, L);
 }
 
 // LDS bytes of the exchange of a compiled-in column tile (SN positions, ST
@@ -393,20 +444,20 @@ __device__ __forceinline__ void col_transform(cx<F> (&v)[G][E], const ColArgs& a
   }
 }
 
-template <typename F, int E, int G, bool BL, int SN, int ST>
-__device__ __forceinline__ void col_store(const cx<F> (&v)[G][E], const ColArgs& a, const PassInst& io,
-                                          const ColTile& t) {
+template <typename F, int E, int G, bool BL, int SN, int ST, int MODE = COL_FWD>
+__device__ __forceinline__ void col_store(cx<F> (&v)[G][E], const ColArgs& a, const PassInst& io, const ColTile& t) {
   constexpr int kMinCol = col_min<SN>();
   const int T = SN > 0 ? ST : a.T;
   const int L = SN > 0 ? SN / E : a.lp.L;
   int c, r0;
   col_thread<E, SN, ST>(a, c, r0);
+  if constexpr (SN == 0 && MODE == COL_FWD) four_step_twiddle<-1>(v, a, t.outer, r0, L);
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     const int col = (t.tile * G + g) * T + c - t.sh;
     if (col >= kMinCol && col < a.ncols) {
       long long ob, os;
-      side_addr(a.out, t.outer, col, r0, L, ob, os);
+      side_addr<SN == 0>(a.out, t.outer, col, r0, L, ob, os);
       FF_GLB cx<F>* dst = glb<cx<F>>(io.dst) + ob;
 #pragma unroll
       for (int i = 0; i < E; ++i)
@@ -419,9 +470,9 @@ template <typename F, int E, int MODE, bool SPLIT, int G, bool BL, int SN, int S
 __device__ __forceinline__ void col_tile(const ColArgs& a, const PassInst& io, int b) {
   const ColTile t = col_tile_at<F, SN>(a, io, b);
   cx<F> v[G][E];
-  col_load<F, E, G, BL, SN, ST>(v, a, io, t);
+  col_load<F, E, G, BL, SN, ST, MODE>(v, a, io, t);
   col_transform<F, E, MODE, SPLIT, G, BL, SN, ST, TWL>(v, a, io, t);
-  col_store<F, E, G, BL, SN, ST>(v, a, io, t);
+  col_store<F, E, G, BL, SN, ST, MODE>(v, a, io, t);
 }
 
 // Copies the first half of the twiddle table (SN/2 entries, swizzled) to LDS
@@ -653,10 +704,9 @@ __device__ __forceinline__ void c2r_tile(const RowArgs& a, const PassInst& io, i
     const int m = tid / a.T;
     long long r = (long long)blk * a.T + rr;
     if (r >= a.nrows) r = a.nrows - 1;
-    const unsigned ii = (unsigned)r / (unsigned)a.ny;  // nx*ny < 2^32
-    const unsigned jj = (unsigned)r - ii * (unsigned)a.ny;
-    const FF_GLB cx<F>* __restrict__ src =
-        glb<const cx<F>>(io.src) + ((long long)ii * a.spec.fo + (long long)jj * a.spec.fr);
+    const long long ii = r / a.ny;
+    const long long jj = r - ii * a.ny;
+    const FF_GLB cx<F>* __restrict__ src = glb<const cx<F>>(io.src) + (ii * a.spec.fo + jj * a.spec.fr);
     const RowIdx sidx{rr * a.pitch, a.ps};
     cx<F> t[E];
 #pragma unroll

@@ -45,6 +45,40 @@ static double dd_const(int n, double lo, double hi) {
   return 2.0 * M_PI * double(n - 1) / (double(n) * (hi - lo));
 }
 
+// Four-step split of an x or y extent n that no workgroup transforms whole:
+// n = n1*n2 (n1 <= n2), both sub-line lengths planned (generic kernels,
+// Bluestein allowed), the split with the least work n2*cost(n1) +
+// n1*cost(n2), cost = engine length * (passes + 0.04 E).  False when n has
+// no such split (a prime above the Bluestein limit).
+static bool four_step_cfg(int n, int ncols, int eb, int kind, int prec, ff_plan::FourStep& fs) {
+  auto cost = [](const ColCfg& c) { return double(c.line.lp.n) * (double(c.line.lp.np) + 0.04 * c.line.E); };
+  double best = 1e300;
+  int b1 = 0;
+  ColCfg bc1, bc2;
+  for (int n1 = 2; (long long)n1 * n1 <= n; ++n1) {
+    if (n % n1) continue;
+    const int n2 = n / n1;
+    ColCfg c1, c2;
+    if (!col_cfg_with(n1, ncols, eb, c1, 0, 0, FF_TUNE_GENERIC, kind) ||
+        !col_cfg_with(n2, ncols, eb, c2, 0, 0, FF_TUNE_GENERIC, kind))
+      continue;
+    const double w = double(n2) * cost(c1) + double(n1) * cost(c2);
+    if (w < best) {
+      best = w;
+      b1 = n1;
+      bc1 = c1;
+      bc2 = c2;
+    }
+  }
+  if (!b1) return false;
+  fs.n = n;
+  fs.n1 = b1;
+  fs.n2 = n / b1;
+  fs.c1[prec] = bc1;
+  fs.c2[prec] = bc2;
+  return true;
+}
+
 ff_status plan_create(const int n[3], const double lo[3], const double hi[3], const ff_options* opts,
                       ff_plan** out, bool own_scratch) {
   *out = nullptr;
@@ -52,7 +86,6 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     if (n[a] < 1) return set_error(FF_ERR_INVALID, "grid extents must be >= 1");
     if (!(hi[a] > lo[a])) return set_error(FF_ERR_INVALID, "hi must exceed lo on every axis");
   }
-  if (n[0] > 65535 || n[1] > 65535) return set_error(FF_ERR_INVALID, "nx, ny must be <= 65535");
   ff_options o{FF_FP64, 0, 0};
   if (opts) o = *opts;
   if (o.precision != FF_FP64 && o.precision != FF_FP32_REFINE)
@@ -117,11 +150,37 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     return st;
   }
   const int nprec = (o.precision == FF_FP64) ? 1 : 2;
+  // x or y extents that no workgroup transforms whole: four-step lines
+  p->fy.n = p->fx.n = 0;
   for (int prec = 0; prec < nprec; ++prec) {
     const int eb = prec ? 8 : 16;
+    ColCfg c;
+    if (!make_col_cfg(n[1], p->nh, eb, c, 1, prec)) four_step_cfg(n[1], p->nh, eb, 1, prec, p->fy);
+    if (!make_col_cfg(n[0], p->nh, eb, c, 2, prec)) four_step_cfg(n[0], p->nh, eb, 2, prec, p->fx);
+  }
+  const bool four = p->fy.n || p->fx.n;
+  if (four) {
+    if (o.flags & (FF_PENCIL_LAYOUT | FF_BLOCKED_LAYOUT | FF_SLAB_PIPELINE)) {
+      delete p;
+      return set_error(FF_ERR_UNSUPPORTED,
+                       "x or y extents transformed in four steps take the plain spectrum layout only");
+    }
+    p->pencil = 0;
+  }
+  for (int prec = 0; prec < nprec; ++prec) {
+    const int eb = prec ? 8 : 16;
+    // with four-step y lines the x pass maps its j planes to frequencies: a
+    // generic kernel (compiled-in ones take the plain order)
+    auto x_cfg = [&](ColCfg& c) -> bool {
+      if (p->fx.n) return true;  // four-step x lines: p->fx
+      if (!make_col_cfg(n[0], p->nh, eb, c, 2, prec, p->pencil && prec == mainp)) return false;
+      if (!p->fy.n || !c.stat) return true;
+      const int fl = FF_TUNE_GENERIC | (c.split ? FF_TUNE_SPLIT_LDS : 0) | (c.groups == 2 ? FF_TUNE_TWO_GROUPS : 0);
+      return col_cfg_with(n[0], p->nh, eb, c, c.line.E, c.T, fl, 2);
+    };
     if (!make_row_cfg(n[2], eb, p->zr[prec], 0, prec) || !make_row_cfg(n[2], eb, p->zi[prec], 3, prec) ||
-        !make_col_cfg(n[1], p->nh, eb, p->cy[prec], 1, prec, p->pencil && prec == mainp) ||
-        !make_col_cfg(n[0], p->nh, eb, p->cx[prec], 2, prec, p->pencil && prec == mainp)) {
+        !(p->fy.n || make_col_cfg(n[1], p->nh, eb, p->cy[prec], 1, prec, p->pencil && prec == mainp)) ||
+        !x_cfg(p->cx[prec]) || (p->fy.n && !p->fy.c1[prec].threads) || (p->fx.n && !p->fx.c1[prec].threads)) {
       delete p;
       char buf[160];
       snprintf(buf, sizeof buf,
@@ -163,8 +222,14 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     auto pencil_blocks = [&](const ColCfg& c, int ncols) {
       return p->pencil ? (long long)p->nh * ((ncols + c.T - 1) / c.T) : 0LL;
     };
-    if (rb > INT_MAX || col_blocks(p->cy[prec], n[0]) > INT_MAX || col_blocks(p->cx[prec], n[1]) > INT_MAX ||
-        pencil_blocks(p->cx[prec], n[1]) > INT_MAX || pencil_blocks(p->cy[prec], n[0]) > INT_MAX) {
+    // (four-step lines: sub-line passes over n1 and n2 times as many planes)
+    auto axis_blocks = [&](const ff_plan::FourStep& fs, const ColCfg& c, int nouter) {
+      if (!fs.n) return col_blocks(c, nouter);
+      return std::max(col_blocks(fs.c1[prec], nouter) * fs.n2, col_blocks(fs.c2[prec], nouter) * fs.n1);
+    };
+    if (rb > INT_MAX || axis_blocks(p->fy, p->cy[prec], n[0]) > INT_MAX ||
+        axis_blocks(p->fx, p->cx[prec], n[1]) > INT_MAX || pencil_blocks(p->cx[prec], n[1]) > INT_MAX ||
+        pencil_blocks(p->cy[prec], n[0]) > INT_MAX) {
       delete p;
       return set_error(FF_ERR_UNSUPPORTED, "grid too large: a pass would need more than 2^31-1 workgroups");
     }
@@ -183,13 +248,27 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     p->tw_z[prec] = table_twiddle(p->zr[prec].line.lp.n, prec);
     p->tw_zi[prec] = table_twiddle(p->zi[prec].line.lp.n, prec);
     p->tw_zpost[prec] = table_post(nz, prec);
-    p->tw_y[prec] = table_twiddle(p->cy[prec].line.lp.n, prec);
-    p->tw_x[prec] = table_twiddle(p->cx[prec].line.lp.n, prec);
-    ok = p->tw_z[prec] && p->tw_zi[prec] && p->tw_zpost[prec] && p->tw_y[prec] && p->tw_x[prec];
+    ok = p->tw_z[prec] && p->tw_zi[prec] && p->tw_zpost[prec];
     blue(p->zr[prec].line, prec, p->bl_z[prec]);
     blue(p->zi[prec].line, prec, p->bl_zi[prec]);
-    blue(p->cy[prec].line, prec, p->bl_y[prec]);
-    blue(p->cx[prec].line, prec, p->bl_x[prec]);
+    // column axes: whole lines, or the two sub-line lengths and W_n
+    for (int axis = 0; axis < 2; ++axis) {
+      ff_plan::FourStep& fs = axis ? p->fy : p->fx;
+      const ColCfg& c = axis ? p->cy[prec] : p->cx[prec];
+      if (fs.n) {
+        fs.tw1[prec] = table_twiddle(fs.c1[prec].line.lp.n, prec);
+        fs.tw2[prec] = table_twiddle(fs.c2[prec].line.lp.n, prec);
+        fs.twn[prec] = table_twiddle(fs.n, prec);
+        ok = ok && fs.tw1[prec] && fs.tw2[prec] && fs.twn[prec];
+        blue(fs.c1[prec].line, prec, fs.bl1[prec]);
+        blue(fs.c2[prec].line, prec, fs.bl2[prec]);
+      } else {
+        const void*& tw = axis ? p->tw_y[prec] : p->tw_x[prec];
+        tw = table_twiddle(c.line.lp.n, prec);
+        ok = ok && tw;
+        blue(c.line, prec, axis ? p->bl_y[prec] : p->bl_x[prec]);
+      }
+    }
   }
   for (int a = 0; a < 3 && ok && p->sym; ++a) ok = (p->symtab[a] = table_symbol(n[a], p->sym)) != nullptr;
   if (!ok) {
@@ -377,9 +456,21 @@ ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void
                    "z inverse launch");
 }
 
-ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* src, void* dst, hipStream_t s,
-                  double* dc_out, int lay_in, int lay_out, int outer0, int nouter) {
-  const ColCfg& c = axis == 1 ? p->cy[prec] : p->cx[prec];
+// Plane and position geometry of a four-step sub-line pass (plain layout, in
+// rows of nh): planes (lo, hi) with lo < nlo, strides fo (lo), fo2 (hi) and
+// fr (positions); the W_n twiddle and the x filter's i map.
+struct SubLines {
+  int nouter, nlo;
+  long long fo, fo2, fr;
+  const void* xtw;
+  int imap_n2;
+};
+
+// One column pass launch with line configuration c (engine twiddles tw,
+// Bluestein tables bl); sub: four-step geometry (null: whole lines).
+static ff_status run_col_cfg(const ff_plan* p, int prec, int axis, int mode, const void* src, void* dst,
+                             hipStream_t s, double* dc_out, int lay_in, int lay_out, int outer0, int nouter,
+                             const ColCfg& c, const void* tw, const void* const* bl, const SubLines* sub) {
   if (src == dst && lay_in != lay_out) return set_error(FF_ERR_INVALID, "in-place column pass must keep its layout");
   if (mode == COL_FILTER && p->sym) mode = COL_FILTER_FD;
   if (mode == COL_SYMBOL && p->sym)
@@ -412,6 +503,23 @@ ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* sr
   }
   a.in = spec_side(p, role, lay_in);
   a.out = spec_side(p, role, lay_out);
+  if (sub) {
+    a.nouter = sub->nouter;
+    for (SpecSide* sd : {&a.in, &a.out}) {
+      sd->nlo = sub->nlo;
+      sd->fo = sub->fo;
+      sd->fo2 = sub->fo2;
+      sd->fr = sub->fr;
+    }
+    a.xtw = sub->xtw;
+    a.imap_n2 = sub->imap_n2;
+  }
+  // four-step y lines leave the y frequencies of the x pass's j planes in
+  // four-step order
+  if (axis == 0 && p->fy.n) {
+    a.jmap_n1 = p->fy.n1;
+    a.jmap_n2 = p->fy.n2;
+  }
   if (nouter >= 0) {
     // planes [outer0, outer0 + nouter) of a y pass over the plain layout
     if (axis != 1 || lay_in != LAY_PLAIN || lay_out != LAY_PLAIN)
@@ -458,16 +566,69 @@ ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* sr
     a.ddk = p->ddk;
   }
   a.dc_out = dc_out;
-  a.tw = axis == 1 ? p->tw_y[prec] : p->tw_x[prec];
+  a.tw = tw;
   a.lp = c.line.lp;
   a.nline = c.line.n;
-  const void* const* bl = axis == 1 ? p->bl_y[prec] : p->bl_x[prec];
   a.chirp = bl[0];
   a.bsf = bl[1];
   a.bsi = bl[2];
   const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
   return hip_check(launch_col(c.line.E, prec, mode, c.split, c.groups, c.line.bluestein(), c.stat, l, a),
                    "column pass launch");
+}
+
+// Four-step lines (ParisPencil.cpp:78-79 plans any length through
+// cufftPlanMany): a line x[j], j = j1 + n1*j2, of length n = n1*n2 is
+//   A: for each j1, the n2-point DFT over j2 (stride n1), times W_n^(j1*k2),
+//      in place at j1 + n1*k2;
+//   B: for each k2, the n1-point DFT over j1 (stride 1), in place: frequency
+//      k2 + n2*k1 lands at k1 + n1*k2.
+// The inverse runs B^-1, then the conjugate twiddle and A^-1, and restores
+// natural order.  The x pass between the y passes reads its j planes in that
+// order (jmap); an x pass of four-step lines runs A, B with the filter at
+// frequency i = k2 + n2*k1, then A^-1.
+ff_status run_col(const ff_plan* p, int prec, int axis, int mode, const void* src, void* dst, hipStream_t s,
+                  double* dc_out, int lay_in, int lay_out, int outer0, int nouter) {
+  const ff_plan::FourStep& fs = axis == 1 ? p->fy : p->fx;
+  if (!fs.n) {
+    const ColCfg& c = axis == 1 ? p->cy[prec] : p->cx[prec];
+    return run_col_cfg(p, prec, axis, mode, src, dst, s, dc_out, lay_in, lay_out, outer0, nouter, c,
+                       axis == 1 ? p->tw_y[prec] : p->tw_x[prec], axis == 1 ? p->bl_y[prec] : p->bl_x[prec],
+                       nullptr);
+  }
+  if (lay_in != LAY_PLAIN || lay_out != LAY_PLAIN || nouter >= 0)
+    return set_error(FF_ERR_INVALID, "four-step column lines run over whole passes in the plain layout");
+  const long long n1 = fs.n1, n2 = fs.n2, nx = p->n[0], ny = p->n[1];
+  // geometry in rows: y lines (planes i, row j), x lines (planes j, row i*ny)
+  SubLines ga, gb;
+  if (axis == 1) {
+    ga = SubLines{(int)(nx * n1), (int)n1, 1, ny, n1, fs.twn[prec], 0};
+    gb = SubLines{(int)(nx * n2), (int)n2, n1, ny, 1, nullptr, 0};
+  } else {
+    ga = SubLines{(int)(ny * n1), (int)n1, ny, 1, n1 * ny, fs.twn[prec], 0};
+    gb = SubLines{(int)(ny * n2), (int)n2, n1 * ny, 1, ny, nullptr, (int)n2};
+  }
+  auto A = [&](int m, const void* in, void* out) {
+    return run_col_cfg(p, prec, axis, m, in, out, s, nullptr, LAY_PLAIN, LAY_PLAIN, 0, -1, fs.c2[prec], fs.tw2[prec],
+                       fs.bl2[prec], &ga);
+  };
+  auto B = [&](int m, const void* in, void* out, double* dc) {
+    return run_col_cfg(p, prec, axis, m, in, out, s, dc, LAY_PLAIN, LAY_PLAIN, 0, -1, fs.c1[prec], fs.tw1[prec],
+                       fs.bl1[prec], &gb);
+  };
+  ff_status st;
+  if (mode == COL_FWD) {
+    if ((st = A(COL_FWD, src, dst))) return st;
+    return B(COL_FWD, dst, dst, nullptr);
+  }
+  if (mode == COL_INV) {
+    if ((st = B(COL_INV, src, dst, nullptr))) return st;
+    return A(COL_INV, dst, dst);
+  }
+  // the x pass: forward, filter / symbol at the four-step frequencies, inverse
+  if ((st = A(COL_FWD, src, dst))) return st;
+  if ((st = B(mode, dst, dst, dc_out))) return st;
+  return A(COL_INV, dst, dst);
 }
 
 // The FP32 transform round trip of the solve and of each correction, from

@@ -48,6 +48,17 @@ struct ff_plan {
   double* dcf;  // sum of f (removed from the residual before it is rounded to FP32)
   // FP64 with FF_PRESERVE_INPUT: third work array (16*Nh bytes)
   void* work;
+  // Four-step column lines: an x or y extent n that no workgroup transforms
+  // whole runs as n1-point and n2-point sub-line passes, n = n1*n2.
+  struct FourStep {
+    int n, n1, n2;              // n = 0: the axis is transformed whole
+    ff::ColCfg c1[2], c2[2];    // n1-point and n2-point sub-lines (generic kernels)
+    const void* tw1[2];         // engine twiddles of the sub-lines
+    const void* tw2[2];
+    const void* twn[2];         // W_n, t < n
+    const void* bl1[2][3];      // Bluestein tables of the sub-lines
+    const void* bl2[2][3];
+  } fy, fx;
 };
 
 namespace ff {

@@ -454,10 +454,79 @@ def test_stamps_and_contract():
         plan.solve(nb, a, a)
     assert e.value.status == ff.FF_ERR_INVALID
     plan.close()
-    # a line too long for one workgroup's transform, even through Bluestein
+    # a prime x extent above the Bluestein limit: no whole-line plan and no
+    # four-step split
     with pytest.raises(ff.FishFryError) as e:
         ff.ParisPencil((5003, 4, 4), *ff.benchmark_grid((5003, 4, 4))[:2])
     assert e.value.status == ff.FF_ERR_UNSUPPORTED
+    assert "5003" in str(e.value)
+
+
+# x or y extents no workgroup transforms whole: four-step lines n = n1*n2
+# (4100 = 41*100: a Bluestein sub-line; 4097 = 17*241: two Bluestein
+# sub-lines; 8448 = 2^8*3*11; 70000 = 2^4*5^4*7, above the former 65535 bound;
+# 4400 x 4200: both axes; 9000: 5-smooth, but 300 threads of 30 elements
+# exceed a workgroup's register budget)
+FOUR_STEP_SHAPES = [(8, 4100, 6), (4100, 6, 8), (4097, 5, 4), (6, 8448, 5), (70000, 3, 4), (4400, 4200, 2),
+                    (6, 9000, 5)]
+
+
+@pytest.mark.parametrize("n", FOUR_STEP_SHAPES)
+def test_four_step_lines_match_oracle(n):
+    """The reference plans any line length (cufftPlanMany,
+    ParisPencil.cpp:78-81).  Extents too long for one workgroup's line run
+    as four-step sub-line passes; the x filter reads y frequencies in
+    four-step order.  FP64 solves match the oracle; FP32 + 2 steps converge
+    to it; the 3-point symbol indexes its tables by the true frequencies."""
+    rng = np.random.default_rng(n[0] + 7 * n[1] + 13 * n[2])
+    f = rng.standard_normal(n)
+    f -= f.mean()
+    lo, hi, _ = ff.benchmark_grid(n)
+    plan = ff.ParisPencil(n, lo, hi)
+    info = json.loads(plan.describe())
+    say("four-step", n, info["four_step"])
+    fs = info["four_step"]
+    if n[1] > 8192 or n[1] in (4100, 4200):
+        assert fs["y"][0] * fs["y"][1] == n[1] and fs["y"][0] > 1
+    if n[0] > 4096:
+        assert fs["x"][0] * fs["x"][1] == n[0] and fs["x"][0] > 1
+    plan.close()
+    want = oracle.solve(f)
+    got = gpu_solve(f)
+    assert rel(got, want) < TOL, rel(got, want)
+    got32 = gpu_solve(f, precision=ff.FF_FP32_REFINE, refine_steps=2)
+    assert rel(got32, want) < 1e-12, rel(got32, want)
+    if n[0] * n[1] * n[2] <= 200000:
+        fd = gpu_solve(f, flags=ff.FF_SYMBOL_3PT)
+        assert rel(fd, oracle.solve(f, symbol=3)) < TOL
+
+
+def test_four_step_lines_in_a_batch():
+    """A four-step instance in a step-batched batch falls back to its own
+    launches and matches its single-plan solve bit for bit."""
+    shapes = [(8, 4100, 6), (16, 16, 16), (4097, 5, 4)]
+    items = []
+    fs, outs = [], []
+    rng = np.random.default_rng(11)
+    for n in shapes:
+        lo, hi, _ = ff.benchmark_grid(n)
+        items.append({"n": n, "lo": lo, "hi": hi})
+        f = rng.standard_normal(n)
+        fs.append(f)
+        outs.append(gpu_solve(f))
+    b = ff.Batch(items, ff.FF_BATCH_STEPS)
+    dens, pots = [], []
+    for n, f in zip(shapes, fs):
+        nb = b.bytes(len(dens))
+        a = torch.zeros(nb // 8, dtype=torch.float64, device="cuda")
+        put(a, f)
+        dens.append(a)
+        pots.append(torch.zeros(nb // 8, dtype=torch.float64, device="cuda"))
+    b.solve(dens, pots)
+    torch.cuda.synchronize()
+    for k, n in enumerate(shapes):
+        assert np.array_equal(get_real(pots[k], n), outs[k]), n
+    b.close()
 
 
 @pytest.mark.parametrize("n", [(512, 512, 512), (768, 768, 768), (1000, 1000, 1000), (1024, 1024, 1024)])
