@@ -99,8 +99,10 @@ struct ff_server {
   std::atomic<bool> stop{false};
   std::thread worker;
   uint64_t next_id = 1;
-  // statistics (under mu)
+  // statistics (under mu); arena_peak: the scheduler thread's arena peak,
+  // published under mu (the arena itself belongs to that thread)
   std::vector<double> latencies;
+  size_t arena_peak = 0;
   double first_submit = 0, last_done = 0, points = 0;
   long long completed = 0, rejected = 0, dispatched_groups = 0;
   std::string fatal;
@@ -248,8 +250,9 @@ void dispatch(ff_server* s, std::vector<Req*>& group, int k) {
   s->stream_load[k]++;
   for (Req* q : g.reqs) g.work += work_of(q->r);
   s->stream_work[k] += g.work;
-  s->dispatched_groups++;
   s->inflight.push_back(g);
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->dispatched_groups++;
 }
 
 void complete(ff_server* s, Group& g) {
@@ -368,6 +371,7 @@ void worker_main(ff_server* s) {
       std::this_thread::sleep_for(std::chrono::microseconds(20));
     {
       std::lock_guard<std::mutex> lk(s->mu);
+      s->arena_peak = s->arena.peak();
       s->cv_done.notify_all();
     }
   }
@@ -538,7 +542,7 @@ ff_status ff_server_stats(ff_server* s, ff_serving_stats* out) {
   out->p99_s = pct(0.99);
   out->points_per_s = (s->last_done > s->first_submit) ? s->points / (s->last_done - s->first_submit) : 0.0;
   out->arena_bytes = s->arena.size();
-  out->arena_peak_bytes = s->arena.peak();
+  out->arena_peak_bytes = s->arena_peak;
   return FF_OK;
 }
 
