@@ -239,7 +239,7 @@ ff_status ff_tune_query(int kind, int n, int precision, int ncols, ff_tune* out)
       return set_error(FF_ERR_UNSUPPORTED, "ff_tune_query: no plan for that length");
     E = c.line.E;
     T = c.T;
-    flags = c.stat ? 0 : FF_TUNE_GENERIC;
+    flags = (c.stat ? 0 : FF_TUNE_GENERIC) | (c.stat == 2 ? FF_TUNE_LDS_TWIDDLES : 0);
   } else {
     ColCfg c;
     if (!make_col_cfg(n, ncols, eb, c, kind, precision))

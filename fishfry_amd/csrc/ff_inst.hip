@@ -119,14 +119,14 @@ hipError_t PassLaunchers<FF_E>::col(int f32, int mode, int split, int groups, in
   return launch(fn, l, &a, sizeof(a));
 }
 
-template <bool BL, int SN = 0>
+template <bool BL, int SN = 0, bool TWL = false>
 static const void* r2c_fn(int f32, int in_f32, int odd) {
   constexpr int E = FF_E;
   if constexpr (SN > 0) {  // compiled-in plans: even nz only
     if (odd) return nullptr;
-    if (!f32) return in_f32 ? nullptr : (const void*)&k_r2c<double, double, E, false, BL, SN>;
-    return in_f32 ? (const void*)&k_r2c<float, float, E, false, BL, SN>
-                  : (const void*)&k_r2c<float, double, E, false, BL, SN>;
+    if (!f32) return in_f32 ? nullptr : (const void*)&k_r2c<double, double, E, false, BL, SN, TWL>;
+    return in_f32 ? (const void*)&k_r2c<float, float, E, false, BL, SN, TWL>
+                  : (const void*)&k_r2c<float, double, E, false, BL, SN, TWL>;
   }
   if (!f32) {
     if (in_f32) return nullptr;
@@ -137,15 +137,17 @@ static const void* r2c_fn(int f32, int in_f32, int odd) {
   return odd ? (const void*)&k_r2c<float, float, E, true, BL> : (const void*)&k_r2c<float, float, E, false, BL>;
 }
 
-// Row kernels with the plan of a kStaticRows entry of this E compiled in.
+// Row kernels with the plan of a kStaticRows entry of this E compiled in
+// (stat 2: twiddles from LDS).
 template <int K>
-static const void* r2c_static_fn(int f32, int in_f32, int odd, int M) {
+static const void* r2c_static_fn(int f32, int in_f32, int odd, int M, int stat) {
   if constexpr (K < kNumStaticRows) {
     constexpr StaticRow sr = kStaticRows[K];
     if constexpr (sr.E == FF_E) {
-      if (sr.M == M) return r2c_fn<false, sr.M>(f32, in_f32, odd);
+      if (sr.M == M)
+        return stat == 2 ? r2c_fn<false, sr.M, true>(f32, in_f32, odd) : r2c_fn<false, sr.M>(f32, in_f32, odd);
     }
-    return r2c_static_fn<K + 1>(f32, in_f32, odd, M);
+    return r2c_static_fn<K + 1>(f32, in_f32, odd, M, stat);
   } else {
     return nullptr;
   }
@@ -154,20 +156,20 @@ static const void* r2c_static_fn(int f32, int in_f32, int odd, int M) {
 template <>
 hipError_t PassLaunchers<FF_E>::r2c(int f32, int in_f32, int odd, int blue, int stat, const Launch& l,
                                     const RowArgs& a) {
-  const void* fn = stat ? r2c_static_fn<0>(f32, in_f32, odd, a.lp.n)
+  const void* fn = stat ? r2c_static_fn<0>(f32, in_f32, odd, a.lp.n, stat)
                         : (blue ? r2c_fn<true>(f32, in_f32, odd) : r2c_fn<false>(f32, in_f32, odd));
   if (!fn) return hipErrorInvalidValue;
   return launch(fn, l, &a, sizeof(a));
 }
 
 // out_kind: 0 = double output, 1 = float output.
-template <bool BL, int SN = 0>
+template <bool BL, int SN = 0, bool TWL = false>
 static const void* c2r_fn(int f32, int out_kind, int odd, int epi) {
   constexpr int E = FF_E;
   // (compiled-in plans: even nz only)
 #define FF_PICK(F, OUT, EPI)                                                                           \
   if constexpr (SN > 0)                                                                                \
-    return odd ? nullptr : (const void*)&k_c2r<F, OUT, E, false, EPI, BL, SN>;                         \
+    return odd ? nullptr : (const void*)&k_c2r<F, OUT, E, false, EPI, BL, SN, TWL>;                    \
   else                                                                                                 \
     return odd ? (const void*)&k_c2r<F, OUT, E, true, EPI, BL> : (const void*)&k_c2r<F, OUT, E, false, EPI, BL>
   if (!f32) {
@@ -183,13 +185,15 @@ static const void* c2r_fn(int f32, int out_kind, int odd, int epi) {
 }
 
 template <int K>
-static const void* c2r_static_fn(int f32, int out_kind, int odd, int epi, int M) {
+static const void* c2r_static_fn(int f32, int out_kind, int odd, int epi, int M, int stat) {
   if constexpr (K < kNumStaticRows) {
     constexpr StaticRow sr = kStaticRows[K];
     if constexpr (sr.E == FF_E) {
-      if (sr.M == M) return c2r_fn<false, sr.M>(f32, out_kind, odd, epi);
+      if (sr.M == M)
+        return stat == 2 ? c2r_fn<false, sr.M, true>(f32, out_kind, odd, epi)
+                         : c2r_fn<false, sr.M>(f32, out_kind, odd, epi);
     }
-    return c2r_static_fn<K + 1>(f32, out_kind, odd, epi, M);
+    return c2r_static_fn<K + 1>(f32, out_kind, odd, epi, M, stat);
   } else {
     return nullptr;
   }
@@ -198,7 +202,7 @@ static const void* c2r_static_fn(int f32, int out_kind, int odd, int epi, int M)
 template <>
 hipError_t PassLaunchers<FF_E>::c2r(int f32, int out_kind, int odd, int epi, int blue, int stat, const Launch& l,
                                     const RowArgs& a) {
-  const void* fn = stat ? c2r_static_fn<0>(f32, out_kind, odd, epi, a.lp.n)
+  const void* fn = stat ? c2r_static_fn<0>(f32, out_kind, odd, epi, a.lp.n, stat)
                         : (blue ? c2r_fn<true>(f32, out_kind, odd, epi) : c2r_fn<false>(f32, out_kind, odd, epi));
   if (!fn) return hipErrorInvalidValue;
   return launch(fn, l, &a, sizeof(a));

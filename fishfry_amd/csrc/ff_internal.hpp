@@ -89,7 +89,8 @@ struct RowCfg {
   size_t lds = 0;
   int threads = 0;
   uint32_t Linv = 0;
-  int stat = 0;  // a kStaticRows kernel runs this configuration (FF_TUNE_GENERIC clears it)
+  int stat = 0;  // a kStaticRows kernel runs this configuration (FF_TUNE_GENERIC clears it);
+                 // 2: the one that reads its twiddles from LDS (FF_TUNE_LDS_TWIDDLES)
 };
 
 // kind: 0 z forward rows, 1 y columns, 2 x columns, 3 z inverse rows;

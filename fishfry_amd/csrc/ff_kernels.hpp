@@ -551,7 +551,17 @@ __device__ __forceinline__ long long spec_row(const RowArgs& a, long long row) {
 // SN > 0: the plan of the row transform, length SN, is compiled in (even nz).
 // r2c_tile: the work of one workgroup, row block blk (< a.nblocks), with the
 // per-instance fields io (k_r2c: one pass; ff_multi.hip: many instances).
-template <typename F, typename IN, int E, bool ODD, bool BL, int SN>
+// TWL (compiled-in plans): the first half of the W_SN table is copied to LDS
+// after the T row tiles and the passes read their twiddles there.
+template <typename F, int SN>
+__device__ __forceinline__ TwHalf<F, SN / 2> row_twiddles_to_lds(const RowArgs& a, cx<F>* lds) {
+  cx<F>* twl = lds + (long long)a.T * a.pitch;
+  const cx<F>* __restrict__ twg = reinterpret_cast<const cx<F>*>(a.tw);
+  for (int i = threadIdx.x; i < SN / 2; i += blockDim.x) twl[tw_swizzle(i)] = twg[i];
+  return TwHalf<F, SN / 2>{twl};
+}
+
+template <typename F, typename IN, int E, bool ODD, bool BL, int SN, bool TWL = false>
 __device__ __forceinline__ void r2c_tile(const RowArgs& a, const PassInst& io, int blk) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
@@ -584,7 +594,14 @@ __device__ __forceinline__ void r2c_tile(const RowArgs& a, const PassInst& io, i
         v[i] = cx<F>{F(0), F(0)};
       }
     }
-    line_dft<E, -1, F, RowIdx, false, BL, SN>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    if constexpr (TWL) {
+      static_assert(SN > 0 && !BL, "LDS twiddles: compiled-in plans");
+      const TwHalf<F, SN / 2> twl = row_twiddles_to_lds<F, SN>(a, lds);
+      __syncthreads();
+      line_dft<E, -1, F, RowIdx, false, BL, SN>(v, a.lp, M, twl, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    } else {
+      line_dft<E, -1, F, RowIdx, false, BL, SN>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    }
     __syncthreads();
     put_natural<E, F>(v, lds, idx, r0, L);
     __syncthreads();
@@ -626,9 +643,9 @@ __device__ __forceinline__ void r2c_tile(const RowArgs& a, const PassInst& io, i
   }
 }
 
-template <typename F, typename IN, int E, bool ODD, bool BL, int SN = 0>
+template <typename F, typename IN, int E, bool ODD, bool BL, int SN = 0, bool TWL = false>
 __global__ __launch_bounds__(max_threads_for(E)) void k_r2c(const RowArgs a) {
-  r2c_tile<F, IN, E, ODD, BL, SN>(a, inst_of(a), xcd_remap(blockIdx.x, a.nblocks));
+  r2c_tile<F, IN, E, ODD, BL, SN, TWL>(a, inst_of(a), xcd_remap(blockIdx.x, a.nblocks));
 }
 
 // Epilogues of the z inverse: store scale*x, store the residual
@@ -669,7 +686,7 @@ __device__ __forceinline__ void c2r_out2(FF_GLB OUT* out, const FF_GLB double* f
 // SN > 0: the plan of the row transform, length SN, is compiled in (even nz).
 // c2r_tile: the work of one workgroup, row block blk (< a.nblocks), with the
 // per-instance fields io (k_c2r: one pass; ff_multi.hip: many instances).
-template <typename F, typename OUT, int E, bool ODD, int EPI, bool BL, int SN>
+template <typename F, typename OUT, int E, bool ODD, int EPI, bool BL, int SN, bool TWL = false>
 __device__ __forceinline__ void c2r_tile(const RowArgs& a, const PassInst& io, int blk) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* lds = reinterpret_cast<cx<F>*>(smem_raw);
@@ -749,6 +766,11 @@ __device__ __forceinline__ void c2r_tile(const RowArgs& a, const PassInst& io, i
           src[(long long)kb * a.spec.KS + r * W + kt];
     }
   }
+  TwHalf<F, (SN > 0 ? SN / 2 : 1)> twl{nullptr};
+  if constexpr (TWL) {
+    static_assert(SN > 0 && !BL, "LDS twiddles: compiled-in plans");
+    twl = row_twiddles_to_lds<F, SN>(a, lds);
+  }
   __syncthreads();
   cx<F> v[E];
   if constexpr (!ODD) {
@@ -784,7 +806,11 @@ __device__ __forceinline__ void c2r_tile(const RowArgs& a, const PassInst& io, i
       v[i] = y;
     }
   }
-  line_dft<E, +1, F, RowIdx, false, BL, SN>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+  if constexpr (TWL) {
+    line_dft<E, +1, F, RowIdx, false, BL, SN>(v, a.lp, M, twl, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+  } else {
+    line_dft<E, +1, F, RowIdx, false, BL, SN>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+  }
   if (valid) {
     const double shift = io.shiftp ? a.shift * *glb<const double>(io.shiftp) : a.shift;
     FF_GLB OUT* out = glb<OUT>(io.dst) + row * a.nz;
@@ -802,9 +828,9 @@ __device__ __forceinline__ void c2r_tile(const RowArgs& a, const PassInst& io, i
   }
 }
 
-template <typename F, typename OUT, int E, bool ODD, int EPI, bool BL, int SN = 0>
+template <typename F, typename OUT, int E, bool ODD, int EPI, bool BL, int SN = 0, bool TWL = false>
 __global__ __launch_bounds__(max_threads_for(E)) void k_c2r(const RowArgs a) {
-  c2r_tile<F, OUT, E, ODD, EPI, BL, SN>(a, inst_of(a), xcd_remap(blockIdx.x, a.nblocks));
+  c2r_tile<F, OUT, E, ODD, EPI, BL, SN, TWL>(a, inst_of(a), xcd_remap(blockIdx.x, a.nblocks));
 }
 
 // ---------------------------------------------------------------------------

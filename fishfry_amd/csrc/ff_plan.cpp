@@ -505,13 +505,19 @@ bool row_cfg_with(int nz, int elem_bytes, RowCfg& out, int E, int T, int flags) 
   out.stat = 0;
   if (!(flags & FF_TUNE_GENERIC) && nz % 2 == 0 && !out.line.bluestein() && out.ps == 4)
     out.stat = static_row_plan_matches<0>(M, out.line.E, out.line.lp) ? 1 : 0;
+  if (flags & FF_TUNE_LDS_TWIDDLES) {
+    // the first half of the W_M table follows the row tiles in LDS
+    if (out.stat != 1 || (flags & FF_TUNE_GENERIC)) return false;
+    out.stat = 2;
+    out.lds += (size_t)(M / 2) * elem_bytes;
+  }
   return out.lds <= 160 * 1024;
 }
 
 // Configurations measured fastest on MI355X where the default heuristic
 // picks a slower one: elements per thread at 768^3 and 1000^3 (FP64), and
-// the kernels with LDS twiddles at 1024 (FP64: 19.7 -> 18.4 ms per 1024^3
-// solve, FP32: 12.3 -> 11.0 ms); with the pencil layout the y passes at 1024
+// the kernels with LDS twiddles at 1024 (column passes, FP64: 19.7 -> 18.4 ms
+// per 1024^3 solve, FP32: 12.3 -> 11.0 ms); with the pencil layout the y passes at 1024
 // also take the split exchange (three workgroups per CU: 18.4 -> 17.3 ms;
 // in the plain layout the same kernel is slower).  lay: -1 any layout, 0
 // plain, 1 pencil.  User overrides (ff_tune_set) take precedence.
@@ -519,6 +525,13 @@ struct BuiltinCfg {
   int kind, n, prec, lay, E, T, flags;
 };
 static const BuiltinCfg kBuiltin[] = {
+    // z rows, nz = 1024: LDS twiddles (FP64 z forward 3.10 -> 2.86 ms, z
+    // inverse 3.12 -> 2.97 ms plain, 3.53 -> 3.44 ms pencil; FP32 solve
+    // 11.0 -> 10.6 ms)
+    {0, 512, 0, -1, 8, 2, FF_TUNE_LDS_TWIDDLES},
+    {3, 512, 0, -1, 8, 2, FF_TUNE_LDS_TWIDDLES},
+    {0, 512, 1, -1, 8, 4, FF_TUNE_LDS_TWIDDLES},
+    {3, 512, 1, -1, 8, 4, FF_TUNE_LDS_TWIDDLES},
     {0, 384, 0, -1, 12, 4, 0},   // z forward, nz = 768
     {3, 384, 0, -1, 12, 4, 0},   // z inverse, nz = 768
     {1, 768, 0, -1, 24, 4, 0},   // y, ny = 768
@@ -576,7 +589,8 @@ ff_status tune_set(const ff_tune* t) {
   bool ok;
   if (t->kind == 0 || t->kind == 3) {
     RowCfg c;
-    ok = (t->flags & ~FF_TUNE_GENERIC) == 0 && row_cfg_with(2 * t->n, eb, c, t->E, t->T, t->flags);
+    ok = (t->flags & ~(FF_TUNE_GENERIC | FF_TUNE_LDS_TWIDDLES)) == 0 &&
+         row_cfg_with(2 * t->n, eb, c, t->E, t->T, t->flags);
   } else {
     ColCfg c;
     ok = col_cfg_with(t->n, 1 << 20, eb, c, t->E, t->T, t->flags, t->kind);

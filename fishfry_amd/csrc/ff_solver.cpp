@@ -201,7 +201,8 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     const int eb = mainp ? 8 : 16;
     for (int T : {128 / eb, 64 / eb}) {
       RowCfg c;
-      if (row_cfg_with(n[2], eb, c, zi.line.E, T, zi.stat ? 0 : FF_TUNE_GENERIC) &&
+      const int fl = zi.stat == 2 ? FF_TUNE_LDS_TWIDDLES : (zi.stat ? 0 : FF_TUNE_GENERIC);
+      if (row_cfg_with(n[2], eb, c, zi.line.E, T, fl) &&
           (T == 64 / eb || c.lds <= 80 * 1024)) {
         zi = c;
         break;

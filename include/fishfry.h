@@ -183,7 +183,7 @@ typedef struct ff_tune {
   int precision;
   int E;
   int T;
-  int flags; /* FF_TUNE_SPLIT_LDS | FF_TUNE_TWO_GROUPS | FF_TUNE_LDS_TWIDDLES (column passes) | FF_TUNE_GENERIC */
+  int flags; /* FF_TUNE_SPLIT_LDS | FF_TUNE_TWO_GROUPS (column passes) | FF_TUNE_LDS_TWIDDLES | FF_TUNE_GENERIC */
 } ff_tune;
 
 /* ff_tune.flags (column passes):
@@ -199,10 +199,10 @@ typedef struct ff_tune {
  *                     (ff_tune_query reports FF_TUNE_GENERIC when no such
  *                     kernel exists for the configuration)                 */
 #define FF_TUNE_GENERIC 4
-/* FF_TUNE_LDS_TWIDDLES column passes with a compiled-in plan: the workgroup
- *                     copies half the twiddle table to LDS and the transforms
- *                     read it there instead of through the vector memory path
- *                     (more LDS per workgroup; results unchanged)          */
+/* FF_TUNE_LDS_TWIDDLES column or row passes with a compiled-in plan: the
+ *                     workgroup copies half the twiddle table to LDS and the
+ *                     transforms read it there instead of through the vector
+ *                     memory path (more LDS per workgroup; results unchanged) */
 #define FF_TUNE_LDS_TWIDDLES 8
 
 ff_status ff_tune_set(const ff_tune* t);

@@ -102,7 +102,10 @@ int main() {
         }
         ff::RowCfg r;
         if (ff::make_row_cfg(n, eb, r, kind == 1 ? 0 : 3, eb == 8)) {
-          REQUIRE(r.T >= 1 && r.threads == r.T * r.line.lp.L && r.lds == (size_t)r.T * r.pitch * eb);
+          // the row tiles, then half the twiddle table where it is read from LDS
+          const size_t twl = r.stat == 2 ? (size_t)(r.line.lp.n / 2) * eb : 0;
+          REQUIRE(r.T >= 1 && r.threads == r.T * r.line.lp.L && r.lds == (size_t)r.T * r.pitch * eb + twl);
+          REQUIRE(r.stat != 2 || n == 1024);
           REQUIRE(r.lds <= 160 * 1024 && r.threads <= 1024);
         }
       }

@@ -178,16 +178,29 @@ def test_tuning_host_logic_without_gpu_work():
         with pytest.raises(ff.FishFryError):
             ff.tune_set(1, 1024, 0, 16, 4, TWL | ff.FF_TUNE_GENERIC)
         with pytest.raises(ff.FishFryError):
-            ff.tune_set(0, 512, 0, 8, 2, ff.FF_TUNE_SPLIT_LDS)  # row passes take FF_TUNE_GENERIC only
+            ff.tune_set(0, 512, 0, 8, 2, ff.FF_TUNE_SPLIT_LDS)  # row passes: FF_TUNE_GENERIC or LDS twiddles
+        with pytest.raises(ff.FishFryError):
+            ff.tune_set(0, 300, 0, 10, 2, TWL)  # LDS twiddles need a row kernel with the plan compiled in
+        with pytest.raises(ff.FishFryError):
+            ff.tune_set(3, 512, 0, 8, 2, TWL | ff.FF_TUNE_GENERIC)
+        ff.tune_set(3, 256, 1, 16, 4, TWL)  # on request at other compiled-in row lengths
+        t = ff.tune_query(3, 256, 1, 1)
+        assert (t.E, t.T, t.flags) == (16, 4, TWL)
+        ff.tune_set(0, 512, 0, 8, 2, 0)  # the row kernel with twiddles from memory, on request
+        assert ff.tune_query(0, 512, 0, 1).flags == 0
         ff.tune_reset()
         assert ff.tune_query(2, 1024, 0, 513).flags == ff.FF_TUNE_SPLIT_LDS | TWL
         # row passes: compiled-in plans at the default configurations of even
-        # nz = 32 ... 1024 (length nz/2), generic otherwise or on request
+        # nz = 32 ... 1024 (length nz/2), generic otherwise or on request; at
+        # nz = 1024 the twiddles come from LDS (FP64 two rows per workgroup,
+        # FP32 four)
         for kind in (0, 3):
             for M, E in ((512, 8), (256, 16), (80, 20), (16, 16)):
                 for prec in (0, 1):
                     t = ff.tune_query(kind, M, prec, 1)
-                    assert (t.E, t.flags) == (E, 0), (kind, M, prec)
+                    assert (t.E, t.flags) == (E, TWL if M == 512 else 0), (kind, M, prec)
+                    if M == 512:
+                        assert t.T == (4 if prec else 2)
             assert ff.tune_query(kind, 450, 0, 1).flags == ff.FF_TUNE_GENERIC
             ff.tune_set(kind, 512, 0, 8, 2, ff.FF_TUNE_GENERIC)
             assert ff.tune_query(kind, 512, 0, 1).flags == ff.FF_TUNE_GENERIC

@@ -317,6 +317,43 @@ def test_column_variants_agree(n, E, variant):
         ff.tune_reset()
 
 
+@pytest.mark.parametrize("layout", [ff.FF_PLAIN_LAYOUT, ff.FF_PENCIL_LAYOUT])
+@pytest.mark.parametrize("n,E,T", [((6, 5, 1024), 8, 2), ((5, 7, 1024), 8, 8), ((6, 5, 1024), 8, 1),
+                                   ((8, 6, 512), 16, 0), ((5, 4, 768), 12, 4), ((5, 4, 768), 24, 0),
+                                   ((6, 6, 1000), 10, 0), ((7, 9, 160), 20, 0), ((12, 10, 32), 16, 0),
+                                   ((10, 9, 48), 12, 0), ((9, 11, 128), 8, 3)])
+def test_row_variants_agree(n, E, T, layout):
+    """The z passes (both directions) with twiddles read from a half-table LDS
+    copy, and with the generic kernel, move the same values as the kernel
+    with the row plan compiled in and twiddles from memory: bit-identical
+    solves (FP64, and FP32 + one refinement step, whose residual passes run
+    FP64 rows too) that match the oracle, in the plain and the pencil spectrum
+    layout (the pencil z inverse stages its rows from a strided spectrum).
+    T = 0 keeps the default rows per workgroup."""
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal(n)
+    M = n[2] // 2
+    try:
+        for prec, steps in ((ff.FF_FP64, 0), (ff.FF_FP32_REFINE, 1)):
+            pi = 0 if prec == ff.FF_FP64 else 1
+            outs = []
+            for flags in (0, ff.FF_TUNE_LDS_TWIDDLES, ff.FF_TUNE_GENERIC):
+                ff.tune_reset()
+                for kind in (0, 3):
+                    for p in sorted({0, pi}):
+                        cur = ff.tune_query(kind, M, p, 1)
+                        ff.tune_set(kind, M, p, E, T or cur.T, flags)
+                        got = ff.tune_query(kind, M, p, 1)
+                        assert (got.E, got.flags) == (E, flags), (kind, p, got.E, got.flags)
+                say("row variant", n, E, T, layout, prec, flags)
+                outs.append(gpu_solve(f, precision=prec, refine_steps=steps, flags=layout))
+            assert np.array_equal(outs[1], outs[0]) and np.array_equal(outs[2], outs[0])
+            want = oracle.solve(f)
+            assert rel(outs[0], want) < (TOL if prec == ff.FF_FP64 else 1e-9)
+    finally:
+        ff.tune_reset()
+
+
 @pytest.mark.parametrize("n,slab", [((10, 12, 16), 3), ((64, 48, 40), 5), ((30, 25, 27), 7), ((9, 8, 12), 1),
                                     ((96, 160, 48), 0), ((12, 10, 9), 12)])
 def test_slab_pipeline_bitwise(n, slab):
