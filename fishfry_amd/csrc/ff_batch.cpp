@@ -584,6 +584,15 @@ ff_status ff_batch_create(int count, const ff_batch_item* items, int flags, ff_b
   b->flags = flags;
   b->items.assign(items, items + count);
   ff_status st = make_plans(b);
+  // the grouped and step modes replay recorded pass launches; four-step z
+  // rows also run row-local steps that are not recorded
+  if (st == FF_OK && (flags & (FF_BATCH_GROUPED | FF_BATCH_STEPS)))
+    for (const ff_plan* p : b->plans)
+      if (p->fz.n) {
+        st = set_error(FF_ERR_UNSUPPORTED,
+                       "ff_batch_create: grids whose z rows run in four steps take the stream-parallel mode");
+        break;
+      }
   const int ns = std::min(kBatchStreams, count);
   for (int k = 0; k < ns && st == FF_OK; ++k) {
     hipStream_t s = nullptr;
@@ -691,8 +700,8 @@ ff_status ff_batch_autotune(ff_batch* b, double* const* density, double* const* 
       const int M = p->n[2] % 2 ? p->n[2] : p->n[2] / 2;
       const int keys[4][2] = {{0, M}, {1, p->n[1]}, {2, p->n[0]}, {3, M}};
       for (auto& kv : keys) {
-        // four-step column lines have no single configuration to tune
-        if ((kv[0] == 1 && p->fy.n) || (kv[0] == 2 && p->fx.n)) continue;
+        // four-step lines have no single configuration to tune
+        if ((kv[0] == 1 && p->fy.n) || (kv[0] == 2 && p->fx.n) || ((kv[0] == 0 || kv[0] == 3) && p->fz.n)) continue;
         if (!seen.count({kv[0], kv[1], prec})) seen[{kv[0], kv[1], prec}] = i;
       }
     }

@@ -168,9 +168,9 @@ ff_status ff_plan_describe(const ff_plan* p, char* buf, size_t len) {
   s += tmp;
   snprintf(tmp, sizeof tmp,
            "\"dd\":[%.17g,%.17g,%.17g],\"kt\":%d,\"slab\":%d,\"symbol\":%d,\"pencil\":%d,\"four_step\":"
-           "{\"y\":[%d,%d],\"x\":[%d,%d]},",
+           "{\"y\":[%d,%d],\"x\":[%d,%d],\"z\":[%d,%d]},",
            p->ddi, p->ddj, p->ddk, p->lgKT ? 1 << p->lgKT : 0, p->slab, p->sym, p->pencil, p->fy.n1, p->fy.n2,
-           p->fx.n1, p->fx.n2);
+           p->fx.n1, p->fx.n2, p->fz.n1, p->fz.n2);
   s += tmp;
   auto line = [&](const char* key, const LineCfg& c, int T, size_t lds, int threads) {
     std::string r = "\"";
@@ -186,14 +186,21 @@ ff_status ff_plan_describe(const ff_plan* p, char* buf, size_t len) {
     r += "]}";
     return r;
   };
-  s += line("z", p->zr[0].line, p->zr[0].T, p->zr[0].lds, p->zr[0].threads) + ",";
-  s += line("zi", p->zi[0].line, p->zi[0].T, p->zi[0].lds, p->zi[0].threads) + ",";
   // four-step axes: the n1-point and n2-point sub-line configurations
-  auto axis = [&](const char* key, const ff_plan::FourStep& fs, const ColCfg& c) {
-    if (!fs.n) return line(key, c.line, c.T, c.lds, c.threads);
+  auto sub = [&](const char* key, const ff_plan::FourStep& fs) {
     std::string k1 = std::string(key) + "1", k2 = std::string(key) + "2";
     return line(k1.c_str(), fs.c1[0].line, fs.c1[0].T, fs.c1[0].lds, fs.c1[0].threads) + "," +
            line(k2.c_str(), fs.c2[0].line, fs.c2[0].T, fs.c2[0].lds, fs.c2[0].threads);
+  };
+  if (p->fz.n) {
+    s += sub("z", p->fz) + ",";
+  } else {
+    s += line("z", p->zr[0].line, p->zr[0].T, p->zr[0].lds, p->zr[0].threads) + ",";
+    s += line("zi", p->zi[0].line, p->zi[0].T, p->zi[0].lds, p->zi[0].threads) + ",";
+  }
+  auto axis = [&](const char* key, const ff_plan::FourStep& fs, const ColCfg& c) {
+    if (!fs.n) return line(key, c.line, c.T, c.lds, c.threads);
+    return sub(key, fs);
   };
   s += axis("y", p->fy, p->cy[0]) + ",";
   s += axis("x", p->fx, p->cx[0]);

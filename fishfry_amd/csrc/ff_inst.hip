@@ -168,18 +168,21 @@ static const void* c2r_fn(int f32, int out_kind, int odd, int epi) {
   constexpr int E = FF_E;
   // (compiled-in plans: even nz only)
 #define FF_PICK(F, OUT, EPI)                                                                           \
-  if constexpr (SN > 0)                                                                                \
-    return odd ? nullptr : (const void*)&k_c2r<F, OUT, E, false, EPI, BL, SN, TWL>;                    \
-  else                                                                                                 \
-    return odd ? (const void*)&k_c2r<F, OUT, E, true, EPI, BL> : (const void*)&k_c2r<F, OUT, E, false, EPI, BL>
+  {                                                                                                    \
+    if constexpr (SN > 0)                                                                              \
+      return odd ? nullptr : (const void*)&k_c2r<F, OUT, E, false, EPI, BL, SN, TWL>;                  \
+    else                                                                                               \
+      return odd ? (const void*)&k_c2r<F, OUT, E, true, EPI, BL>                                       \
+                 : (const void*)&k_c2r<F, OUT, E, false, EPI, BL>;                                     \
+  }
   if (!f32) {
-    if (epi == EPI_STORE && out_kind == 0) FF_PICK(double, double, EPI_STORE);
-    if (epi == EPI_RESIDUAL && out_kind == 1) FF_PICK(double, float, EPI_RESIDUAL);
-    if (epi == EPI_RESIDUAL && out_kind == 0) FF_PICK(double, double, EPI_RESIDUAL);
+    if (epi == EPI_STORE && out_kind == 0) FF_PICK(double, double, EPI_STORE)
+    if (epi == EPI_RESIDUAL && out_kind == 1) FF_PICK(double, float, EPI_RESIDUAL)
+    if (epi == EPI_RESIDUAL && out_kind == 0) FF_PICK(double, double, EPI_RESIDUAL)
     return nullptr;
   }
-  if (epi == EPI_STORE && out_kind == 0) FF_PICK(float, double, EPI_STORE);
-  if (epi == EPI_ACCUM && out_kind == 0) FF_PICK(float, double, EPI_ACCUM);
+  if (epi == EPI_STORE && out_kind == 0) FF_PICK(float, double, EPI_STORE)
+  if (epi == EPI_ACCUM && out_kind == 0) FF_PICK(float, double, EPI_ACCUM)
   return nullptr;
 #undef FF_PICK
 }

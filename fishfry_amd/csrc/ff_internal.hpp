@@ -125,6 +125,15 @@ hipError_t replay_launch(const PassLaunch& r);
 hipError_t launch_r2c(int E, int f32, int in_f32, int odd, int blue, int stat, const Launch& l, const RowArgs& a);
 hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, int stat, const Launch& l,
                       const RowArgs& a);
+// whether a recorder is installed on the calling thread
+bool launch_recording();
+
+// Row-local steps of z passes over four-step rows (ff_zsplit.hip; FP64):
+// the forward split (in place on io.dst), the inverse merge (src -> dst) and
+// the inverse epilogue (packed rows src -> real dst).  a.nline = M = nz/2.
+// Never recorded: refused while a recorder is installed.
+enum { ZSPLIT_FWD = 0, ZSPLIT_INV = 1, ZSPLIT_EPI = 2 };
+hipError_t launch_zsplit(int step, int f32, int out_kind, int epi, const RowArgs& a, hipStream_t s);
 
 // ---- driver kernels ---------------------------------------------------------
 ff_status field_check(const ff_field* f);

@@ -653,6 +653,7 @@ FF_DECL(30)
 static thread_local std::vector<PassLaunch>* t_recorder = nullptr;
 
 void set_launch_recorder(std::vector<PassLaunch>* rec) { t_recorder = rec; }
+bool launch_recording() { return t_recorder != nullptr; }
 
 static hipError_t do_col(int E, int f32, int mode, int split, int groups, int blue, int stat, const Launch& l,
                          const ColArgs& a) {

@@ -49,8 +49,11 @@ static double dd_const(int n, double lo, double hi) {
 // n = n1*n2 (n1 <= n2), both sub-line lengths planned (generic kernels,
 // Bluestein allowed), the split with the least work n2*cost(n1) +
 // n1*cost(n2), cost = engine length * (passes + 0.04 E).  False when n has
-// no such split (a prime above the Bluestein limit).
-static bool four_step_cfg(int n, int ncols, int eb, int kind, int prec, ff_plan::FourStep& fs) {
+// no such split (a prime above the Bluestein limit).  ncols: the columns of
+// the sub-line passes (x, y: nh); z rows (zrows): the n1-point lines run over
+// n2 columns and the n2-point ones over n1.
+static bool four_step_cfg(int n, int ncols, int eb, int kind, int prec, ff_plan::FourStep& fs,
+                          bool zrows = false) {
   auto cost = [](const ColCfg& c) { return double(c.line.lp.n) * (double(c.line.lp.np) + 0.04 * c.line.E); };
   double best = 1e300;
   int b1 = 0;
@@ -59,8 +62,8 @@ static bool four_step_cfg(int n, int ncols, int eb, int kind, int prec, ff_plan:
     if (n % n1) continue;
     const int n2 = n / n1;
     ColCfg c1, c2;
-    if (!col_cfg_with(n1, ncols, eb, c1, 0, 0, FF_TUNE_GENERIC, kind) ||
-        !col_cfg_with(n2, ncols, eb, c2, 0, 0, FF_TUNE_GENERIC, kind))
+    if (!col_cfg_with(n1, zrows ? n2 : ncols, eb, c1, 0, 0, FF_TUNE_GENERIC, kind) ||
+        !col_cfg_with(n2, zrows ? n1 : ncols, eb, c2, 0, 0, FF_TUNE_GENERIC, kind))
       continue;
     const double w = double(n2) * cost(c1) + double(n1) * cost(c2);
     if (w < best) {
@@ -151,21 +154,31 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   }
   const int nprec = (o.precision == FF_FP64) ? 1 : 2;
   // x or y extents that no workgroup transforms whole: four-step lines
-  p->fy.n = p->fx.n = 0;
+  p->fy.n = p->fx.n = p->fz.n = 0;
   for (int prec = 0; prec < nprec; ++prec) {
     const int eb = prec ? 8 : 16;
     ColCfg c;
     if (!make_col_cfg(n[1], p->nh, eb, c, 1, prec)) four_step_cfg(n[1], p->nh, eb, 1, prec, p->fy);
     if (!make_col_cfg(n[0], p->nh, eb, c, 2, prec)) four_step_cfg(n[0], p->nh, eb, 2, prec, p->fx);
+    // even nz whose packed rows (nz/2 complex values) no workgroup
+    // transforms whole: four-step rows
+    RowCfg r0, r1;
+    if (n[2] % 2 == 0 && (!make_row_cfg(n[2], eb, r0, 0, prec) || !make_row_cfg(n[2], eb, r1, 3, prec)))
+      four_step_cfg(n[2] / 2, 0, eb, 1, prec, p->fz, true);
   }
-  const bool four = p->fy.n || p->fx.n;
+  const bool four = p->fy.n || p->fx.n || p->fz.n;
   if (four) {
     if (o.flags & (FF_PENCIL_LAYOUT | FF_BLOCKED_LAYOUT | FF_SLAB_PIPELINE)) {
       delete p;
-      return set_error(FF_ERR_UNSUPPORTED,
-                       "x or y extents transformed in four steps take the plain spectrum layout only");
+      return set_error(FF_ERR_UNSUPPORTED, "extents transformed in four steps take the plain spectrum layout only");
     }
     p->pencil = 0;
+  }
+  if (p->fz.n && o.precision != FF_FP64) {
+    delete p;
+    char buf[160];
+    snprintf(buf, sizeof buf, "nz = %d: z rows transformed in four steps run in FP64 plans only", n[2]);
+    return set_error(FF_ERR_UNSUPPORTED, buf);
   }
   for (int prec = 0; prec < nprec; ++prec) {
     const int eb = prec ? 8 : 16;
@@ -178,8 +191,9 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
       const int fl = FF_TUNE_GENERIC | (c.split ? FF_TUNE_SPLIT_LDS : 0) | (c.groups == 2 ? FF_TUNE_TWO_GROUPS : 0);
       return col_cfg_with(n[0], p->nh, eb, c, c.line.E, c.T, fl, 2);
     };
-    if (!make_row_cfg(n[2], eb, p->zr[prec], 0, prec) || !make_row_cfg(n[2], eb, p->zi[prec], 3, prec) ||
-        !(p->fy.n || make_col_cfg(n[1], p->nh, eb, p->cy[prec], 1, prec, p->pencil && prec == mainp)) ||
+    const bool z_ok = p->fz.n ? p->fz.c1[prec].threads > 0
+                              : make_row_cfg(n[2], eb, p->zr[prec], 0, prec) && make_row_cfg(n[2], eb, p->zi[prec], 3, prec);
+    if (!z_ok || !(p->fy.n || make_col_cfg(n[1], p->nh, eb, p->cy[prec], 1, prec, p->pencil && prec == mainp)) ||
         !x_cfg(p->cx[prec]) || (p->fy.n && !p->fy.c1[prec].threads) || (p->fx.n && !p->fx.c1[prec].threads)) {
       delete p;
       char buf[160];
@@ -213,8 +227,11 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   // one workgroup per tile: every pass's grid must fit a 32-bit grid size
   for (int prec = 0; prec < nprec; ++prec) {
     const long long rows = (long long)n[0] * n[1];
+    // (four-step z rows: n2-point lines over n1 columns, n1-point ones over n2)
+    auto tiles = [](long long cols, int T) { return (cols + T - 1) / T; };
     const long long rb =
-        std::max((rows + p->zr[prec].T - 1) / p->zr[prec].T, (rows + p->zi[prec].T - 1) / p->zi[prec].T);
+        p->fz.n ? rows * std::max(tiles(p->fz.n1, p->fz.c2[prec].T), tiles(p->fz.n2, p->fz.c1[prec].T))
+                : std::max((rows + p->zr[prec].T - 1) / p->zr[prec].T, (rows + p->zi[prec].T - 1) / p->zi[prec].T);
     auto col_blocks = [&](const ColCfg& c, int nouter) {
       const long long tiles = (p->nh + 7 + (long long)c.groups * c.T - 1) / ((long long)c.groups * c.T);
       return (long long)nouter * tiles;
@@ -246,15 +263,20 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     if (c.bluestein()) ok = ok && table_bluestein(c.n, c.lp.n, prec, o);
   };
   for (int prec = 0; prec < nprec && ok; ++prec) {
-    p->tw_z[prec] = table_twiddle(p->zr[prec].line.lp.n, prec);
-    p->tw_zi[prec] = table_twiddle(p->zi[prec].line.lp.n, prec);
     p->tw_zpost[prec] = table_post(nz, prec);
-    ok = p->tw_z[prec] && p->tw_zi[prec] && p->tw_zpost[prec];
-    blue(p->zr[prec].line, prec, p->bl_z[prec]);
-    blue(p->zi[prec].line, prec, p->bl_zi[prec]);
-    // column axes: whole lines, or the two sub-line lengths and W_n
-    for (int axis = 0; axis < 2; ++axis) {
-      ff_plan::FourStep& fs = axis ? p->fy : p->fx;
+    ok = p->tw_zpost[prec] != nullptr;
+    if (!p->fz.n) {
+      p->tw_z[prec] = table_twiddle(p->zr[prec].line.lp.n, prec);
+      p->tw_zi[prec] = table_twiddle(p->zi[prec].line.lp.n, prec);
+      ok = ok && p->tw_z[prec] && p->tw_zi[prec];
+      blue(p->zr[prec].line, prec, p->bl_z[prec]);
+      blue(p->zi[prec].line, prec, p->bl_zi[prec]);
+    }
+    // column axes and four-step z rows: whole lines, or the two sub-line
+    // lengths and W_n
+    for (int axis = 0; axis < 3; ++axis) {
+      ff_plan::FourStep& fs = axis == 2 ? p->fz : axis ? p->fy : p->fx;
+      if (axis == 2 && !fs.n) continue;
       const ColCfg& c = axis ? p->cy[prec] : p->cx[prec];
       if (fs.n) {
         fs.tw1[prec] = table_twiddle(fs.c1[prec].line.lp.n, prec);
@@ -287,7 +309,7 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     }
     plan_bind_scratch(p, p->scratch);
   }
-  if (o.precision == FF_FP64 && (o.flags & FF_PRESERVE_INPUT)) {
+  if (o.precision == FF_FP64 && ((o.flags & FF_PRESERVE_INPUT) || p->fz.n)) {
     st = hip_check(hipMalloc(&p->work, (size_t)p->Nh * 16), "work array hipMalloc");
     if (st != FF_OK) {
       plan_destroy(p);
@@ -419,8 +441,104 @@ static RowArgs row_args(const ff_plan* p, int prec, bool inverse) {
   return a;
 }
 
+// z rows in four steps (ParisPencil.cpp:188-189 and 402-403 transform z
+// rows of any length): the packed row z[t] = x[2t] + i x[2t+1], t = t1 +
+// m1*t2 < M = nz/2 = m1*m2, in rows of M complex values, is transformed as
+//   A: for each t1, the m2-point DFT over t2 (stride m1), times W_M^(t1*k2),
+//      at t1 + m1*k2 (column t1, stride 1);
+//   B: for each k2, the m1-point DFT over t1 (stride 1), written in natural
+//      order, frequency k2 + m2*k1, into rows of nh;
+// then the real-to-complex split in place (ff_zsplit.hip).  The inverse
+// merges the half spectrum into packed rows of nh, runs B^-1 into rows of M,
+// the conjugate twiddle and A^-1 in place, and the epilogue.  FP64, whole
+// passes, plain layout; the plan's work array holds the rows of M.
+static ff_status run_zsub(const ff_plan* p, int mode, bool stepA, const void* src, void* dst, hipStream_t s) {
+  const ff_plan::FourStep& fs = p->fz;
+  const long long m1 = fs.n1, m2 = fs.n2, M = fs.n;
+  const ColCfg& c = stepA ? fs.c2[0] : fs.c1[0];
+  auto side = [](long long KS, long long fo, long long fr) {
+    SpecSide sd;
+    memset(&sd, 0, sizeof sd);
+    sd.KS = KS;  // (lgKT 0: element (col, outer, r) at col*KS + outer*fo + r*fr)
+    sd.fo = fo;
+    sd.fr = fr;
+    sd.lgKT = 0;
+    sd.nb = INT_MAX;
+    sd.Wlast = 1;
+    return sd;
+  };
+  ColArgs a;
+  memset(&a, 0, sizeof a);
+  a.src = src;
+  a.dst = dst;
+  if (stepA) {
+    a.in = a.out = side(1, M, m1);  // column t1, position t2 (k2)
+    a.ncols = (int)m1;
+    a.xtw = fs.twn[0];
+    a.xtw_col = 1;
+  } else {
+    const SpecSide packed = side(m1, M, 1);          // column k2, position t1
+    const SpecSide natural = side(1, p->nh, m2);     // column k2, position k1: k2 + m2*k1
+    a.in = mode == COL_FWD ? packed : natural;
+    a.out = mode == COL_FWD ? natural : packed;
+    a.ncols = (int)m2;
+  }
+  a.nouter = p->n[0] * p->n[1];
+  a.T = c.T;
+  a.lgT = c.lgT;
+  a.tiles = (a.ncols + c.T - 1) / c.T;
+  a.nblocks = a.nouter * a.tiles;  // bounded at plan creation
+  a.ps = c.ps;
+  a.nx = p->n[0];
+  a.ny = p->n[1];
+  a.tw = stepA ? fs.tw2[0] : fs.tw1[0];
+  a.lp = c.line.lp;
+  a.nline = c.line.n;
+  const void* const* bl = stepA ? fs.bl2[0] : fs.bl1[0];
+  a.chirp = bl[0];
+  a.bsf = bl[1];
+  a.bsi = bl[2];
+  const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
+  return hip_check(launch_col(c.line.E, 0, mode, c.split, c.groups, c.line.bluestein(), c.stat, l, a),
+                   "z four-step launch");
+}
+
+// Arguments of the row-local steps of four-step z rows.
+static RowArgs zsplit_args(const ff_plan* p, const void* src, void* dst) {
+  RowArgs a;
+  memset(&a, 0, sizeof a);
+  a.src = src;
+  a.dst = dst;
+  a.nrows = (long long)p->n[0] * p->n[1];
+  a.nz = p->n[2];
+  a.nh = p->nh;
+  a.ny = p->n[1];
+  a.nline = p->fz.n;
+  a.scale = 1.0 / (double(p->n[0]) * double(p->n[1]) * double(p->n[2]));  // ParisPencil.cpp:408
+  a.twpost = p->tw_zpost[0];
+  return a;
+}
+
+// src_work: the source may be the work array (the inverse's merge reads it
+// before B^-1 overwrites it)
+static ff_status check_zfour(const ff_plan* p, int prec, const void* src, const void* dst, long long nrows,
+                             bool src_work) {
+  if (prec || nrows >= 0) return set_error(FF_ERR_INVALID, "z rows in four steps: whole FP64 passes only");
+  if (!p->work) return set_error(FF_ERR_INVALID, "z rows in four steps need the plan's work array");
+  if (src == dst || dst == p->work || (src == p->work && !src_work))
+    return set_error(FF_ERR_INVALID, "z rows in four steps: source, destination and work array must differ");
+  return FF_OK;
+}
+
 ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void* dst, hipStream_t s,
                    long long row0, long long nrows) {
+  if (p->fz.n) {
+    ff_status st = check_zfour(p, prec || in_f32, src, dst, nrows, false);
+    if (st) return st;
+    if ((st = run_zsub(p, COL_FWD, true, src, p->work, s))) return st;
+    if ((st = run_zsub(p, COL_FWD, false, p->work, dst, s))) return st;
+    return hip_check(launch_zsplit(ZSPLIT_FWD, 0, 0, 0, zsplit_args(p, dst, dst), s), "z split launch");
+  }
   RowArgs a = row_args(p, prec, false);
   const RowCfg& c = p->zr[prec];
   if (nrows >= 0) {
@@ -436,6 +554,18 @@ ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void
 
 ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void* src, void* dst, const double* f,
                    const double* shiftp, hipStream_t s, int lay_in, long long row0, long long nrows) {
+  if (p->fz.n) {
+    ff_status st = check_zfour(p, prec, src, dst, nrows, true);
+    if (st) return st;
+    if (out_kind || epi != EPI_STORE || f || shiftp || lay_in != LAY_PLAIN)
+      return set_error(FF_ERR_INVALID, "z rows in four steps: the stored FP64 inverse only");
+    if ((st = hip_check(launch_zsplit(ZSPLIT_INV, 0, 0, 0, zsplit_args(p, src, dst), s), "z merge launch")))
+      return st;
+    if ((st = run_zsub(p, COL_INV, false, dst, p->work, s))) return st;
+    if ((st = run_zsub(p, COL_INV, true, p->work, p->work, s))) return st;
+    return hip_check(launch_zsplit(ZSPLIT_EPI, 0, 0, EPI_STORE, zsplit_args(p, p->work, dst), s),
+                     "z epilogue launch");
+  }
   RowArgs a = row_args(p, prec, true);
   a.spec = spec_side(p, 1, lay_in);
   if (nrows >= 0) {

@@ -46,10 +46,12 @@ struct ff_plan {
   float* res;   // N floats
   double* dc;   // sum of phi (refinement mean removal)
   double* dcf;  // sum of f (removed from the residual before it is rounded to FP32)
-  // FP64 with FF_PRESERVE_INPUT: third work array (16*Nh bytes)
+  // FP64 with FF_PRESERVE_INPUT or four-step z rows: third work array
+  // (16*Nh bytes)
   void* work;
-  // Four-step column lines: an x or y extent n that no workgroup transforms
-  // whole runs as n1-point and n2-point sub-line passes, n = n1*n2.
+  // Four-step lines: an x or y extent n that no workgroup transforms whole
+  // runs as n1-point and n2-point sub-line passes, n = n1*n2; so do the
+  // packed z rows (n = nz/2, even nz, FP64 plans) when nz/2 is too long.
   struct FourStep {
     int n, n1, n2;              // n = 0: the axis is transformed whole
     ff::ColCfg c1[2], c2[2];    // n1-point and n2-point sub-lines (generic kernels)
@@ -58,7 +60,7 @@ struct ff_plan {
     const void* twn[2];         // W_n, t < n
     const void* bl1[2][3];      // Bluestein tables of the sub-lines
     const void* bl2[2][3];
-  } fy, fx;
+  } fy, fx, fz;
 };
 
 namespace ff {

@@ -1,0 +1,127 @@
+// Row-local steps of the z passes whose packed rows run as four-step
+// sub-line passes (even nz, M = nz/2 too long for one workgroup's line
+// transform): the real-to-complex split after the forward M-point transform,
+// the complex-to-real merge before the inverse one, and the inverse's
+// epilogue.  The whole-row kernels (k_r2c, k_c2r) do the same arithmetic on
+// rows held in LDS (ParisPencil.cpp:188-189 and 402-403: D2Z and Z2D along z).
+#include <algorithm>
+
+#include "ff_internal.hpp"
+#include "ff_kernels.hpp"
+
+namespace ff {
+
+namespace {
+
+constexpr int kZThreads = 256;
+
+// Workgroups of a grid-stride launch over total items.
+unsigned int zsplit_blocks(long long total) {
+  const long long b = (total + kZThreads - 1) / kZThreads;
+  return (unsigned int)std::max(1LL, std::min(b, 256LL * 64));
+}
+
+// Forward split, in place on rows of nh complex values (row r at r*nh) that
+// hold the packed transform Z[k], k < M, in natural order: X[k] = (Z[k] +
+// conj Z[M-k])/2 - i W_nz^k (Z[k] - conj Z[M-k])/2 for k < M and X[M] =
+// Re Z[0] - Im Z[0].  Item (r, k), k <= M/2, reads Z[k] and Z[M-k] and writes
+// X[k] and X[M-k] (and X[M] for k = 0), so the pairs never overlap.
+template <typename F>
+__global__ __launch_bounds__(kZThreads) void k_zsplit_fwd(const RowArgs a) {
+  const PassInst io = inst_of(a);
+  const int M = a.nline;
+  const long long P = M / 2 + 1;
+  const long long total = a.nrows * P;
+  FF_GLB cx<F>* z = glb<cx<F>>(io.dst);
+  const FF_GLB cx<F>* twp = glb<const cx<F>>(a.twpost);
+  auto half = [&](cx<F> zk, cx<F> zo, int k) {
+    const cx<F> zm = conj(zo);
+    // even spectrum (zk + zm)/2, odd spectrum (zk - zm)/(2i)
+    const cx<F> ev = {F(0.5) * (zk.x + zm.x), F(0.5) * (zk.y + zm.y)};
+    const cx<F> od = {F(0.5) * (zk.y - zm.y), F(-0.5) * (zk.x - zm.x)};
+    return ev + twmul<-1>(od, twp[k]);
+  };
+  for (long long it = (long long)blockIdx.x * kZThreads + threadIdx.x; it < total;
+       it += (long long)gridDim.x * kZThreads) {
+    const long long r = it / P;
+    const int k = (int)(it - r * P);
+    FF_GLB cx<F>* row = z + r * a.nh;
+    const int km = k ? M - k : 0;
+    const cx<F> zk = row[k];
+    const cx<F> zo = row[km];
+    row[k] = half(zk, zo, k);
+    if (km != k) row[km] = half(zo, zk, km);
+    if (k == 0) row[M] = cx<F>{zk.x - zk.y, F(0)};
+  }
+}
+
+// Inverse merge, src -> dst, both rows of nh complex values: the half
+// spectrum X (Hermitian: the imaginary parts of X[0] and X[M] are ignored)
+// to the packed spectrum Z'[k] = A + i B, A = X[k] + conj X[M-k],
+// B = W_nz^-k (X[k] - conj X[M-k]), k < M.
+template <typename F>
+__global__ __launch_bounds__(kZThreads) void k_zsplit_inv(const RowArgs a) {
+  const PassInst io = inst_of(a);
+  const int M = a.nline;
+  const long long total = a.nrows * M;
+  const FF_GLB cx<F>* x = glb<const cx<F>>(io.src);
+  FF_GLB cx<F>* z = glb<cx<F>>(io.dst);
+  const FF_GLB cx<F>* twp = glb<const cx<F>>(a.twpost);
+  for (long long it = (long long)blockIdx.x * kZThreads + threadIdx.x; it < total;
+       it += (long long)gridDim.x * kZThreads) {
+    const long long r = it / M;
+    const int k = (int)(it - r * M);
+    const FF_GLB cx<F>* row = x + r * a.nh;
+    cx<F> xk = row[k];
+    cx<F> xm = row[M - k];
+    if (k == 0) {
+      xk.y = F(0);
+      xm.y = F(0);
+    }
+    const cx<F> cm = conj(xm);
+    const cx<F> A = xk + cm;
+    const cx<F> B = twmul<+1>(xk - cm, twp[k]);
+    z[r * a.nh + k] = cx<F>{A.x - B.y, A.y + B.x};
+  }
+}
+
+// Inverse epilogue: the packed rows z (M complex values per row, rows
+// contiguous, so z is nrows*M pairs of real values in order) to the real
+// output, stored, as the residual or accumulated (c2r_out2).
+template <typename F, typename OUT, int EPI>
+__global__ __launch_bounds__(kZThreads) void k_zepi(const RowArgs a) {
+  const PassInst io = inst_of(a);
+  const long long total = a.nrows * a.nline;
+  const FF_GLB cx<F>* z = glb<const cx<F>>(io.src);
+  FF_GLB OUT* out = glb<OUT>(io.dst);
+  const FF_GLB double* f = glb<const double>(io.aux);
+  const double shift = io.shiftp ? a.shift * *glb<const double>(io.shiftp) : a.shift;
+  for (long long it = (long long)blockIdx.x * kZThreads + threadIdx.x; it < total;
+       it += (long long)gridDim.x * kZThreads) {
+    const cx<F> v = z[it];
+    c2r_out2<EPI, OUT>(out, f, it, to_d(v.x), to_d(v.y), a.scale, shift);
+  }
+}
+
+}  // namespace
+
+hipError_t launch_zsplit(int step, int f32, int out_kind, int epi, const RowArgs& a, hipStream_t s) {
+  if (f32 || launch_recording()) return hipErrorInvalidValue;  // FP64 plans, never recorded
+  const void* fn = nullptr;
+  long long total = 0;
+  if (step == ZSPLIT_FWD) {
+    fn = (const void*)&k_zsplit_fwd<double>;
+    total = a.nrows * (a.nline / 2 + 1);
+  } else if (step == ZSPLIT_INV) {
+    fn = (const void*)&k_zsplit_inv<double>;
+    total = a.nrows * a.nline;
+  } else if (step == ZSPLIT_EPI && out_kind == 0 && epi == EPI_STORE) {
+    fn = (const void*)&k_zepi<double, double, EPI_STORE>;
+    total = a.nrows * a.nline;
+  }
+  if (!fn) return hipErrorInvalidValue;
+  void* args[] = {const_cast<RowArgs*>(&a)};
+  return hipLaunchKernel(fn, dim3(zsplit_blocks(total)), dim3(kZThreads), args, 0, s);
+}
+
+}  // namespace ff

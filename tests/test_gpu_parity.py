@@ -566,6 +566,109 @@ def test_four_step_lines_in_a_batch():
     b.close()
 
 
+# even nz whose packed rows (nz/2 complex values) no workgroup transforms
+# whole: 5-smooth (10000, 16384) and with Bluestein sub-lines (8500 = 85 *
+# 100, 8195 = 5 * 11 * 149, 10001 = 73 * 137)
+LONG_Z_SHAPES = [(3, 2, 20000), (2, 2, 32768), (4, 3, 17000), (2, 5, 16390), (5, 2, 20002)]
+
+
+@pytest.mark.parametrize("n", LONG_Z_SHAPES)
+def test_four_step_z_rows_match_oracle(n):
+    """The reference transforms z rows of any length (D2Z / Z2D plans,
+    ParisPencil.cpp:74-77).  Packed rows too long for one workgroup run as
+    four-step sub-line passes through the plan's work array, with the
+    real-to-complex split and the complex-to-real merge as row-local steps:
+    the solve, the z forward pass (pass 0) and the z inverse pass (pass 4)
+    match the oracle; the density is left unchanged; the allocation-free
+    entry point gives the same bits."""
+    rng = np.random.default_rng(n[2] + n[1])
+    f = rng.standard_normal(n)
+    lo, hi, _ = ff.benchmark_grid(n)
+    plan, nb, a, b = make(n)
+    fs = json.loads(plan.describe())["four_step"]
+    say("four-step z", n, fs)
+    assert fs["z"][0] * fs["z"][1] == n[2] // 2 and fs["z"][0] > 1
+    put(a, f)
+    plan.solve(nb, a, b)
+    torch.cuda.synchronize()
+    got = get_real(b, n)
+    assert rel(got, oracle.solve(f)) < TOL, rel(got, oracle.solve(f))
+    assert np.array_equal(get_real(a, n), f)  # the rows of M live in the plan's work array
+    c = torch.zeros_like(b)
+    plan.solve_noalloc(nb, a, c)
+    torch.cuda.synchronize()
+    assert np.array_equal(get_real(c, n), got)
+    # pass 0: z forward (D2Z)
+    plan.solve_pass(0, a, b)
+    torch.cuda.synchronize()
+    assert rel(get_half(b, n), oracle.rfft(f)) < TOL
+    # pass 4: z inverse (Z2D) and the 1/N scale (Im of DC / Nyquist ignored)
+    nh = n[2] // 2 + 1
+    spec = rng.standard_normal((n[0], n[1], nh)) + 1j * rng.standard_normal((n[0], n[1], nh))
+    put_half(a, spec)
+    plan.solve_pass(4, a, b)
+    torch.cuda.synchronize()
+    want = oracle.irfft(spec, n[2]) * (1.0 / (float(n[0]) * float(n[1]) * float(n[2])))
+    assert rel(get_real(b, n), want) < TOL
+    assert np.array_equal(get_half(a, n), spec)  # the source half spectrum is read only
+    with pytest.raises(ff.FishFryError) as e:
+        plan.solve_pass(4, a, a)  # the merge cannot run in place
+    assert e.value.status == ff.FF_ERR_INVALID
+    plan.close()
+
+
+@pytest.mark.parametrize("n", [(2, 4100, 16400), (4100, 2, 16400)])
+def test_four_step_z_with_four_step_columns(n):
+    """Four-step z rows with four-step y (or x) lines in one plan (1.3e8
+    points; largest prime factor 41, so the oracle finishes in seconds)."""
+    rng = np.random.default_rng(n[0])
+    f = rng.standard_normal(n)
+    plan, nb, a, b = make(n)
+    fs = json.loads(plan.describe())["four_step"]
+    say("four-step z + columns", n, fs)
+    assert fs["z"][0] > 1 and (fs["y"][0] > 1 or fs["x"][0] > 1)
+    plan.close()
+    got = gpu_solve(f)
+    want = oracle.solve(f)
+    assert rel(got, want) < TOL, rel(got, want)
+
+
+def test_four_step_z_limits():
+    """Four-step z rows: FP64 plans in the plain layout; FP32 plans, the
+    pencil, blocked and slab layouts and the batch's recorded modes are
+    refused with a message; the stream-parallel batch runs them and matches
+    the single-plan solve bit for bit."""
+    n = (3, 2, 20000)
+    lo, hi, _ = ff.benchmark_grid(n)
+    for kw in ({"precision": ff.FF_FP32_REFINE, "refine_steps": 1}, {"flags": ff.FF_PENCIL_LAYOUT},
+               {"flags": ff.FF_BLOCKED_LAYOUT}, {"flags": ff.FF_SLAB_PIPELINE}):
+        with pytest.raises(ff.FishFryError) as e:
+            ff.ParisPencil(n, lo, hi, **kw)
+        assert e.value.status == ff.FF_ERR_UNSUPPORTED, kw
+        assert "four steps" in str(e.value)
+    items = [{"n": n, "lo": lo, "hi": hi}, {"n": (8, 8, 8)}]
+    for flags in (ff.FF_BATCH_GROUPED, ff.FF_BATCH_STEPS):
+        with pytest.raises(ff.FishFryError) as e:
+            ff.Batch(items, flags)
+        assert e.value.status == ff.FF_ERR_UNSUPPORTED
+    rng = np.random.default_rng(12)
+    fs = [rng.standard_normal(n), rng.standard_normal((8, 8, 8))]
+    outs = [gpu_solve(f) for f in fs]
+    bt = ff.Batch(items, 0)
+    dens, pots = [], []
+    for k, f in enumerate(fs):
+        nb = bt.bytes(k)
+        d = torch.zeros(nb // 8, dtype=torch.float64, device="cuda")
+        put(d, f)
+        dens.append(d)
+        pots.append(torch.zeros(nb // 8, dtype=torch.float64, device="cuda"))
+    bt.solve(dens, pots)
+    torch.cuda.synchronize()
+    for k, f in enumerate(fs):
+        assert np.array_equal(get_real(pots[k], f.shape), outs[k])
+    bt.close()
+
+
 @pytest.mark.parametrize("n", [(512, 512, 512), (768, 768, 768), (1000, 1000, 1000), (1024, 1024, 1024)])
 def test_large_benchmark_norms(n):
     """Full-size FP64 solves: the analytic norms stay at round-off, and the
