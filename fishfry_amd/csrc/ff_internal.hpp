@@ -129,10 +129,12 @@ hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, 
 bool launch_recording();
 
 // Row-local steps of z passes over four-step rows (ff_zsplit.hip; FP64):
-// the forward split (in place on io.dst), the inverse merge (src -> dst) and
-// the inverse epilogue (packed rows src -> real dst).  a.nline = M = nz/2.
-// Never recorded: refused while a recorder is installed.
-enum { ZSPLIT_FWD = 0, ZSPLIT_INV = 1, ZSPLIT_EPI = 2 };
+// the forward split (in place on io.dst), the inverse merge (src -> dst),
+// the inverse epilogue (packed rows src -> real dst) and the four-step
+// twiddle between the sub-line passes (forward or conjugate, in place on
+// io.dst; a.tw = W_M, a.m1).  a.nline = M = nz/2.  Never recorded: refused
+// while a recorder is installed.
+enum { ZSPLIT_FWD = 0, ZSPLIT_INV = 1, ZSPLIT_EPI = 2, ZSPLIT_TWF = 3, ZSPLIT_TWI = 4 };
 hipError_t launch_zsplit(int step, int f32, int out_kind, int epi, const RowArgs& a, hipStream_t s);
 
 // ---- driver kernels ---------------------------------------------------------

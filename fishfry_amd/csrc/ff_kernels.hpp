@@ -107,9 +107,6 @@ struct ColArgs {
   // (y lines held in four-step order).
   const void* xtw;
   int imap_n2, jmap_n1, jmap_n2;
-  // xtw_col = 1: the twiddle's lo is the column instead (z rows in four
-  // steps: the columns are the stride-1 index of the sub-lines)
-  int xtw_col;
   // Alignment of the written row segments (kernels with a compiled-in plan).
   // When all rows of a plane start at one offset within a 64-byte block but
   // planes start at different ones (plain x pass, odd nh), a plane's tiles
@@ -369,28 +366,19 @@ constexpr int col_min() {
 // sides use the same layout (in-place pass): every thread stores exactly the
 // elements it loaded, after all of its loads.
 // Four-step sub-lines (generic kernels): position r of plane lo times W_n[lo*r]
-// (DIR < 0, after a forward transform) or its conjugate (before an inverse);
-// with xtw_col, lo is the column (clamped as the loads clamp it).
+// (DIR < 0, after a forward transform) or its conjugate (before an inverse).
 template <int DIR, typename F, int E, int G>
-__device__ __forceinline__ void four_step_twiddle(cx<F> (&v)[G][E], const ColArgs& a, const ColTile& t, int c,
-                                                  int r0, int L) {
+__device__ __forceinline__ void four_step_twiddle(cx<F> (&v)[G][E], const ColArgs& a, int outer, int r0, int L) {
   if (!a.xtw) return;
+  const int lo = outer % a.in.nlo;
   const FF_GLB cx<F>* w = glb<const cx<F>>(a.xtw);
 #pragma unroll
-  for (int g = 0; g < G; ++g) {
-    int lo;
-    if (a.xtw_col) {
-      const int col = (t.tile * G + g) * a.T + c;
-      lo = col < a.ncols ? col : a.ncols - 1;
-    } else {
-      lo = t.outer % a.in.nlo;
-    }
+  for (int g = 0; g < G; ++g)
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int r = r0 + i * L;
-      if (r < a.nline) v[g][i] = twmul<DIR>(v[g][i], cx<F>(w[(long long)lo * r]));
+      if (r < a.nline) v[g][i] = twmul<DIR>(v[g][i], cx<F>(w[lo * r]));
     }
-  }
 }
 
 template <typename F, int E, int G, bool BL, int SN, int ST, int MODE = COL_FWD>
@@ -414,7 +402,7 @@ __device__ __forceinline__ void col_load(cx<F> (&v)[G][E], const ColArgs& a, con
       for (int i = 0; i < E; ++i) v[g][i] = r0 + i * L < a.nline ? cx<F>(src[(long long)i * is]) : cx<F>{F(0), F(0)};
     }
   }
-  if constexpr (SN == 0 && MODE == COL_INV) four_step_twiddle<+1>(v, a, t, c, r0, L);
+  if constexpr (SN == 0 && MODE == COL_INV) four_step_twiddle<+1>(v, a, t.outer, r0, L);
 }
 
 // LDS bytes of the exchange of a compiled-in column tile (SN positions, ST
@@ -463,7 +451,7 @@ __device__ __forceinline__ void col_store(cx<F> (&v)[G][E], const ColArgs& a, co
   const int L = SN > 0 ? SN / E : a.lp.L;
   int c, r0;
   col_thread<E, SN, ST>(a, c, r0);
-  if constexpr (SN == 0 && MODE == COL_FWD) four_step_twiddle<-1>(v, a, t, c, r0, L);
+  if constexpr (SN == 0 && MODE == COL_FWD) four_step_twiddle<-1>(v, a, t.outer, r0, L);
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     const int col = (t.tile * G + g) * T + c - t.sh;
@@ -544,6 +532,7 @@ struct RowArgs {
   int nline;                     // transform length M (lp.n: engine length; larger for Bluestein)
   const void *chirp, *bsf, *bsi;  // Bluestein tables (see line_dft)
   const PassInst* inst;           // grouped launch: per-instance fields (null: one solve)
+  int m1;                         // four-step z rows: the stride-1 sub-line length (ff_zsplit.hip)
 };
 
 __device__ __forceinline__ PassInst inst_of(const RowArgs& a) {

@@ -444,8 +444,9 @@ static RowArgs row_args(const ff_plan* p, int prec, bool inverse) {
 // z rows in four steps (ParisPencil.cpp:188-189 and 402-403 transform z
 // rows of any length): the packed row z[t] = x[2t] + i x[2t+1], t = t1 +
 // m1*t2 < M = nz/2 = m1*m2, in rows of M complex values, is transformed as
-//   A: for each t1, the m2-point DFT over t2 (stride m1), times W_M^(t1*k2),
-//      at t1 + m1*k2 (column t1, stride 1);
+//   A: for each t1, the m2-point DFT over t2 (stride m1), in place at t1 +
+//      m1*k2 (column t1, stride 1), then times W_M^(t1*k2) (a row-local
+//      kernel: the column kernels take their four-step twiddle by plane);
 //   B: for each k2, the m1-point DFT over t1 (stride 1), written in natural
 //      order, frequency k2 + m2*k1, into rows of nh;
 // then the real-to-complex split in place (ff_zsplit.hip).  The inverse
@@ -474,8 +475,6 @@ static ff_status run_zsub(const ff_plan* p, int mode, bool stepA, const void* sr
   if (stepA) {
     a.in = a.out = side(1, M, m1);  // column t1, position t2 (k2)
     a.ncols = (int)m1;
-    a.xtw = fs.twn[0];
-    a.xtw_col = 1;
   } else {
     const SpecSide packed = side(m1, M, 1);          // column k2, position t1
     const SpecSide natural = side(1, p->nh, m2);     // column k2, position k1: k2 + m2*k1
@@ -514,8 +513,10 @@ static RowArgs zsplit_args(const ff_plan* p, const void* src, void* dst) {
   a.nh = p->nh;
   a.ny = p->n[1];
   a.nline = p->fz.n;
+  a.m1 = p->fz.n1;
   a.scale = 1.0 / (double(p->n[0]) * double(p->n[1]) * double(p->n[2]));  // ParisPencil.cpp:408
   a.twpost = p->tw_zpost[0];
+  a.tw = p->fz.twn[0];
   return a;
 }
 
@@ -536,6 +537,8 @@ ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void
     ff_status st = check_zfour(p, prec || in_f32, src, dst, nrows, false);
     if (st) return st;
     if ((st = run_zsub(p, COL_FWD, true, src, p->work, s))) return st;
+    if ((st = hip_check(launch_zsplit(ZSPLIT_TWF, 0, 0, 0, zsplit_args(p, p->work, p->work), s), "z twiddle launch")))
+      return st;
     if ((st = run_zsub(p, COL_FWD, false, p->work, dst, s))) return st;
     return hip_check(launch_zsplit(ZSPLIT_FWD, 0, 0, 0, zsplit_args(p, dst, dst), s), "z split launch");
   }
@@ -562,6 +565,8 @@ ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void
     if ((st = hip_check(launch_zsplit(ZSPLIT_INV, 0, 0, 0, zsplit_args(p, src, dst), s), "z merge launch")))
       return st;
     if ((st = run_zsub(p, COL_INV, false, dst, p->work, s))) return st;
+    if ((st = hip_check(launch_zsplit(ZSPLIT_TWI, 0, 0, 0, zsplit_args(p, p->work, p->work), s), "z twiddle launch")))
+      return st;
     if ((st = run_zsub(p, COL_INV, true, p->work, p->work, s))) return st;
     return hip_check(launch_zsplit(ZSPLIT_EPI, 0, 0, EPI_STORE, zsplit_args(p, p->work, dst), s),
                      "z epilogue launch");

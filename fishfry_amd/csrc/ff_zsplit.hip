@@ -85,6 +85,26 @@ __global__ __launch_bounds__(kZThreads) void k_zsplit_inv(const RowArgs a) {
   }
 }
 
+// Four-step twiddle, in place on packed rows of M = m1*m2 complex values
+// (rows contiguous): element t1 + m1*k2 of every row times W_M^(t1*k2)
+// (DIR < 0: between the forward sub-line passes) or its conjugate (DIR > 0:
+// between the inverse ones).  a.tw: W_M, t < M.
+template <typename F, int DIR>
+__global__ __launch_bounds__(kZThreads) void k_ztwiddle(const RowArgs a) {
+  const PassInst io = inst_of(a);
+  const int M = a.nline, m1 = a.m1;
+  const long long total = a.nrows * M;
+  FF_GLB cx<F>* z = glb<cx<F>>(io.dst);
+  const FF_GLB cx<F>* w = glb<const cx<F>>(a.tw);
+  for (long long it = (long long)blockIdx.x * kZThreads + threadIdx.x; it < total;
+       it += (long long)gridDim.x * kZThreads) {
+    const int e = (int)(it % M);
+    const int k2 = e / m1;
+    const int t1 = e - k2 * m1;
+    z[it] = twmul<DIR>(z[it], w[(long long)t1 * k2]);
+  }
+}
+
 // Inverse epilogue: the packed rows z (M complex values per row, rows
 // contiguous, so z is nrows*M pairs of real values in order) to the real
 // output, stored, as the residual or accumulated (c2r_out2).
@@ -114,6 +134,9 @@ hipError_t launch_zsplit(int step, int f32, int out_kind, int epi, const RowArgs
     total = a.nrows * (a.nline / 2 + 1);
   } else if (step == ZSPLIT_INV) {
     fn = (const void*)&k_zsplit_inv<double>;
+    total = a.nrows * a.nline;
+  } else if (step == ZSPLIT_TWF || step == ZSPLIT_TWI) {
+    fn = step == ZSPLIT_TWF ? (const void*)&k_ztwiddle<double, -1> : (const void*)&k_ztwiddle<double, +1>;
     total = a.nrows * a.nline;
   } else if (step == ZSPLIT_EPI && out_kind == 0 && epi == EPI_STORE) {
     fn = (const void*)&k_zepi<double, double, EPI_STORE>;
