@@ -264,6 +264,14 @@ template <typename F, int H>
 struct TwHalf {
   const cx<F>* p;
 };
+template <typename T>
+struct is_tw_half {
+  static constexpr bool value = false;
+};
+template <typename F, int H>
+struct is_tw_half<TwHalf<F, H>> {
+  static constexpr bool value = true;
+};
 template <typename F, int H>
 __device__ __forceinline__ cx<F> tw_at(const TwHalf<F, H>& tw, int t) {
   const bool hi = t >= H;

@@ -269,7 +269,11 @@ __device__ __forceinline__ void col_line(cx<F> (&v)[E], const ColArgs& a, const 
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const int ii = plo + istep * (r0 + i * L);
-        const double gi = glb<const double>(a.symx)[ii < a.nx ? ii : 0];
+        double gi;
+        if constexpr (is_tw_half<TW>::value)
+          gi = reinterpret_cast<const double*>(tw.p + SN / 2)[ii];  // LDS copy (compiled-in: ii < SN = nx)
+        else
+          gi = glb<const double>(a.symx)[ii < a.nx ? ii : 0];
         const double s2 = fmadd(gk, io.ddk, fmadd(gi, io.ddi, tj));  // (i + j) + k
         const double d = (ii | jj | kk) ? -1.0 / s2 : 0.0;
         const F df = F(d);
@@ -481,12 +485,20 @@ __device__ __forceinline__ void col_tile(const ColArgs& a, const PassInst& io, i
 // the transforms otherwise go through the vector memory path that the tile's
 // own loads and stores keep busy: from LDS the FP64 x pass at 1024 takes 7 %
 // less time on MI355X.  Launch LDS: col_exchange_bytes + (SN/2) complex values.
-template <typename F, bool SPLIT, int SN, int ST, int THREADS>
+template <typename F, bool SPLIT, int SN, int ST, int THREADS, int MODE = COL_FWD>
 __device__ __forceinline__ void col_twiddles_to_lds(const ColArgs& a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* twl = reinterpret_cast<cx<F>*>(smem_raw + col_exchange_bytes<F, SPLIT, SN, ST>());
   const FF_GLB cx<F>* twg = glb<const cx<F>>(a.tw);
   for (int i = threadIdx.x; i < SN / 2; i += THREADS) twl[tw_swizzle(i)] = twg[i];
+  if constexpr (MODE == COL_FILTER_FD) {
+    // the x axis's finite-difference factors (SN doubles) follow the
+    // twiddles: the filter then reads them from LDS instead of through the
+    // vector memory path (launch LDS: SN * 8 bytes more)
+    double* sym = reinterpret_cast<double*>(twl + SN / 2);
+    const FF_GLB double* symg = glb<const double>(a.symx);
+    for (int i = threadIdx.x; i < SN; i += THREADS) sym[i] = symg[i];
+  }
   __syncthreads();
 }
 
@@ -502,7 +514,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, SN > 0 ? SN / E * ST : 
     const ColTile t = col_tile_at<F, SN>(a, io, b);
     cx<F> v[1][E];
     col_load<F, E, 1, false, SN, ST>(v, a, io, t);
-    col_twiddles_to_lds<F, SPLIT, SN, ST, SN / E * ST>(a);
+    col_twiddles_to_lds<F, SPLIT, SN, ST, SN / E * ST, MODE>(a);
     col_transform<F, E, MODE, SPLIT, 1, false, SN, ST, true>(v, a, io, t);
     col_store<F, E, 1, false, SN, ST>(v, a, io, t);
   } else {

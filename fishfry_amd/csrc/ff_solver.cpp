@@ -708,7 +708,10 @@ static ff_status run_col_cfg(const ff_plan* p, int prec, int axis, int mode, con
   a.chirp = bl[0];
   a.bsf = bl[1];
   a.bsi = bl[2];
-  const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
+  // kernels with LDS twiddles also copy the x axis's finite-difference
+  // factors to LDS, after the twiddles
+  const size_t lds = c.lds + (mode == COL_FILTER_FD && c.stat == 2 ? (size_t)c.line.lp.n * sizeof(double) : 0);
+  const Launch l{dim3(a.nblocks), dim3(c.threads), lds, s};
   return hip_check(launch_col(c.line.E, prec, mode, c.split, c.groups, c.line.bluestein(), c.stat, l, a),
                    "column pass launch");
 }

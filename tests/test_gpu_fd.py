@@ -63,6 +63,45 @@ def test_fd_solve_matches_oracle(n, order):
 
 
 @pytest.mark.parametrize("order", [3, 5])
+@pytest.mark.parametrize("n", [(1024, 8, 6), (1024, 12, 32)])
+def test_fd_x_factors_from_lds_bitwise(n, order):
+    """The compiled-in 1024 x kernels with LDS twiddles also read the x
+    axis's finite-difference factors from an LDS copy: the solves are
+    bit-identical to the generic x kernel's (which reads them from memory),
+    FP64 and FP32 + 0 and + 1 steps, plain and pencil layouts."""
+    rng = np.random.default_rng(82)
+    f = rng.standard_normal(n)
+    try:
+        for prec, steps in ((ff.FF_FP64, 0), (ff.FF_FP32_REFINE, 0), (ff.FF_FP32_REFINE, 1)):
+            for lay in (ff.FF_PLAIN_LAYOUT, ff.FF_PENCIL_LAYOUT):
+                outs = []
+                for generic in (False, True):
+                    ff.tune_reset()
+                    if generic:
+                        for p in (0, 1):
+                            cur = ff.tune_query(2, 1024, p, n[2] // 2 + 1)
+                            ff.tune_set(2, 1024, p, cur.E, cur.T,
+                                        ff.FF_TUNE_GENERIC | (cur.flags & ff.FF_TUNE_SPLIT_LDS))
+                    else:
+                        assert ff.tune_query(2, 1024, 0, n[2] // 2 + 1).flags & ff.FF_TUNE_LDS_TWIDDLES
+                    lo, hi, _ = ff.benchmark_grid(n)
+                    plan = ff.ParisPencil(n, lo, hi, precision=prec, refine_steps=steps, flags=FLAG[order] | lay)
+                    nb = plan.bytes()
+                    a = torch.zeros(nb // 8, dtype=torch.float64, device="cuda")
+                    b = torch.zeros(nb // 8, dtype=torch.float64, device="cuda")
+                    a[: f.size].copy_(torch.from_numpy(np.ascontiguousarray(f).ravel()))
+                    say("fd lds", n, order, prec, steps, lay, generic)
+                    plan.solve(nb, a, b)
+                    torch.cuda.synchronize()
+                    outs.append(b[: f.size].cpu().numpy().reshape(n))
+                    plan.close()
+                assert np.array_equal(outs[0], outs[1])
+    finally:
+        ff.tune_reset()
+    assert rel(outs[0], oracle.solve(f, symbol=order)) < 1e-9
+
+
+@pytest.mark.parametrize("order", [3, 5])
 @pytest.mark.parametrize("n", [(16, 24, 20), (7, 5, 11), (2, 3, 300), (64, 64, 64)])
 def test_stencil_apply_matches_oracle(n, order):
     """ff_stencil_apply (the real-space operator) against the oracle's stencil,
