@@ -119,6 +119,16 @@ size_t norm_bytes() { return 2 * sizeof(double) * kDiffDoubles; }
 
 int size_class(const ff_request& r) { return ff::size_class(r.precision, r.refine_steps, r.n); }
 
+// Device work area of a request's solve beyond density and potential: the
+// FP32 refinement work area, or the work array of FP64 grids whose z rows
+// run in four steps (0: none).
+size_t work_bytes(const ff_request& r) {
+  const size_t N = (size_t)r.n[0] * r.n[1] * r.n[2];
+  const size_t Nh = (size_t)r.n[0] * r.n[1] * (r.n[2] / 2 + 1);
+  if (r.precision == FF_FP32_REFINE) return Nh * 16 + align_up(N * 4, 256) + 256;
+  return z_rows_four_step(r.n, r.precision) ? Nh * 16 : 0;
+}
+
 ff_plan* plan_for(ff_server* s, const Req& q, ff_status& st) {
   // The domain of a field request is the field's own; a host-array request
   // is on the benchmark's grid, [-pi, pi) per axis (FishFry.hpp:108-139).
@@ -195,7 +205,10 @@ void dispatch(ff_server* s, std::vector<Req*>& group, int k) {
       continue;
     }
     ff_plan p = *cached;  // shares twiddles; the work area is the request's own
-    if (q->has_s) plan_bind_scratch(&p, base + q->off_s);
+    if (q->has_s) {
+      if (p.opt.precision == FF_FP32_REFINE) plan_bind_scratch(&p, base + q->off_s);
+      else p.work = base + q->off_s;  // four-step z rows
+    }
     double* d = reinterpret_cast<double*>(base + q->off_d);
     double* phi = reinterpret_cast<double*>(base + q->off_p);
     if (q->has_field) {
@@ -433,11 +446,7 @@ size_t ff_server_footprint(const ff_request* r) {
     if (r->n[a] < 2) return 0;
   const int ones[3] = {1, 1, 1};
   size_t b = 2 * align_up(reference_bytes(r->n, ones), 256);
-  if (r->precision == FF_FP32_REFINE) {
-    const size_t N = (size_t)r->n[0] * r->n[1] * r->n[2];
-    const size_t Nh = (size_t)r->n[0] * r->n[1] * (r->n[2] / 2 + 1);
-    b += align_up(Nh * 16 + align_up(N * 4, 256) + 256, 256);
-  }
+  b += align_up(work_bytes(*r), 256);
   if (r->want_norms && r->field) b += align_up(norm_bytes(), 256);
   return b;
 }
@@ -474,11 +483,7 @@ ff_status ff_server_submit(ff_server* s, const ff_request* r, uint64_t* id) {
   }
   const int ones[3] = {1, 1, 1};
   q->bytes = reference_bytes(r->n, ones);
-  if (r->precision == FF_FP32_REFINE) {
-    const size_t N = (size_t)r->n[0] * r->n[1] * r->n[2];
-    const size_t Nh = (size_t)r->n[0] * r->n[1] * (r->n[2] / 2 + 1);
-    q->scratch = Nh * 16 + align_up(N * 4, 256) + 256;
-  }
+  q->scratch = work_bytes(*r);
   if (r->want_norms && r->field) q->normbytes = norm_bytes();
   q->cls = size_class(*r);
   q->t_submit = now_s();

@@ -309,7 +309,9 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     }
     plan_bind_scratch(p, p->scratch);
   }
-  if (o.precision == FF_FP64 && ((o.flags & FF_PRESERVE_INPUT) || p->fz.n)) {
+  // (four-step z rows: own_scratch false leaves the work array to the caller,
+  // who points the plan, or a copy of it, at 16*Nh bytes of its own)
+  if (o.precision == FF_FP64 && ((o.flags & FF_PRESERVE_INPUT) || (p->fz.n && own_scratch))) {
     st = hip_check(hipMalloc(&p->work, (size_t)p->Nh * 16), "work array hipMalloc");
     if (st != FF_OK) {
       plan_destroy(p);
@@ -318,6 +320,12 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   }
   *out = p;
   return FF_OK;
+}
+
+bool z_rows_four_step(const int n[3], int precision) {
+  if (precision != FF_FP64 || n[2] % 2) return false;
+  RowCfg r0, r1;
+  return !make_row_cfg(n[2], 16, r0, 0, 0) || !make_row_cfg(n[2], 16, r1, 3, 0);
 }
 
 size_t plan_scratch_bytes(const ff_plan* p) {

@@ -91,11 +91,16 @@ void pass_layouts(const ff_plan* p, int pass, int& lay_in, int& lay_out);
 ff_status solve_enqueue(const ff_plan* p, double* density, double* potential, hipStream_t s, hipEvent_t* ev);
 
 // own_scratch false: an FP32 plan gets no refinement work area of its own and
-// must be bound to one (plan_bind_scratch) before it is enqueued; callers that
-// sub-allocate device memory themselves (the server's arena) use this.
+// must be bound to one (plan_bind_scratch) before it is enqueued, and an FP64
+// plan with four-step z rows no work array (the caller sets work); callers
+// that sub-allocate device memory themselves (the server's arena) use this.
 ff_status plan_create(const int n[3], const double lo[3], const double hi[3], const ff_options* opts,
                       ff_plan** out, bool own_scratch = true);
 void plan_destroy(ff_plan* p);
+// Whether an FP64 grid of extents n runs its z rows in four steps, which
+// needs the work array (16*Nh bytes; the plan's own unless created with
+// own_scratch false).
+bool z_rows_four_step(const int n[3], int precision);
 // FP32 refinement work area: its size, and pointing a plan (or a copy of one)
 // at a work area of that size.
 size_t plan_scratch_bytes(const ff_plan* p);

@@ -323,8 +323,9 @@ typedef struct ff_serving_stats {
  * 4 streams, groups of 16, 200 us window). */
 ff_status ff_server_create(const ff_server_config* cfg, ff_server** out);
 /* Device bytes a request occupies while admitted (its admission footprint):
- * density and potential, the FP32 refinement work area, and for want_norms
- * with a field descriptor the work area of the two norm reductions. */
+ * density and potential, the FP32 refinement work area or (FP64 grids whose
+ * z rows run in four steps) the z work array, and for want_norms with a field
+ * descriptor the work area of the two norm reductions. */
 size_t ff_server_footprint(const ff_request* r);
 /* Queues a request; FF_ERR_REJECTED if its footprint exceeds the budget.
  * The field descriptor is copied; f_host / phi_host must stay valid until the

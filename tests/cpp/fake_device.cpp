@@ -149,6 +149,7 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   return FF_OK;
 }
 void plan_destroy(ff_plan* p) { delete p; }
+bool z_rows_four_step(const int[3], int) { return false; }
 void plan_bind_scratch(ff_plan* p, void* base) { p->scratch = base; }
 
 ff_status solve_enqueue(const ff_plan* p, double* density, double* potential, hipStream_t s, hipEvent_t*) {
