@@ -128,13 +128,14 @@ hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, 
 // whether a recorder is installed on the calling thread
 bool launch_recording();
 
-// Row-local steps of z passes over four-step rows (ff_zsplit.hip; FP64):
-// the forward split (in place on io.dst), the inverse merge (src -> dst),
-// the inverse epilogue (packed rows src -> real dst) and the four-step
-// twiddle between the sub-line passes (forward or conjugate, in place on
-// io.dst; a.tw = W_M, a.m1).  a.nline = M = nz/2.  Never recorded: refused
-// while a recorder is installed.
-enum { ZSPLIT_FWD = 0, ZSPLIT_INV = 1, ZSPLIT_EPI = 2, ZSPLIT_TWF = 3, ZSPLIT_TWI = 4 };
+// Row-local steps of z passes over four-step rows (ff_zsplit.hip; f32:
+// FP32 data): the forward split (in place on io.dst), the inverse merge
+// (src -> dst), the inverse epilogue (packed rows src -> real dst, out_kind
+// and epi as for launch_c2r), the four-step twiddle between the sub-line
+// passes (forward or conjugate, in place on io.dst; a.tw = W_M, a.m1) and
+// the rounding of FP64 real rows to FP32 (src -> dst).  a.nline = M = nz/2.
+// Never recorded: refused while a recorder is installed.
+enum { ZSPLIT_FWD = 0, ZSPLIT_INV = 1, ZSPLIT_EPI = 2, ZSPLIT_TWF = 3, ZSPLIT_TWI = 4, ZSPLIT_PACK = 5 };
 hipError_t launch_zsplit(int step, int f32, int out_kind, int epi, const RowArgs& a, hipStream_t s);
 
 // ---- driver kernels ---------------------------------------------------------

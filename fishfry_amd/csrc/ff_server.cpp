@@ -120,13 +120,16 @@ size_t norm_bytes() { return 2 * sizeof(double) * kDiffDoubles; }
 int size_class(const ff_request& r) { return ff::size_class(r.precision, r.refine_steps, r.n); }
 
 // Device work area of a request's solve beyond density and potential: the
-// FP32 refinement work area, or the work array of FP64 grids whose z rows
-// run in four steps (0: none).
-size_t work_bytes(const ff_request& r) {
+// FP32 refinement work area, then the work array of grids whose z rows run
+// in four steps (0: none).
+size_t refine_bytes(const ff_request& r) {
   const size_t N = (size_t)r.n[0] * r.n[1] * r.n[2];
   const size_t Nh = (size_t)r.n[0] * r.n[1] * (r.n[2] / 2 + 1);
-  if (r.precision == FF_FP32_REFINE) return Nh * 16 + align_up(N * 4, 256) + 256;
-  return z_rows_four_step(r.n, r.precision) ? Nh * 16 : 0;
+  return r.precision == FF_FP32_REFINE ? align_up(Nh * 16 + align_up(N * 4, 256) + 256, 256) : 0;
+}
+size_t work_bytes(const ff_request& r) {
+  const size_t Nh = (size_t)r.n[0] * r.n[1] * (r.n[2] / 2 + 1);
+  return refine_bytes(r) + (z_rows_four_step(r.n, r.precision) ? Nh * 16 : 0);
 }
 
 ff_plan* plan_for(ff_server* s, const Req& q, ff_status& st) {
@@ -207,7 +210,7 @@ void dispatch(ff_server* s, std::vector<Req*>& group, int k) {
     ff_plan p = *cached;  // shares twiddles; the work area is the request's own
     if (q->has_s) {
       if (p.opt.precision == FF_FP32_REFINE) plan_bind_scratch(&p, base + q->off_s);
-      else p.work = base + q->off_s;  // four-step z rows
+      if (p.fz.n) p.work = base + q->off_s + refine_bytes(q->r);  // four-step z rows
     }
     double* d = reinterpret_cast<double*>(base + q->off_d);
     double* phi = reinterpret_cast<double*>(base + q->off_p);

@@ -7,6 +7,7 @@
 //   (:259,:263-285,:288), y inverse (:334), z inverse (Z2D, :380) with the
 //   1/(nx*ny*nz) scale of :466-481 in its epilogue.
 #include <algorithm>
+#include <initializer_list>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -54,6 +55,12 @@ static double dd_const(int n, double lo, double hi) {
 // n2 columns and the n2-point ones over n1.
 static bool four_step_cfg(int n, int ncols, int eb, int kind, int prec, ff_plan::FourStep& fs,
                           bool zrows = false) {
+  if (fs.n == n) {
+    // another precision of the plan chose the split: the same one here
+    const int n1 = fs.n1, n2 = fs.n2;
+    return col_cfg_with(n1, zrows ? n2 : ncols, eb, fs.c1[prec], 0, 0, FF_TUNE_GENERIC, kind) &&
+           col_cfg_with(n2, zrows ? n1 : ncols, eb, fs.c2[prec], 0, 0, FF_TUNE_GENERIC, kind);
+  }
   auto cost = [](const ColCfg& c) { return double(c.line.lp.n) * (double(c.line.lp.np) + 0.04 * c.line.E); };
   double best = 1e300;
   int b1 = 0;
@@ -160,12 +167,23 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     ColCfg c;
     if (!make_col_cfg(n[1], p->nh, eb, c, 1, prec)) four_step_cfg(n[1], p->nh, eb, 1, prec, p->fy);
     if (!make_col_cfg(n[0], p->nh, eb, c, 2, prec)) four_step_cfg(n[0], p->nh, eb, 2, prec, p->fx);
-    // even nz whose packed rows (nz/2 complex values) no workgroup
-    // transforms whole: four-step rows
-    RowCfg r0, r1;
-    if (n[2] % 2 == 0 && (!make_row_cfg(n[2], eb, r0, 0, prec) || !make_row_cfg(n[2], eb, r1, 3, prec)))
-      four_step_cfg(n[2] / 2, 0, eb, 1, prec, p->fz, true);
   }
+  // an axis in four steps in one precision runs so in all of the plan's
+  for (int prec = 0; prec < nprec; ++prec) {
+    const int eb = prec ? 8 : 16;
+    if (p->fy.n && !p->fy.c1[prec].threads) four_step_cfg(n[1], p->nh, eb, 1, prec, p->fy);
+    if (p->fx.n && !p->fx.c1[prec].threads) four_step_cfg(n[0], p->nh, eb, 2, prec, p->fx);
+  }
+  // even nz whose packed rows (nz/2 complex values) no workgroup transforms
+  // whole, in any of the plan's precisions: four-step rows in all of them
+  bool zfour = false;
+  for (int prec = 0; prec < nprec; ++prec) {
+    RowCfg r0, r1;
+    const int eb = prec ? 8 : 16;
+    if (n[2] % 2 == 0 && (!make_row_cfg(n[2], eb, r0, 0, prec) || !make_row_cfg(n[2], eb, r1, 3, prec))) zfour = true;
+  }
+  for (int prec = 0; prec < nprec && zfour; ++prec)
+    if (!four_step_cfg(n[2] / 2, 0, prec ? 8 : 16, 1, prec, p->fz, true)) p->fz.c1[prec].threads = 0;
   const bool four = p->fy.n || p->fx.n || p->fz.n;
   if (four) {
     if (o.flags & (FF_PENCIL_LAYOUT | FF_BLOCKED_LAYOUT | FF_SLAB_PIPELINE)) {
@@ -173,12 +191,6 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
       return set_error(FF_ERR_UNSUPPORTED, "extents transformed in four steps take the plain spectrum layout only");
     }
     p->pencil = 0;
-  }
-  if (p->fz.n && o.precision != FF_FP64) {
-    delete p;
-    char buf[160];
-    snprintf(buf, sizeof buf, "nz = %d: z rows transformed in four steps run in FP64 plans only", n[2]);
-    return set_error(FF_ERR_UNSUPPORTED, buf);
   }
   for (int prec = 0; prec < nprec; ++prec) {
     const int eb = prec ? 8 : 16;
@@ -311,7 +323,7 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   }
   // (four-step z rows: own_scratch false leaves the work array to the caller,
   // who points the plan, or a copy of it, at 16*Nh bytes of its own)
-  if (o.precision == FF_FP64 && ((o.flags & FF_PRESERVE_INPUT) || (p->fz.n && own_scratch))) {
+  if ((o.precision == FF_FP64 && (o.flags & FF_PRESERVE_INPUT)) || (p->fz.n && own_scratch)) {
     st = hip_check(hipMalloc(&p->work, (size_t)p->Nh * 16), "work array hipMalloc");
     if (st != FF_OK) {
       plan_destroy(p);
@@ -323,9 +335,13 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
 }
 
 bool z_rows_four_step(const int n[3], int precision) {
-  if (precision != FF_FP64 || n[2] % 2) return false;
-  RowCfg r0, r1;
-  return !make_row_cfg(n[2], 16, r0, 0, 0) || !make_row_cfg(n[2], 16, r1, 3, 0);
+  if (n[2] % 2) return false;
+  for (int prec = 0; prec < (precision == FF_FP64 ? 1 : 2); ++prec) {
+    RowCfg r0, r1;
+    const int eb = prec ? 8 : 16;
+    if (!make_row_cfg(n[2], eb, r0, 0, prec) || !make_row_cfg(n[2], eb, r1, 3, prec)) return true;
+  }
+  return false;
 }
 
 size_t plan_scratch_bytes(const ff_plan* p) {
@@ -459,12 +475,14 @@ static RowArgs row_args(const ff_plan* p, int prec, bool inverse) {
 //      order, frequency k2 + m2*k1, into rows of nh;
 // then the real-to-complex split in place (ff_zsplit.hip).  The inverse
 // merges the half spectrum into packed rows of nh, runs B^-1 into rows of M,
-// the conjugate twiddle and A^-1 in place, and the epilogue.  FP64, whole
-// passes, plain layout; the plan's work array holds the rows of M.
-static ff_status run_zsub(const ff_plan* p, int mode, bool stepA, const void* src, void* dst, hipStream_t s) {
+// the conjugate twiddle and A^-1 in place, and the epilogue.  Whole passes,
+// plain layout; the plan's work array (16*Nh bytes) holds the rows of M or
+// the merged rows.  prec: 0 FP64, 1 FP32.
+static ff_status run_zsub(const ff_plan* p, int prec, int mode, bool stepA, const void* src, void* dst,
+                          hipStream_t s) {
   const ff_plan::FourStep& fs = p->fz;
   const long long m1 = fs.n1, m2 = fs.n2, M = fs.n;
-  const ColCfg& c = stepA ? fs.c2[0] : fs.c1[0];
+  const ColCfg& c = stepA ? fs.c2[prec] : fs.c1[prec];
   auto side = [](long long KS, long long fo, long long fr) {
     SpecSide sd;
     memset(&sd, 0, sizeof sd);
@@ -498,20 +516,20 @@ static ff_status run_zsub(const ff_plan* p, int mode, bool stepA, const void* sr
   a.ps = c.ps;
   a.nx = p->n[0];
   a.ny = p->n[1];
-  a.tw = stepA ? fs.tw2[0] : fs.tw1[0];
+  a.tw = stepA ? fs.tw2[prec] : fs.tw1[prec];
   a.lp = c.line.lp;
   a.nline = c.line.n;
-  const void* const* bl = stepA ? fs.bl2[0] : fs.bl1[0];
+  const void* const* bl = stepA ? fs.bl2[prec] : fs.bl1[prec];
   a.chirp = bl[0];
   a.bsf = bl[1];
   a.bsi = bl[2];
   const Launch l{dim3(a.nblocks), dim3(c.threads), c.lds, s};
-  return hip_check(launch_col(c.line.E, 0, mode, c.split, c.groups, c.line.bluestein(), c.stat, l, a),
+  return hip_check(launch_col(c.line.E, prec, mode, c.split, c.groups, c.line.bluestein(), c.stat, l, a),
                    "z four-step launch");
 }
 
 // Arguments of the row-local steps of four-step z rows.
-static RowArgs zsplit_args(const ff_plan* p, const void* src, void* dst) {
+static RowArgs zsplit_args(const ff_plan* p, int prec, const void* src, void* dst) {
   RowArgs a;
   memset(&a, 0, sizeof a);
   a.src = src;
@@ -523,32 +541,82 @@ static RowArgs zsplit_args(const ff_plan* p, const void* src, void* dst) {
   a.nline = p->fz.n;
   a.m1 = p->fz.n1;
   a.scale = 1.0 / (double(p->n[0]) * double(p->n[1]) * double(p->n[2]));  // ParisPencil.cpp:408
-  a.twpost = p->tw_zpost[0];
-  a.tw = p->fz.twn[0];
+  a.twpost = p->tw_zpost[prec];
+  a.tw = p->fz.twn[prec];
   return a;
 }
 
-// src_work: the source may be the work array (the inverse's merge reads it
-// before B^-1 overwrites it)
-static ff_status check_zfour(const ff_plan* p, int prec, const void* src, const void* dst, long long nrows,
-                             bool src_work) {
-  if (prec || nrows >= 0) return set_error(FF_ERR_INVALID, "z rows in four steps: whole FP64 passes only");
+static ff_status zsplit(int step, int prec, int out_kind, int epi, const RowArgs& a, hipStream_t s) {
+  return hip_check(launch_zsplit(step, prec, out_kind, epi, a, s), "z four-step row-local launch");
+}
+
+// The buffers of a four-step z pass must differ, except where noted.
+static ff_status check_zfour(const ff_plan* p, long long nrows, std::initializer_list<const void*> distinct) {
+  if (nrows >= 0) return set_error(FF_ERR_INVALID, "z rows in four steps: whole passes only");
   if (!p->work) return set_error(FF_ERR_INVALID, "z rows in four steps need the plan's work array");
-  if (src == dst || dst == p->work || (src == p->work && !src_work))
-    return set_error(FF_ERR_INVALID, "z rows in four steps: source, destination and work array must differ");
+  for (auto x = distinct.begin(); x != distinct.end(); ++x)
+    for (auto y = x + 1; y != distinct.end(); ++y)
+      if (*x == *y)
+        return set_error(FF_ERR_INVALID, "z rows in four steps: source, destination and work array must differ");
   return FF_OK;
+}
+
+// Forward: A from src (rows of M; FP64 input of an FP32 pass is first
+// rounded into the work array) into the work array, the twiddle, B into dst,
+// the split in dst.  src is read only.
+static ff_status zfwd_four(const ff_plan* p, int prec, int in_f32, const void* src, void* dst, hipStream_t s) {
+  void* w = p->work;
+  ff_status st;
+  const void* a_in = src;
+  if (prec && !in_f32) {
+    if ((st = zsplit(ZSPLIT_PACK, 1, 0, 0, zsplit_args(p, 1, src, w), s))) return st;
+    a_in = w;
+  }
+  if ((st = run_zsub(p, prec, COL_FWD, true, a_in, w, s))) return st;
+  if ((st = zsplit(ZSPLIT_TWF, prec, 0, 0, zsplit_args(p, prec, w, w), s))) return st;
+  if ((st = run_zsub(p, prec, COL_FWD, false, w, dst, s))) return st;
+  return zsplit(ZSPLIT_FWD, prec, 0, 0, zsplit_args(p, prec, dst, dst), s);
+}
+
+// Inverse.  The stored FP64 pass keeps src: the merge into dst, B^-1 into the
+// work array, the twiddle and A^-1 there, the epilogue into dst (src may be
+// the work array itself).  The other epilogues, whose dst holds phi or a
+// residual too small for the merged rows, use src as their rows of M: the
+// merge into the work array, B^-1 back into src, the twiddle and A^-1 there,
+// the epilogue from src.
+static ff_status zinv_four(const ff_plan* p, int prec, int out_kind, int epi, const void* src, void* dst,
+                           const double* f, const double* shiftp, hipStream_t s) {
+  void* w = p->work;
+  ff_status st;
+  RowArgs e;
+  void* rows;
+  if (!prec && !out_kind && epi == EPI_STORE && !f && !shiftp) {
+    if ((st = zsplit(ZSPLIT_INV, 0, 0, 0, zsplit_args(p, 0, src, dst), s))) return st;
+    if ((st = run_zsub(p, 0, COL_INV, false, dst, w, s))) return st;
+    rows = w;
+  } else {
+    void* srcw = const_cast<void*>(src);
+    if ((st = zsplit(ZSPLIT_INV, prec, 0, 0, zsplit_args(p, prec, src, w), s))) return st;
+    if ((st = run_zsub(p, prec, COL_INV, false, w, srcw, s))) return st;
+    rows = srcw;
+  }
+  if ((st = zsplit(ZSPLIT_TWI, prec, 0, 0, zsplit_args(p, prec, rows, rows), s))) return st;
+  if ((st = run_zsub(p, prec, COL_INV, true, rows, rows, s))) return st;
+  e = zsplit_args(p, prec, rows, dst);
+  e.aux = f;
+  if (shiftp) {
+    e.shiftp = shiftp;
+    e.shift = e.scale;  // shift = (sum of phi) / N
+  }
+  return zsplit(ZSPLIT_EPI, prec, out_kind, epi, e, s);
 }
 
 ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void* dst, hipStream_t s,
                    long long row0, long long nrows) {
   if (p->fz.n) {
-    ff_status st = check_zfour(p, prec || in_f32, src, dst, nrows, false);
+    ff_status st = check_zfour(p, nrows, {src, dst, p->work});
     if (st) return st;
-    if ((st = run_zsub(p, COL_FWD, true, src, p->work, s))) return st;
-    if ((st = hip_check(launch_zsplit(ZSPLIT_TWF, 0, 0, 0, zsplit_args(p, p->work, p->work), s), "z twiddle launch")))
-      return st;
-    if ((st = run_zsub(p, COL_FWD, false, p->work, dst, s))) return st;
-    return hip_check(launch_zsplit(ZSPLIT_FWD, 0, 0, 0, zsplit_args(p, dst, dst), s), "z split launch");
+    return zfwd_four(p, prec, in_f32, src, dst, s);
   }
   RowArgs a = row_args(p, prec, false);
   const RowCfg& c = p->zr[prec];
@@ -566,18 +634,13 @@ ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void
 ff_status run_zinv(const ff_plan* p, int prec, int out_kind, int epi, const void* src, void* dst, const double* f,
                    const double* shiftp, hipStream_t s, int lay_in, long long row0, long long nrows) {
   if (p->fz.n) {
-    ff_status st = check_zfour(p, prec, src, dst, nrows, true);
+    if (lay_in != LAY_PLAIN) return set_error(FF_ERR_INVALID, "z rows in four steps: plain layout only");
+    // (the stored FP64 pass may read the work array itself)
+    const bool keep_src = !prec && !out_kind && epi == EPI_STORE && !f && !shiftp;
+    ff_status st = keep_src ? check_zfour(p, nrows, {src, dst}) : check_zfour(p, nrows, {src, dst, p->work});
+    if (!st && keep_src && dst == p->work) st = check_zfour(p, nrows, {dst, p->work});
     if (st) return st;
-    if (out_kind || epi != EPI_STORE || f || shiftp || lay_in != LAY_PLAIN)
-      return set_error(FF_ERR_INVALID, "z rows in four steps: the stored FP64 inverse only");
-    if ((st = hip_check(launch_zsplit(ZSPLIT_INV, 0, 0, 0, zsplit_args(p, src, dst), s), "z merge launch")))
-      return st;
-    if ((st = run_zsub(p, COL_INV, false, dst, p->work, s))) return st;
-    if ((st = hip_check(launch_zsplit(ZSPLIT_TWI, 0, 0, 0, zsplit_args(p, p->work, p->work), s), "z twiddle launch")))
-      return st;
-    if ((st = run_zsub(p, COL_INV, true, p->work, p->work, s))) return st;
-    return hip_check(launch_zsplit(ZSPLIT_EPI, 0, 0, EPI_STORE, zsplit_args(p, p->work, dst), s),
-                     "z epilogue launch");
+    return zinv_four(p, prec, out_kind, epi, src, dst, f, shiftp, s);
   }
   RowArgs a = row_args(p, prec, true);
   a.spec = spec_side(p, 1, lay_in);

@@ -5,6 +5,7 @@
 // epilogue.  The whole-row kernels (k_r2c, k_c2r) do the same arithmetic on
 // rows held in LDS (ParisPencil.cpp:188-189 and 402-403: D2Z and Z2D along z).
 #include <algorithm>
+#include <type_traits>
 
 #include "ff_internal.hpp"
 #include "ff_kernels.hpp"
@@ -105,6 +106,18 @@ __global__ __launch_bounds__(kZThreads) void k_ztwiddle(const RowArgs a) {
   }
 }
 
+// FP64 real rows -> FP32 (the packed rows of an FP32 pass whose input is
+// FP64): N values, src -> dst.
+__global__ __launch_bounds__(kZThreads) void k_zpack(const RowArgs a) {
+  const PassInst io = inst_of(a);
+  const long long total = 2 * a.nrows * a.nline;
+  const FF_GLB double* x = glb<const double>(io.src);
+  FF_GLB float* y = glb<float>(io.dst);
+  for (long long it = (long long)blockIdx.x * kZThreads + threadIdx.x; it < total;
+       it += (long long)gridDim.x * kZThreads)
+    y[it] = float(x[it]);
+}
+
 // Inverse epilogue: the packed rows z (M complex values per row, rows
 // contiguous, so z is nrows*M pairs of real values in order) to the real
 // output, stored, as the residual or accumulated (c2r_out2).
@@ -123,26 +136,39 @@ __global__ __launch_bounds__(kZThreads) void k_zepi(const RowArgs a) {
   }
 }
 
+template <typename F>
+const void* zsplit_fn(int step, int out_kind, int epi) {
+  switch (step) {
+    case ZSPLIT_FWD: return (const void*)&k_zsplit_fwd<F>;
+    case ZSPLIT_INV: return (const void*)&k_zsplit_inv<F>;
+    case ZSPLIT_TWF: return (const void*)&k_ztwiddle<F, -1>;
+    case ZSPLIT_TWI: return (const void*)&k_ztwiddle<F, +1>;
+    case ZSPLIT_PACK: return std::is_same<F, float>::value ? (const void*)&k_zpack : nullptr;
+    case ZSPLIT_EPI:
+      // the epilogues the passes use: FP64 stored or residual (FP64 or FP32
+      // output), FP32 stored or accumulated into FP64 phi
+      if constexpr (std::is_same<F, double>::value) {
+        if (epi == EPI_STORE && out_kind == 0) return (const void*)&k_zepi<double, double, EPI_STORE>;
+        if (epi == EPI_RESIDUAL && out_kind == 1) return (const void*)&k_zepi<double, float, EPI_RESIDUAL>;
+        if (epi == EPI_RESIDUAL && out_kind == 0) return (const void*)&k_zepi<double, double, EPI_RESIDUAL>;
+      } else {
+        if (epi == EPI_STORE && out_kind == 0) return (const void*)&k_zepi<float, double, EPI_STORE>;
+        if (epi == EPI_ACCUM && out_kind == 0) return (const void*)&k_zepi<float, double, EPI_ACCUM>;
+      }
+      return nullptr;
+    default: return nullptr;
+  }
+}
+
 }  // namespace
 
 hipError_t launch_zsplit(int step, int f32, int out_kind, int epi, const RowArgs& a, hipStream_t s) {
-  if (f32 || launch_recording()) return hipErrorInvalidValue;  // FP64 plans, never recorded
-  const void* fn = nullptr;
-  long long total = 0;
-  if (step == ZSPLIT_FWD) {
-    fn = (const void*)&k_zsplit_fwd<double>;
-    total = a.nrows * (a.nline / 2 + 1);
-  } else if (step == ZSPLIT_INV) {
-    fn = (const void*)&k_zsplit_inv<double>;
-    total = a.nrows * a.nline;
-  } else if (step == ZSPLIT_TWF || step == ZSPLIT_TWI) {
-    fn = step == ZSPLIT_TWF ? (const void*)&k_ztwiddle<double, -1> : (const void*)&k_ztwiddle<double, +1>;
-    total = a.nrows * a.nline;
-  } else if (step == ZSPLIT_EPI && out_kind == 0 && epi == EPI_STORE) {
-    fn = (const void*)&k_zepi<double, double, EPI_STORE>;
-    total = a.nrows * a.nline;
-  }
+  if (launch_recording()) return hipErrorInvalidValue;  // never recorded
+  const void* fn = f32 ? zsplit_fn<float>(step, out_kind, epi) : zsplit_fn<double>(step, out_kind, epi);
   if (!fn) return hipErrorInvalidValue;
+  // items: the split's pairs, else the rows' complex values (real values for the pack)
+  const long long total = step == ZSPLIT_FWD ? a.nrows * (a.nline / 2 + 1)
+                                             : (step == ZSPLIT_PACK ? 2 : 1) * a.nrows * a.nline;
   void* args[] = {const_cast<RowArgs*>(&a)};
   return hipLaunchKernel(fn, dim3(zsplit_blocks(total)), dim3(kZThreads), args, 0, s);
 }

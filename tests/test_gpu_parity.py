@@ -580,7 +580,8 @@ def test_four_step_z_rows_match_oracle(n):
     real-to-complex split and the complex-to-real merge as row-local steps:
     the solve, the z forward pass (pass 0) and the z inverse pass (pass 4)
     match the oracle; the density is left unchanged; the allocation-free
-    entry point gives the same bits."""
+    entry point gives the same bits; FP32 + 2 steps (FP32 passes and FP64
+    residual passes through the same work array) converges to the oracle."""
     rng = np.random.default_rng(n[2] + n[1])
     f = rng.standard_normal(n)
     lo, hi, _ = ff.benchmark_grid(n)
@@ -615,12 +616,15 @@ def test_four_step_z_rows_match_oracle(n):
         plan.solve_pass(4, a, a)  # the merge cannot run in place
     assert e.value.status == ff.FF_ERR_INVALID
     plan.close()
+    got32 = gpu_solve(f, precision=ff.FF_FP32_REFINE, refine_steps=2)
+    assert rel(got32, oracle.solve(f)) < 1e-12, rel(got32, oracle.solve(f))
 
 
 @pytest.mark.parametrize("n", [(2, 4100, 16400), (4100, 2, 16400)])
 def test_four_step_z_with_four_step_columns(n):
     """Four-step z rows with four-step y (or x) lines in one plan (1.3e8
-    points; largest prime factor 41, so the oracle finishes in seconds)."""
+    points; largest prime factor 41, so the oracle finishes in seconds), FP64
+    and FP32 + 2 steps."""
     rng = np.random.default_rng(n[0])
     f = rng.standard_normal(n)
     plan, nb, a, b = make(n)
@@ -631,17 +635,19 @@ def test_four_step_z_with_four_step_columns(n):
     got = gpu_solve(f)
     want = oracle.solve(f)
     assert rel(got, want) < TOL, rel(got, want)
+    got32 = gpu_solve(f, precision=ff.FF_FP32_REFINE, refine_steps=2)
+    assert rel(got32, want) < 1e-12, rel(got32, want)
 
 
 def test_four_step_z_limits():
-    """Four-step z rows: FP64 plans in the plain layout; FP32 plans, the
-    pencil, blocked and slab layouts and the batch's recorded modes are
-    refused with a message; the stream-parallel batch runs them and matches
-    the single-plan solve bit for bit."""
+    """Four-step z rows run in the plain layout: the pencil, blocked and slab
+    layouts and the batch's recorded modes are refused with a message; the
+    stream-parallel batch runs them and matches the single-plan solve bit for
+    bit."""
     n = (3, 2, 20000)
     lo, hi, _ = ff.benchmark_grid(n)
-    for kw in ({"precision": ff.FF_FP32_REFINE, "refine_steps": 1}, {"flags": ff.FF_PENCIL_LAYOUT},
-               {"flags": ff.FF_BLOCKED_LAYOUT}, {"flags": ff.FF_SLAB_PIPELINE}):
+    for kw in ({"flags": ff.FF_PENCIL_LAYOUT}, {"flags": ff.FF_BLOCKED_LAYOUT}, {"flags": ff.FF_SLAB_PIPELINE},
+               {"precision": ff.FF_FP32_REFINE, "refine_steps": 1, "flags": ff.FF_PENCIL_LAYOUT}):
         with pytest.raises(ff.FishFryError) as e:
             ff.ParisPencil(n, lo, hi, **kw)
         assert e.value.status == ff.FF_ERR_UNSUPPORTED, kw

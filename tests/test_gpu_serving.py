@@ -87,17 +87,18 @@ def test_host_requests_match_oracle():
 
 def test_four_step_z_requests():
     """Grids whose z rows run in four steps get their z work array from the
-    arena per request (the footprint counts it), so same-shape requests on
-    different streams do not share one: every result matches the oracle.  An
-    FP32 request of such a grid completes with FF_ERR_UNSUPPORTED."""
+    arena per request (the footprint counts it, after the FP32 refinement
+    work area), so same-shape requests on different streams do not share
+    one: every result matches the oracle, FP64 and FP32 + 2 steps."""
     rng = np.random.default_rng(14)
     n = (3, 2, 20000)
     srv = ff.Server(hbm_budget=1 << 30, streams=4, max_batch=1)
     nh = n[2] // 2 + 1
     block = lambda b: (b + 255) // 256 * 256
     assert srv.footprint(n) == 3 * block(16 * n[0] * n[1] * nh)
+    refine = block(16 * n[0] * n[1] * nh + block(4 * n[0] * n[1] * n[2]) + 256)
     assert srv.footprint(n, ff.FF_FP32_REFINE, 2) == \
-        2 * block(16 * n[0] * n[1] * nh) + block(16 * n[0] * n[1] * nh + block(4 * n[0] * n[1] * n[2]) + 256)
+        2 * block(16 * n[0] * n[1] * nh) + block(refine + 16 * n[0] * n[1] * nh)
     fs = [rng.standard_normal(n) for _ in range(6)]
     phis = [np.empty(n) for _ in fs]
     ids = [srv.submit(n, f_host=f, phi_host=p) for f, p in zip(fs, phis)]
@@ -106,10 +107,13 @@ def test_four_step_z_requests():
     for f, p in zip(fs, phis):
         want = oracle.solve(f)
         assert np.linalg.norm(p - want) / np.linalg.norm(want) < 1e-13
-    rid = srv.submit(n, ff.FF_FP32_REFINE, 1, f_host=fs[0], phi_host=np.empty(n))
-    with pytest.raises(ff.FishFryError) as e:
-        srv.wait(rid)
-    assert e.value.status == ff.FF_ERR_UNSUPPORTED
+    phis32 = [np.empty(n) for _ in range(3)]
+    ids = [srv.submit(n, ff.FF_FP32_REFINE, 2, f_host=f, phi_host=p) for f, p in zip(fs, phis32)]
+    for rid in ids:
+        assert srv.wait(rid)["status"] == 0
+    for f, p in zip(fs, phis32):
+        want = oracle.solve(f)
+        assert np.linalg.norm(p - want) / np.linalg.norm(want) < 1e-12
     srv.close()
 
 
