@@ -279,6 +279,31 @@ __device__ __forceinline__ cx<F> tw_at(const TwHalf<F, H>& tw, int t) {
   return hi ? cx<F>{-w.x, -w.y} : w;
 }
 
+// Row tiles: line-major with a padded pitch.
+struct RowIdx {
+  int base, ps;
+  __device__ __forceinline__ int operator()(int r) const { return base + r + (r >> ps); }
+};
+// A row held whole by the lanes of one wavefront (its length / E divides 64):
+// the exchanges between its passes touch only that wavefront's lanes and LDS
+// row, so they need a wavefront-level barrier, not a workgroup one, and the
+// workgroup's rows proceed independently.
+struct RowIdxW {
+  int base, ps;
+  __device__ __forceinline__ int operator()(int r) const { return base + r + (r >> ps); }
+};
+// The barrier around an exchange through LDS laid out by Idx.
+template <typename Idx>
+__device__ __forceinline__ void exchange_sync() {
+  __syncthreads();
+}
+template <>
+__device__ __forceinline__ void exchange_sync<RowIdxW>() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Butterflies of one pass on registers (twiddles applied to the inputs).
 template <int E, int R, int DIR, typename F, typename TW>
 __device__ __forceinline__ void pass_regs(cx<F> (&v)[E], int r0, int L, int ns, uint32_t nsinv, int twstep,
@@ -364,19 +389,19 @@ __device__ __forceinline__ void run_pass(cx<F> (&v)[E], const LinePlan& lp, int 
     pass_regs<E, R, DIR, F>(v, r0, lp.L, lp.ns[p], lp.nsinv[p], lp.twstep[p], tw);
     if (!last) {
       if constexpr (!SPLIT) {
-        __syncthreads();
+        exchange_sync<Idx>();
         scatter<E, R, F>(v, lds, idx, r0, lp.L, lp.ns[p], lp.nsinv[p]);
-        __syncthreads();
+        exchange_sync<Idx>();
         gather<E, F>(v, lds, idx, r0, lp.L);
       } else {
         F* l = reinterpret_cast<F*>(lds);
-        __syncthreads();
+        exchange_sync<Idx>();
         scatter_part<E, R, 0, F>(v, l, idx, r0, lp.L, lp.ns[p], lp.nsinv[p]);
-        __syncthreads();
+        exchange_sync<Idx>();
         gather_part<E, 0, F>(v, l, idx, r0, lp.L);
-        __syncthreads();
+        exchange_sync<Idx>();
         scatter_part<E, R, 1, F>(v, l, idx, r0, lp.L, lp.ns[p], lp.nsinv[p]);
-        __syncthreads();
+        exchange_sync<Idx>();
         gather_part<E, 1, F>(v, l, idx, r0, lp.L);
       }
     }
@@ -488,7 +513,7 @@ __device__ __forceinline__ void fft_line_static(cx<F> (&v)[E], const TW& tw, cx<
 }
 
 // LDS layouts.  Column tiles: element (line c, position r) at r*T + c plus one
-// pad slot every 2^PS positions.  Row tiles: line-major with a padded pitch.
+// pad slot every 2^PS positions (row tiles: RowIdx above).
 // The column pad shift of every column kernel:
 #ifndef FF_COL_PS
 #define FF_COL_PS 4
@@ -497,10 +522,6 @@ constexpr int kColPadShift = FF_COL_PS;
 struct ColIdx {
   int T, c, ps;
   __device__ __forceinline__ int operator()(int r) const { return r * T + c + (r >> ps); }
-};
-struct RowIdx {
-  int base, ps;
-  __device__ __forceinline__ int operator()(int r) const { return base + r + (r >> ps); }
 };
 
 }  // namespace ff

@@ -12,6 +12,8 @@
 //   k_c2r   z inverse + the 1/(nx*ny*nz) scale                (:380, :406-421)
 #pragma once
 
+#include <type_traits>
+
 #include <climits>
 
 #include "ff_fft.hpp"
@@ -588,7 +590,9 @@ __device__ __forceinline__ void r2c_tile(const RowArgs& a, const PassInst& io, i
   const int M = a.nline;                  // transform length (the engine's lp.n is longer for Bluestein)
   constexpr bool blue = BL;               // positions >= M exist only for Bluestein
   const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
-  const RowIdx idx{line * a.pitch, a.ps};
+  // (compiled-in plans whose rows fit a wavefront: wavefront-level exchanges)
+  using RI = std::conditional_t<(SN > 0 && 64 % (SN > 0 ? SN / E : 64) == 0), RowIdxW, RowIdx>;
+  const RI idx{line * a.pitch, a.ps};
   cx<F> v[E];
   if constexpr (!ODD) {
     const FF_GLB IN* __restrict__ src = glb<const IN>(io.src) + rowc * a.nz;
@@ -611,13 +615,13 @@ __device__ __forceinline__ void r2c_tile(const RowArgs& a, const PassInst& io, i
       static_assert(SN > 0 && !BL, "LDS twiddles: compiled-in plans");
       const TwHalf<F, SN / 2> twl = row_twiddles_to_lds<F, SN>(a, lds);
       __syncthreads();
-      line_dft<E, -1, F, RowIdx, false, BL, SN>(v, a.lp, M, twl, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+      line_dft<E, -1, F, RI, false, BL, SN>(v, a.lp, M, twl, a.chirp, a.bsf, a.bsi, lds, idx, r0);
     } else {
-      line_dft<E, -1, F, RowIdx, false, BL, SN>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+      line_dft<E, -1, F, RI, false, BL, SN>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
     }
-    __syncthreads();
+    exchange_sync<RI>();
     put_natural<E, F>(v, lds, idx, r0, L);
-    __syncthreads();
+    exchange_sync<RI>();
     const cx<F>* __restrict__ twp = reinterpret_cast<const cx<F>*>(a.twpost);
     FF_GLB cx<F>* __restrict__ dst = glb<cx<F>>(io.dst) + (valid ? spec_row(a, row) : 0);
     if (valid) {
@@ -644,7 +648,7 @@ __device__ __forceinline__ void r2c_tile(const RowArgs& a, const PassInst& io, i
       const int k = r0 + i * L;
       v[i] = cx<F>{(!blue || k < M) ? F(src[k]) : F(0), F(0)};
     }
-    line_dft<E, -1, F, RowIdx, false, BL, SN>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    line_dft<E, -1, F, RI, false, BL, SN>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
     FF_GLB cx<F>* __restrict__ dst = glb<cx<F>>(io.dst) + (valid ? spec_row(a, row) : 0);
     if (valid) {
 #pragma unroll
@@ -712,7 +716,10 @@ __device__ __forceinline__ void c2r_tile(const RowArgs& a, const PassInst& io, i
   const int M = a.nline;                // transform length (the engine's lp.n is longer for Bluestein)
   constexpr bool blue = BL;             // positions >= M exist only for Bluestein
   const cx<F>* __restrict__ tw = reinterpret_cast<const cx<F>*>(a.tw);
-  const RowIdx idx{line * a.pitch, a.ps};
+  // (compiled-in plans whose rows fit a wavefront: wavefront-level exchanges
+  // after the staging)
+  using RI = std::conditional_t<(SN > 0 && 64 % (SN > 0 ? SN / E : 64) == 0), RowIdxW, RowIdx>;
+  const RI idx{line * a.pitch, a.ps};
   // stage the half spectrum of the workgroup's rows in LDS; rows past the end
   // load the last row (in bounds), their stores are predicated
   if (a.spec.lgKT >= 30) {
@@ -820,9 +827,9 @@ __device__ __forceinline__ void c2r_tile(const RowArgs& a, const PassInst& io, i
     }
   }
   if constexpr (TWL) {
-    line_dft<E, +1, F, RowIdx, false, BL, SN>(v, a.lp, M, twl, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    line_dft<E, +1, F, RI, false, BL, SN>(v, a.lp, M, twl, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   } else {
-    line_dft<E, +1, F, RowIdx, false, BL, SN>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
+    line_dft<E, +1, F, RI, false, BL, SN>(v, a.lp, M, tw, a.chirp, a.bsf, a.bsi, lds, idx, r0);
   }
   if (valid) {
     const double shift = io.shiftp ? a.shift * *glb<const double>(io.shiftp) : a.shift;
