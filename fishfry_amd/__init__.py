@@ -102,7 +102,8 @@ class Result(ctypes.Structure):
 class ServingStats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_longlong), ("rejected", ctypes.c_longlong), ("groups", ctypes.c_longlong),
                 ("p50_s", ctypes.c_double), ("p99_s", ctypes.c_double), ("points_per_s", ctypes.c_double),
-                ("arena_bytes", ctypes.c_size_t), ("arena_peak_bytes", ctypes.c_size_t)]
+                ("arena_bytes", ctypes.c_size_t), ("arena_peak_bytes", ctypes.c_size_t),
+                ("grouped_solves", ctypes.c_longlong)]
 
 
 FF_BATCH_GRAPH = 1

@@ -129,34 +129,6 @@ ff_status make_plans(ff_batch* b) {
   return FF_OK;
 }
 
-// The per-instance fields of a recorded launch.
-PassInst inst_fields(const PassLaunch& r) {
-  if (r.kind == 0)
-    return PassInst{r.col.src, r.col.dst, nullptr, r.col.dc_out, nullptr, r.col.ddi, r.col.ddj, r.col.ddk};
-  return PassInst{r.row.src, r.row.dst, r.row.aux, nullptr, r.row.shiftp, 0.0, 0.0, 0.0};
-}
-
-// Whether two recorded launches differ only in their per-instance fields and
-// stream (the recorded argument blocks are zero-filled before use, so their
-// bytes compare).
-bool same_launch(const PassLaunch& x, const PassLaunch& y) {
-  PassLaunch a, b;
-  memcpy(&a, &x, sizeof a);
-  memcpy(&b, &y, sizeof b);
-  for (PassLaunch* r : {&a, &b}) {
-    r->l.stream = nullptr;
-    r->col.src = nullptr;
-    r->col.dst = nullptr;
-    r->col.dc_out = nullptr;
-    r->col.ddi = r->col.ddj = r->col.ddk = 0.0;
-    r->row.src = nullptr;
-    r->row.dst = nullptr;
-    r->row.aux = nullptr;
-    r->row.shiftp = nullptr;
-  }
-  return memcmp(&a, &b, sizeof a) == 0;
-}
-
 // Records one solve per instance, forms the groups and uploads their
 // per-instance tables (blocking: waits for the device before rewriting them).
 ff_status prepare_groups(ff_batch* b, double* const* d, double* const* p) {
@@ -180,7 +152,7 @@ ff_status prepare_groups(ff_batch* b, double* const* d, double* const* p) {
           lt.refine_steps != it.refine_steps || rec[l].size() != rec[i].size())
         continue;
       bool same = true;
-      for (size_t k = 0; k < rec[i].size() && same; ++k) same = same_launch(rec[l][k], rec[i][k]);
+      for (size_t k = 0; k < rec[i].size() && same; ++k) same = launch_same(rec[l][k], rec[i][k]);
       if (same) {
         home = &g;
         break;
@@ -197,7 +169,7 @@ ff_status prepare_groups(ff_batch* b, double* const* d, double* const* p) {
   for (const BatchGroup& g : groups) {
     off.push_back(tab.size());
     for (size_t k = 0; k < rec[g.inst[0]].size(); ++k)
-      for (int i : g.inst) tab.push_back(inst_fields(rec[i][k]));
+      for (int i : g.inst) tab.push_back(launch_inst_fields(rec[i][k]));
   }
   // kernels of earlier solves may still read the old tables
   ff_status st = hip_check(hipDeviceSynchronize(), "batch table sync");
@@ -447,7 +419,7 @@ ff_status prepare_steps(ff_batch* b, double* const* d, double* const* p) {
       for (int i : rest) {
         std::vector<int>* home = nullptr;
         for (auto& c : clusters)
-          if (same_launch(rec[c[0]][s], rec[i][s])) {
+          if (launch_same(rec[c[0]][s], rec[i][s])) {
             home = &c;
             break;
           }
@@ -459,7 +431,7 @@ ff_status prepare_steps(ff_batch* b, double* const* d, double* const* p) {
       }
       for (const auto& c : clusters) {
         std::vector<PassInst> tab;
-        for (int i : c) tab.push_back(inst_fields(rec[i][s]));
+        for (int i : c) tab.push_back(launch_inst_fields(rec[i][s]));
         StepLaunch sl;
         sl.kind = -1;
         sl.op = rec[c[0]][s];

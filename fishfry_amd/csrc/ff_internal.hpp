@@ -122,6 +122,11 @@ struct PassLaunch {
 };
 void set_launch_recorder(std::vector<PassLaunch>* rec);
 hipError_t replay_launch(const PassLaunch& r);
+// The per-instance fields of a recorded launch, and whether two recorded
+// launches differ only in those fields and their stream (so that one launch
+// with a per-instance table runs both).
+PassInst launch_inst_fields(const PassLaunch& r);
+bool launch_same(const PassLaunch& x, const PassLaunch& y);
 hipError_t launch_r2c(int E, int f32, int in_f32, int odd, int blue, int stat, const Launch& l, const RowArgs& a);
 hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, int stat, const Launch& l,
                       const RowArgs& a);

@@ -702,6 +702,34 @@ hipError_t launch_c2r(int E, int f32, int out_kind, int odd, int epi, int blue, 
   return do_c2r(E, f32, out_kind, odd, epi, blue, stat, l, a);
 }
 
+// The per-instance fields of a recorded launch.
+PassInst launch_inst_fields(const PassLaunch& r) {
+  if (r.kind == 0)
+    return PassInst{r.col.src, r.col.dst, nullptr, r.col.dc_out, nullptr, r.col.ddi, r.col.ddj, r.col.ddk};
+  return PassInst{r.row.src, r.row.dst, r.row.aux, nullptr, r.row.shiftp, 0.0, 0.0, 0.0};
+}
+
+// Whether two recorded launches differ only in their per-instance fields and
+// stream (the recorded argument blocks are zero-filled before use, so their
+// bytes compare).
+bool launch_same(const PassLaunch& x, const PassLaunch& y) {
+  PassLaunch a, b;
+  memcpy(&a, &x, sizeof a);
+  memcpy(&b, &y, sizeof b);
+  for (PassLaunch* r : {&a, &b}) {
+    r->l.stream = nullptr;
+    r->col.src = nullptr;
+    r->col.dst = nullptr;
+    r->col.dc_out = nullptr;
+    r->col.ddi = r->col.ddj = r->col.ddk = 0.0;
+    r->row.src = nullptr;
+    r->row.dst = nullptr;
+    r->row.aux = nullptr;
+    r->row.shiftp = nullptr;
+  }
+  return memcmp(&a, &b, sizeof a) == 0;
+}
+
 hipError_t replay_launch(const PassLaunch& r) {
   switch (r.kind) {
     case 0: return do_col(r.E, r.f32, r.sel, r.split, r.groups, r.blue, r.stat, r.l, r.col);

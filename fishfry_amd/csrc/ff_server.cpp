@@ -59,7 +59,8 @@ struct Group {
   std::vector<Req*> reqs;
   int stream = 0;
   hipEvent_t ev = nullptr;
-  double work = 0;  // estimated cost of the group's solves (see work_of)
+  double work = 0;          // estimated cost of the group's solves (see work_of)
+  std::vector<int> tables;  // per-instance table slots of its grouped solves
 };
 
 // Estimated cost of a request's solve, in point-passes: five passes per
@@ -96,6 +97,12 @@ struct ff_server {
   // are copied into; owned by the scheduler thread
   double* pinned = nullptr;
   std::vector<int> free_slots;
+  // per-instance tables of grouped solves: pinned staging and device copies,
+  // kTableEntries entries per slot, a slot held until its group completes;
+  // owned by the scheduler thread
+  ff::PassInst* table_host = nullptr;
+  ff::PassInst* table_dev = nullptr;
+  std::vector<int> free_tables;
   std::atomic<bool> stop{false};
   std::thread worker;
   uint64_t next_id = 1;
@@ -104,7 +111,7 @@ struct ff_server {
   std::vector<double> latencies;
   size_t arena_peak = 0;
   double first_submit = 0, last_done = 0, points = 0;
-  long long completed = 0, rejected = 0, dispatched_groups = 0;
+  long long completed = 0, rejected = 0, dispatched_groups = 0, grouped_solves = 0;
   std::string fatal;
 };
 
@@ -113,6 +120,11 @@ namespace {
 // Pinned host slots for in-flight norm results, and the device work area of
 // one request's two norm reductions.
 constexpr int kNormSlots = 4096;
+// Grouped solves: at most kGroupMax requests per grouped launch sequence, and
+// table slots of kTableEntries per-instance entries (launches x requests:
+// 85 launches for FP32 + 8 refinement steps, 16 requests).
+constexpr int kGroupMax = 16;
+constexpr int kTableEntries = 96 * kGroupMax;
 // Cached plans (host structures only).
 constexpr size_t kMaxPlans = 64;
 size_t norm_bytes() { return 2 * sizeof(double) * kDiffDoubles; }
@@ -194,12 +206,81 @@ void finish(ff_server* s, Req* q, ff_status st, bool count) {
   q->done = true;
 }
 
-// Admit and dispatch one size class group on stream k.
+// Requests solved by one plan configuration: same extents, precision and
+// refinement steps (their domains may differ: the filter constants are
+// per-instance fields).
+bool same_solve(const ff_request& a, const ff_request& b) {
+  return a.n[0] == b.n[0] && a.n[1] == b.n[1] && a.n[2] == b.n[2] && a.precision == b.precision &&
+         a.refine_steps == b.refine_steps;
+}
+
+struct Item {
+  Req* q;
+  ff_plan p;  // the cached plan bound to the request's work areas
+  double* d;
+  double* phi;
+  ff_status st;
+};
+
+// Solves the requests mem of items as grouped launches on hs: each one's
+// solve is recorded (nothing runs), and when all recordings agree except in
+// their per-instance fields, every pass step runs once for all of them with a
+// per-instance table (as the batch's grouped mode does).  False (nothing
+// enqueued) when they do not agree, a launch is not recordable or no table
+// slot is free; the caller then solves them one by one.
+bool grouped_solve(ff_server* s, std::vector<Item>& items, const std::vector<size_t>& mem, hipStream_t hs, Group& g) {
+  if (mem.size() < 2 || s->free_tables.empty()) return false;
+  std::vector<std::vector<PassLaunch>> rec(mem.size());
+  for (size_t j = 0; j < mem.size(); ++j) {
+    Item& it = items[mem[j]];
+    set_launch_recorder(&rec[j]);
+    const ff_status st = solve_enqueue(&it.p, it.d, it.phi, hs, nullptr);
+    set_launch_recorder(nullptr);
+    if (st != FF_OK || rec[j].empty() || rec[j].size() != rec[0].size()) {
+      clear_error();
+      return false;
+    }
+  }
+  const size_t nl = rec[0].size(), m = mem.size();
+  if (nl * m > (size_t)kTableEntries) return false;
+  for (size_t j = 1; j < m; ++j)
+    for (size_t k = 0; k < nl; ++k)
+      if (!launch_same(rec[0][k], rec[j][k])) return false;
+  const int slot = s->free_tables.back();
+  PassInst* host = s->table_host + (size_t)slot * kTableEntries;
+  PassInst* dev = s->table_dev + (size_t)slot * kTableEntries;
+  for (size_t k = 0; k < nl; ++k)
+    for (size_t j = 0; j < m; ++j) host[k * m + j] = launch_inst_fields(rec[j][k]);
+  ff_status st = hip_check(hipMemcpyAsync(dev, host, nl * m * sizeof(PassInst), hipMemcpyHostToDevice, hs),
+                           "grouped solve table upload");
+  if (st != FF_OK) return false;  // nothing launched yet: solve one by one
+  s->free_tables.pop_back();
+  g.tables.push_back(slot);  // the upload and the launches read it until the group completes
+  for (size_t k = 0; k < nl && st == FF_OK; ++k) {
+    PassLaunch r = rec[0][k];
+    r.l.stream = hs;
+    r.l.grid.y = (unsigned)m;
+    if (r.kind == 0) r.col.inst = dev + k * m;
+    else r.row.inst = dev + k * m;
+    st = hip_check(replay_launch(r), "grouped solve launch");
+  }
+  for (size_t i : mem) items[i].st = st;
+  if (st == FF_OK) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->grouped_solves += (long long)m;
+  }
+  return true;
+}
+
+// Admit and dispatch one size class group on stream k: inputs, then the
+// solves (requests of one shape, precision and step count together as grouped
+// launches), then outputs and norms.
 void dispatch(ff_server* s, std::vector<Req*>& group, int k) {
   hipStream_t hs = s->streams[k];
   unsigned char* base = static_cast<unsigned char*>(s->arena_mem);
   Group g;
   g.stream = k;
+  std::vector<Item> items;
   for (Req* q : group) {
     ff_status st = FF_OK;
     ff_plan* cached = plan_for(s, *q, st);
@@ -207,23 +288,43 @@ void dispatch(ff_server* s, std::vector<Req*>& group, int k) {
       finish(s, q, st, false);
       continue;
     }
-    ff_plan p = *cached;  // shares twiddles; the work area is the request's own
+    Item it{q, *cached, nullptr, nullptr, FF_OK};  // shares twiddles; the work areas are the request's own
     if (q->has_s) {
-      if (p.opt.precision == FF_FP32_REFINE) plan_bind_scratch(&p, base + q->off_s);
-      if (p.fz.n) p.work = base + q->off_s + refine_bytes(q->r);  // four-step z rows
+      if (it.p.opt.precision == FF_FP32_REFINE) plan_bind_scratch(&it.p, base + q->off_s);
+      if (it.p.fz.n) it.p.work = base + q->off_s + refine_bytes(q->r);  // four-step z rows
     }
-    double* d = reinterpret_cast<double*>(base + q->off_d);
-    double* phi = reinterpret_cast<double*>(base + q->off_p);
+    it.d = reinterpret_cast<double*>(base + q->off_d);
+    it.phi = reinterpret_cast<double*>(base + q->off_p);
     if (q->has_field) {
-      st = field_init(&q->field, d, hs);
+      st = field_init(&q->field, it.d, hs);
     } else {
-      st = hip_check(hipMemcpyAsync(d, q->r.f_host, sizeof(double) * q->r.n[0] * q->r.n[1] * q->r.n[2],
+      st = hip_check(hipMemcpyAsync(it.d, q->r.f_host, sizeof(double) * q->r.n[0] * q->r.n[1] * q->r.n[2],
                                     hipMemcpyHostToDevice, hs),
                      "request input copy");
     }
-    if (st == FF_OK) st = solve_enqueue(&p, d, phi, hs, nullptr);
+    if (st != FF_OK) {
+      // work already enqueued for this request may still use its blocks
+      (void)hipStreamSynchronize(hs);
+      finish(s, q, st, false);
+      continue;
+    }
+    items.push_back(it);
+  }
+  std::vector<char> solved(items.size(), 0);
+  for (size_t a = 0; a < items.size(); ++a) {
+    if (solved[a]) continue;
+    std::vector<size_t> mem{a};
+    for (size_t b = a + 1; b < items.size() && mem.size() < (size_t)kGroupMax; ++b)
+      if (!solved[b] && same_solve(items[a].q->r, items[b].q->r)) mem.push_back(b);
+    if (!grouped_solve(s, items, mem, hs, g))
+      for (size_t i : mem) items[i].st = solve_enqueue(&items[i].p, items[i].d, items[i].phi, hs, nullptr);
+    for (size_t i : mem) solved[i] = 1;
+  }
+  for (Item& it : items) {
+    Req* q = it.q;
+    ff_status st = it.st;
     if (st == FF_OK && q->r.phi_host)
-      st = hip_check(hipMemcpyAsync(q->r.phi_host, phi, sizeof(double) * q->r.n[0] * q->r.n[1] * q->r.n[2],
+      st = hip_check(hipMemcpyAsync(q->r.phi_host, it.phi, sizeof(double) * q->r.n[0] * q->r.n[1] * q->r.n[2],
                                     hipMemcpyDeviceToHost, hs),
                      "request output copy");
     if (st == FF_OK && q->has_n) {
@@ -231,7 +332,7 @@ void dispatch(ff_server* s, std::vector<Req*>& group, int k) {
       // stream into its arena block and copied to a pinned host slot, so
       // completion needs no synchronization
       double* w = reinterpret_cast<double*>(base + q->off_n);
-      st = field_diff_enqueue(&q->field, phi, w, hs);
+      st = field_diff_enqueue(&q->field, it.phi, w, hs);
       if (st == FF_OK) st = field_diff_enqueue(&q->field, nullptr, w + kDiffDoubles, hs);
       if (st == FF_OK && !s->free_slots.empty()) {
         q->slot = s->free_slots.back();
@@ -253,13 +354,18 @@ void dispatch(ff_server* s, std::vector<Req*>& group, int k) {
     q->t_dispatch = now_s();
     g.reqs.push_back(q);
   }
-  if (g.reqs.empty()) return;
+  if (g.reqs.empty()) {
+    if (!g.tables.empty()) (void)hipStreamSynchronize(hs);  // their launches may still read them
+    for (int t : g.tables) s->free_tables.push_back(t);
+    return;
+  }
   if (!s->free_events.empty()) {
     g.ev = s->free_events.back();
     s->free_events.pop_back();
   } else if (hipEventCreateWithFlags(&g.ev, hipEventDisableTiming) != hipSuccess) {
     (void)hipStreamSynchronize(hs);
     for (Req* q : g.reqs) finish(s, q, FF_ERR_HIP, false);
+    for (int t : g.tables) s->free_tables.push_back(t);
     return;
   }
   (void)hipEventRecord(g.ev, hs);
@@ -296,6 +402,7 @@ void complete(ff_server* s, Group& g) {
     }
     finish(s, q, st, true);
   }
+  for (int t : g.tables) s->free_tables.push_back(t);
   s->stream_load[g.stream]--;
   s->stream_work[g.stream] = std::max(0.0, s->stream_work[g.stream] - g.work);
   s->free_events.push_back(g.ev);
@@ -438,6 +545,20 @@ ff_status ff_server_create(const ff_server_config* cfg, ff_server** out) {
     return st;
   }
   for (int k = kNormSlots - 1; k >= 0; --k) s->free_slots.push_back(k);
+  // table slots: two in-flight groups per stream, and two more
+  const int ntables = 2 * c.streams + 2;
+  const size_t tbytes = (size_t)ntables * kTableEntries * sizeof(PassInst);
+  st = hip_check(hipHostMalloc((void**)&s->table_host, tbytes, hipHostMallocDefault), "serving table staging");
+  if (st == FF_OK) st = hip_check(hipMalloc((void**)&s->table_dev, tbytes), "serving table hipMalloc");
+  if (st != FF_OK) {
+    for (hipStream_t hs : s->streams) (void)hipStreamDestroy(hs);
+    (void)hipFree(s->arena_mem);
+    (void)hipHostFree(s->pinned);
+    if (s->table_host) (void)hipHostFree(s->table_host);
+    delete s;
+    return st;
+  }
+  for (int t = ntables - 1; t >= 0; --t) s->free_tables.push_back(t);
   s->worker = std::thread(worker_main, s);
   *out = s;
   return FF_OK;
@@ -546,6 +667,7 @@ ff_status ff_server_stats(ff_server* s, ff_serving_stats* out) {
   out->completed = s->completed;
   out->rejected = s->rejected;
   out->groups = s->dispatched_groups;
+  out->grouped_solves = s->grouped_solves;
   out->p50_s = pct(0.5);
   out->p99_s = pct(0.99);
   out->points_per_s = (s->last_done > s->first_submit) ? s->points / (s->last_done - s->first_submit) : 0.0;
@@ -565,6 +687,8 @@ ff_status ff_server_destroy(ff_server* s) {
   for (hipStream_t hs : s->streams) (void)hipStreamDestroy(hs);
   if (s->arena_mem) (void)hipFree(s->arena_mem);
   if (s->pinned) (void)hipHostFree(s->pinned);
+  if (s->table_host) (void)hipHostFree(s->table_host);
+  if (s->table_dev) (void)hipFree(s->table_dev);
   for (auto& kv : s->all) delete kv.second;
   delete s;
   return FF_OK;

@@ -317,6 +317,9 @@ typedef struct ff_serving_stats {
   double p50_s, p99_s;      /* latency percentiles of completed requests  */
   double points_per_s;      /* completed points / (last done - first submit) */
   size_t arena_bytes, arena_peak_bytes;
+  long long grouped_solves; /* requests solved as grouped launches: requests of
+                               one shape, precision and step count in one
+                               dispatch group run each pass step as one launch */
 } ff_serving_stats;
 
 /* Starts the scheduler thread on the current device (cfg NULL: 8 GiB arena,

@@ -152,7 +152,17 @@ void plan_destroy(ff_plan* p) { delete p; }
 bool z_rows_four_step(const int[3], int) { return false; }
 void plan_bind_scratch(ff_plan* p, void* base) { p->scratch = base; }
 
+// Launch recording: the stand-in solve records nothing, so the server's
+// grouped solves fall back to solving requests one by one.
+thread_local std::vector<PassLaunch>* t_rec = nullptr;
+void set_launch_recorder(std::vector<PassLaunch>* rec) { t_rec = rec; }
+bool launch_recording() { return t_rec != nullptr; }
+hipError_t replay_launch(const PassLaunch&) { return hipErrorInvalidValue; }
+PassInst launch_inst_fields(const PassLaunch&) { return PassInst{}; }
+bool launch_same(const PassLaunch&, const PassLaunch&) { return false; }
+
 ff_status solve_enqueue(const ff_plan* p, double* density, double* potential, hipStream_t s, hipEvent_t*) {
+  if (t_rec) return FF_OK;  // recording: nothing runs, nothing is recorded
   for (long long i = 0; i < p->N; ++i) potential[i] = 0.5 * density[i] + 1.0;
   fs(s)->enqueue(kOpCost + kElemCost * double(p->N) * 5);
   return FF_OK;

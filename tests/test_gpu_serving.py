@@ -69,6 +69,41 @@ def test_alone_equals_in_mix_bitwise():
     srv.close()
 
 
+def test_grouped_solves_bitwise():
+    """Requests of one shape, precision and step count that meet in one
+    dispatch group run each pass step as one launch for all of them (their
+    buffers and domain constants from a per-instance table): the results are
+    the same requests' results solved one at a time, bit for bit, FP64 and
+    FP32 + 2 steps, with two domains in one group."""
+    rng = np.random.default_rng(15)
+    n = (40, 36, 32)
+    reqs = []
+    for k in range(10):
+        prec, steps = (ff.FF_FP64, 0) if k < 5 else (ff.FF_FP32_REFINE, 2)
+        fld = random_field(n, rng)
+        if k % 2:
+            for a in range(3):
+                fld.d[a] *= 1.5  # another domain: other filter constants
+        reqs.append((prec, steps, fld))
+    outs = []
+    for max_batch, streams in ((1, 1), (8, 1)):
+        srv = ff.Server(hbm_budget=2 << 30, streams=streams, max_batch=max_batch, batch_window_us=5000.0)
+        phis = [np.empty(n) for _ in reqs]
+        ids = [srv.submit(n, prec, steps, field=fld, phi_host=phi) for (prec, steps, fld), phi in zip(reqs, phis)]
+        for rid in ids:
+            assert srv.wait(rid)["status"] == 0
+        st = srv.stats()
+        say("grouped", max_batch, st)
+        if max_batch == 1:
+            assert st["grouped_solves"] == 0
+        else:
+            assert st["grouped_solves"] >= 4
+        outs.append(phis)
+        srv.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
 def test_host_requests_match_oracle():
     rng = np.random.default_rng(12)
     srv = ff.Server(hbm_budget=2 << 30)
