@@ -285,6 +285,37 @@ def bench_batching(ff, torch, K):
     return out
 
 
+def bench_serving_grouped(ff, nreq=128):
+    """Same-shape bursts through the server: nreq requests of one small grid
+    submitted at once, with dispatch groups of one (every request its own
+    launches) and of eight (each pass step one grouped launch for the
+    group).  Reports requests/s and p50 for each."""
+    out = []
+    for n, prec, steps in (((48, 48, 48), ff.FF_FP64, 0), ((64, 64, 64), ff.FF_FP32_REFINE, 2)):
+        row = {"n": list(n), "precision": "fp64" if prec == ff.FF_FP64 else "fp32+%d" % steps}
+        for mb in (1, 8):
+            srv = ff.Server(hbm_budget=8 << 30, streams=4, max_batch=mb, batch_window_us=200.0)
+            fld = ff.field_default(n)
+            srv.wait(srv.submit(n, prec, steps, field=fld))  # plan cache
+            best = None
+            for _ in range(3):
+                t0 = time.perf_counter()
+                ids = [srv.submit(n, prec, steps, field=fld) for _ in range(nreq)]
+                for rid in ids:
+                    srv.wait(rid)
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            st = srv.stats()
+            srv.close()
+            row["groups_of_%d" % mb] = {"requests_per_s": round(nreq / best, 1), "p50_ms": round(1e3 * st["p50_s"], 4),
+                                        "grouped_solves": st["grouped_solves"]}
+        log(f"serving bursts {row['precision']} {n}: {nreq} requests, groups of 1 "
+            f"{row['groups_of_1']['requests_per_s']:.0f}/s (p50 {row['groups_of_1']['p50_ms']:.2f} ms), groups of 8 "
+            f"{row['groups_of_8']['requests_per_s']:.0f}/s (p50 {row['groups_of_8']['p50_ms']:.2f} ms)")
+        out.append(row)
+    return out
+
+
 def bench_serving(ff, nreq=96, seed=20261202):
     """The serving configuration: Poisson arrivals, edges log-uniform in
     [32, 512] (any integer; half cubic), half FP64 and half FP32 + 2
@@ -509,6 +540,7 @@ def main():
             "sizes_fp64": bench_sizes(ff, torch, [(512, 512, 512), (768, 768, 768), (1000, 1000, 1000)], 5),
             "batching": bench_batching(ff, torch, 5),
             "serving": bench_serving(ff),
+            "serving_same_shape": bench_serving_grouped(ff),
         }
 
     cpu = None
