@@ -139,8 +139,24 @@ bool launch_recording();
 // and epi as for launch_c2r), the four-step twiddle between the sub-line
 // passes (forward or conjugate, in place on io.dst; a.tw = W_M, a.m1) and
 // the rounding of FP64 real rows to FP32 (src -> dst).  a.nline = M = nz/2.
-// Never recorded: refused while a recorder is installed.
-enum { ZSPLIT_FWD = 0, ZSPLIT_INV = 1, ZSPLIT_EPI = 2, ZSPLIT_TWF = 3, ZSPLIT_TWI = 4, ZSPLIT_PACK = 5 };
+// Odd nz (row pairs, a.nrows pairs, a.nline = nz): the packing of real rows
+// (2p, 2p+1) into one complex row (src -> dst; out_kind: FP32 input), the
+// separation of the two spectra (in place on io.dst, pair stride 2*nh), the
+// merge of two half spectra into the complex row of a pair (src -> dst) and
+// the unpacking epilogue.  Never recorded: refused while a recorder is
+// installed.
+enum {
+  ZSPLIT_FWD = 0,
+  ZSPLIT_INV = 1,
+  ZSPLIT_EPI = 2,
+  ZSPLIT_TWF = 3,
+  ZSPLIT_TWI = 4,
+  ZSPLIT_PACK = 5,
+  ZSPLIT_PPACK = 6,
+  ZSPLIT_PFWD = 7,
+  ZSPLIT_PINV = 8,
+  ZSPLIT_PEPI = 9
+};
 hipError_t launch_zsplit(int step, int f32, int out_kind, int epi, const RowArgs& a, hipStream_t s);
 
 // ---- driver kernels ---------------------------------------------------------

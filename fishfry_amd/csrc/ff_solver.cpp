@@ -176,14 +176,18 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   }
   // even nz whose packed rows (nz/2 complex values) no workgroup transforms
   // whole, in any of the plan's precisions: four-step rows in all of them
+  // (odd nz: the rows are transformed whole as complex rows, two real rows
+  // packed per complex row, which needs an even row count nx*ny)
   bool zfour = false;
   for (int prec = 0; prec < nprec; ++prec) {
     RowCfg r0, r1;
     const int eb = prec ? 8 : 16;
-    if (n[2] % 2 == 0 && (!make_row_cfg(n[2], eb, r0, 0, prec) || !make_row_cfg(n[2], eb, r1, 3, prec))) zfour = true;
+    if (!make_row_cfg(n[2], eb, r0, 0, prec) || !make_row_cfg(n[2], eb, r1, 3, prec)) zfour = true;
   }
+  p->zodd = zfour && n[2] % 2 ? 1 : 0;
+  if (p->zodd && ((long long)n[0] * n[1]) % 2) zfour = false;  // no plan: reported below
   for (int prec = 0; prec < nprec && zfour; ++prec)
-    if (!four_step_cfg(n[2] / 2, 0, prec ? 8 : 16, 1, prec, p->fz, true)) p->fz.c1[prec].threads = 0;
+    if (!four_step_cfg(p->zodd ? n[2] : n[2] / 2, 0, prec ? 8 : 16, 1, prec, p->fz, true)) p->fz.c1[prec].threads = 0;
   const bool four = p->fy.n || p->fx.n || p->fz.n;
   if (four) {
     if (o.flags & (FF_PENCIL_LAYOUT | FF_BLOCKED_LAYOUT | FF_SLAB_PIPELINE)) {
@@ -208,10 +212,10 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     if (!z_ok || !(p->fy.n || make_col_cfg(n[1], p->nh, eb, p->cy[prec], 1, prec, p->pencil && prec == mainp)) ||
         !x_cfg(p->cx[prec]) || (p->fy.n && !p->fy.c1[prec].threads) || (p->fx.n && !p->fx.c1[prec].threads)) {
       delete p;
-      char buf[160];
+      char buf[256];
       snprintf(buf, sizeof buf,
                "no transform plan for %dx%dx%d: an extent (nz/2 for even nz) is too long for one "
-               "workgroup's line transform",
+               "workgroup's line transform and has no four-step split (odd nz: nx*ny must be even)",
                n[0], n[1], n[2]);
       return set_error(FF_ERR_UNSUPPORTED, buf);
     }
@@ -335,7 +339,7 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
 }
 
 bool z_rows_four_step(const int n[3], int precision) {
-  if (n[2] % 2) return false;
+  if (n[2] % 2 && ((long long)n[0] * n[1]) % 2) return false;  // odd nz needs row pairs
   for (int prec = 0; prec < (precision == FF_FP64 ? 1 : 2); ++prec) {
     RowCfg r0, r1;
     const int eb = prec ? 8 : 16;
@@ -483,6 +487,10 @@ static ff_status run_zsub(const ff_plan* p, int prec, int mode, bool stepA, cons
   const ff_plan::FourStep& fs = p->fz;
   const long long m1 = fs.n1, m2 = fs.n2, M = fs.n;
   const ColCfg& c = stepA ? fs.c2[prec] : fs.c1[prec];
+  // (odd nz: rows are the row pairs, and the natural order is written with a
+  // pair stride of nz + 1 = 2*nh, the pair's two spectrum rows)
+  const int rows = p->zodd ? p->n[0] * p->n[1] / 2 : p->n[0] * p->n[1];
+  const long long nat = p->zodd ? 2LL * p->nh : p->nh;
   auto side = [](long long KS, long long fo, long long fr) {
     SpecSide sd;
     memset(&sd, 0, sizeof sd);
@@ -503,12 +511,12 @@ static ff_status run_zsub(const ff_plan* p, int prec, int mode, bool stepA, cons
     a.ncols = (int)m1;
   } else {
     const SpecSide packed = side(m1, M, 1);          // column k2, position t1
-    const SpecSide natural = side(1, p->nh, m2);     // column k2, position k1: k2 + m2*k1
+    const SpecSide natural = side(1, nat, m2);       // column k2, position k1: k2 + m2*k1
     a.in = mode == COL_FWD ? packed : natural;
     a.out = mode == COL_FWD ? natural : packed;
     a.ncols = (int)m2;
   }
-  a.nouter = p->n[0] * p->n[1];
+  a.nouter = rows;
   a.T = c.T;
   a.lgT = c.lgT;
   a.tiles = (a.ncols + c.T - 1) / c.T;
@@ -534,7 +542,7 @@ static RowArgs zsplit_args(const ff_plan* p, int prec, const void* src, void* ds
   memset(&a, 0, sizeof a);
   a.src = src;
   a.dst = dst;
-  a.nrows = (long long)p->n[0] * p->n[1];
+  a.nrows = (long long)p->n[0] * p->n[1] / (p->zodd ? 2 : 1);  // (odd nz: row pairs)
   a.nz = p->n[2];
   a.nh = p->nh;
   a.ny = p->n[1];
@@ -564,18 +572,23 @@ static ff_status check_zfour(const ff_plan* p, long long nrows, std::initializer
 // Forward: A from src (rows of M; FP64 input of an FP32 pass is first
 // rounded into the work array) into the work array, the twiddle, B into dst,
 // the split in dst.  src is read only.
+// Odd nz: the rows (2p, 2p+1) are packed as one complex row first, and the
+// spectra separated in place at the end.
 static ff_status zfwd_four(const ff_plan* p, int prec, int in_f32, const void* src, void* dst, hipStream_t s) {
   void* w = p->work;
   ff_status st;
   const void* a_in = src;
-  if (prec && !in_f32) {
+  if (p->zodd) {
+    if ((st = zsplit(ZSPLIT_PPACK, prec, in_f32, 0, zsplit_args(p, prec, src, w), s))) return st;
+    a_in = w;
+  } else if (prec && !in_f32) {
     if ((st = zsplit(ZSPLIT_PACK, 1, 0, 0, zsplit_args(p, 1, src, w), s))) return st;
     a_in = w;
   }
   if ((st = run_zsub(p, prec, COL_FWD, true, a_in, w, s))) return st;
   if ((st = zsplit(ZSPLIT_TWF, prec, 0, 0, zsplit_args(p, prec, w, w), s))) return st;
   if ((st = run_zsub(p, prec, COL_FWD, false, w, dst, s))) return st;
-  return zsplit(ZSPLIT_FWD, prec, 0, 0, zsplit_args(p, prec, dst, dst), s);
+  return zsplit(p->zodd ? ZSPLIT_PFWD : ZSPLIT_FWD, prec, 0, 0, zsplit_args(p, prec, dst, dst), s);
 }
 
 // Inverse.  The stored FP64 pass keeps src: the merge into dst, B^-1 into the
@@ -583,20 +596,22 @@ static ff_status zfwd_four(const ff_plan* p, int prec, int in_f32, const void* s
 // the work array itself).  The other epilogues, whose dst holds phi or a
 // residual too small for the merged rows, use src as their rows of M: the
 // merge into the work array, B^-1 back into src, the twiddle and A^-1 there,
-// the epilogue from src.
+// the epilogue from src.  Odd nz: the merge forms the complex rows of the row
+// pairs (natural order, pair stride 2*nh) and the epilogue unpacks them.
 static ff_status zinv_four(const ff_plan* p, int prec, int out_kind, int epi, const void* src, void* dst,
                            const double* f, const double* shiftp, hipStream_t s) {
   void* w = p->work;
   ff_status st;
   RowArgs e;
   void* rows;
+  const int merge = p->zodd ? ZSPLIT_PINV : ZSPLIT_INV;
   if (!prec && !out_kind && epi == EPI_STORE && !f && !shiftp) {
-    if ((st = zsplit(ZSPLIT_INV, 0, 0, 0, zsplit_args(p, 0, src, dst), s))) return st;
+    if ((st = zsplit(merge, 0, 0, 0, zsplit_args(p, 0, src, dst), s))) return st;
     if ((st = run_zsub(p, 0, COL_INV, false, dst, w, s))) return st;
     rows = w;
   } else {
     void* srcw = const_cast<void*>(src);
-    if ((st = zsplit(ZSPLIT_INV, prec, 0, 0, zsplit_args(p, prec, src, w), s))) return st;
+    if ((st = zsplit(merge, prec, 0, 0, zsplit_args(p, prec, src, w), s))) return st;
     if ((st = run_zsub(p, prec, COL_INV, false, w, srcw, s))) return st;
     rows = srcw;
   }
@@ -608,7 +623,7 @@ static ff_status zinv_four(const ff_plan* p, int prec, int out_kind, int epi, co
     e.shiftp = shiftp;
     e.shift = e.scale;  // shift = (sum of phi) / N
   }
-  return zsplit(ZSPLIT_EPI, prec, out_kind, epi, e, s);
+  return zsplit(p->zodd ? ZSPLIT_PEPI : ZSPLIT_EPI, prec, out_kind, epi, e, s);
 }
 
 ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void* dst, hipStream_t s,

@@ -61,6 +61,9 @@ struct ff_plan {
     const void* bl1[2][3];      // Bluestein tables of the sub-lines
     const void* bl2[2][3];
   } fy, fx, fz;
+  // four-step z rows of odd nz: the complex row of two real rows (2p, 2p+1),
+  // nz long, is transformed (fz.n = nz) and the two spectra separated
+  int zodd;
 };
 
 namespace ff {

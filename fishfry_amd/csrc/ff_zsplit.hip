@@ -136,6 +136,113 @@ __global__ __launch_bounds__(kZThreads) void k_zepi(const RowArgs a) {
   }
 }
 
+// ---- odd nz: two real rows per complex row ------------------------------------
+// Pair p holds rows (2p, 2p+1): z[t] = x[2p][t] + i x[2p+1][t], t < n = nz
+// (packed rows of n, pair stride n).  Its transform Z has the two spectra
+// X1[k] = (Z[k] + conj Z[n-k])/2 and X2[k] = (Z[k] - conj Z[n-k])/(2i).
+
+// Packing: src real rows (IN) -> dst packed pairs (F).
+template <typename F, typename IN>
+__global__ __launch_bounds__(kZThreads) void k_zpair_pack(const RowArgs a) {
+  const PassInst io = inst_of(a);
+  const long long n = a.nline;
+  const long long total = a.nrows * n;
+  const FF_GLB IN* x = glb<const IN>(io.src);
+  FF_GLB cx<F>* z = glb<cx<F>>(io.dst);
+  for (long long it = (long long)blockIdx.x * kZThreads + threadIdx.x; it < total;
+       it += (long long)gridDim.x * kZThreads) {
+    const long long p = it / n, t = it - p * n;
+    z[it] = cx<F>{F(x[2 * p * n + t]), F(x[(2 * p + 1) * n + t])};
+  }
+}
+
+// Separation in place: pair p's transform, in natural order at p*(n+1)
+// (B's output), becomes the spectrum rows 2p (at p*(n+1)) and 2p+1 (at
+// p*(n+1) + nh).  Item (p, k), k <= k' = nh-1-k, reads Z[k], Z[n-k], Z[k'] and
+// Z[n-k'] = Z[nh+k] and writes X1[k], X2[k], X1[k'], X2[k'] at k, nh+k, k'
+// and nh+k' = n-k: the same four places (k = 0: Z[0] twice, X2[nh-1] into the
+// pair's spare slot n), so the items never overlap.
+template <typename F>
+__global__ __launch_bounds__(kZThreads) void k_zpair_split(const RowArgs a) {
+  const PassInst io = inst_of(a);
+  const int n = a.nline, nh = a.nh;
+  const long long K = (nh + 1) / 2;
+  const long long total = a.nrows * K;
+  FF_GLB cx<F>* z = glb<cx<F>>(io.dst);
+  auto x1 = [](cx<F> u, cx<F> v) { return cx<F>{F(0.5) * (u.x + v.x), F(0.5) * (u.y - v.y)}; };
+  auto x2 = [](cx<F> u, cx<F> v) { return cx<F>{F(0.5) * (u.y + v.y), F(-0.5) * (u.x - v.x)}; };
+  for (long long it = (long long)blockIdx.x * kZThreads + threadIdx.x; it < total;
+       it += (long long)gridDim.x * kZThreads) {
+    const long long p = it / K;
+    const int k = (int)(it - p * K);
+    const int kp = nh - 1 - k;
+    FF_GLB cx<F>* Z = z + p * (long long)(n + 1);
+    const cx<F> zk = Z[k];
+    const cx<F> zmk = Z[k ? n - k : 0];
+    const cx<F> zkp = Z[kp];
+    const cx<F> zmkp = Z[n - kp];
+    Z[k] = x1(zk, zmk);
+    Z[nh + k] = x2(zk, zmk);
+    if (kp != k) {
+      Z[kp] = x1(zkp, zmkp);
+      Z[nh + kp] = x2(zkp, zmkp);
+    }
+  }
+}
+
+// Merge, src -> dst: the spectrum rows 2p and 2p+1 (rows of nh; Hermitian,
+// Im X[0] ignored) into pair p's complex spectrum Z[k] = X1[k] + i X2[k]
+// (X[n-k] = conj X[k] above nh), natural order at p*(n+1).
+template <typename F>
+__global__ __launch_bounds__(kZThreads) void k_zpair_merge(const RowArgs a) {
+  const PassInst io = inst_of(a);
+  const int n = a.nline, nh = a.nh;
+  const long long total = a.nrows * n;
+  const FF_GLB cx<F>* x = glb<const cx<F>>(io.src);
+  FF_GLB cx<F>* z = glb<cx<F>>(io.dst);
+  for (long long it = (long long)blockIdx.x * kZThreads + threadIdx.x; it < total;
+       it += (long long)gridDim.x * kZThreads) {
+    const long long p = it / n;
+    const int k = (int)(it - p * n);
+    const FF_GLB cx<F>* r1 = x + 2 * p * nh;
+    const FF_GLB cx<F>* r2 = r1 + nh;
+    cx<F> u, v;
+    if (k < nh) {
+      u = r1[k];
+      v = r2[k];
+      if (k == 0) {
+        u.y = F(0);
+        v.y = F(0);
+      }
+    } else {
+      u = conj(r1[n - k]);
+      v = conj(r2[n - k]);
+    }
+    z[p * (long long)(n + 1) + k] = cx<F>{u.x - v.y, u.y + v.x};
+  }
+}
+
+// Unpacking epilogue: pair p's packed row (src, pair stride n) to the real
+// rows 2p (real parts) and 2p+1 (imaginary parts), stored, as the residual or
+// accumulated (c2r_out1).
+template <typename F, typename OUT, int EPI>
+__global__ __launch_bounds__(kZThreads) void k_zpair_epi(const RowArgs a) {
+  const PassInst io = inst_of(a);
+  const long long n = a.nline;
+  const long long total = a.nrows * n;
+  const FF_GLB cx<F>* z = glb<const cx<F>>(io.src);
+  FF_GLB OUT* out = glb<OUT>(io.dst);
+  const FF_GLB double* f = glb<const double>(io.aux);
+  const double shift = io.shiftp ? a.shift * *glb<const double>(io.shiftp) : a.shift;
+  for (long long it = (long long)blockIdx.x * kZThreads + threadIdx.x; it < total;
+       it += (long long)gridDim.x * kZThreads) {
+    const long long p = it / n, t = it - p * n;
+    const cx<F> v = z[it];
+    c2r_out1<EPI, OUT>(out, f, 2 * p * n + t, to_d(v.x), a.scale, shift);
+    c2r_out1<EPI, OUT>(out, f, (2 * p + 1) * n + t, to_d(v.y), a.scale, shift);
+  }
+}
+
 template <typename F>
 const void* zsplit_fn(int step, int out_kind, int epi) {
   switch (step) {
@@ -144,6 +251,21 @@ const void* zsplit_fn(int step, int out_kind, int epi) {
     case ZSPLIT_TWF: return (const void*)&k_ztwiddle<F, -1>;
     case ZSPLIT_TWI: return (const void*)&k_ztwiddle<F, +1>;
     case ZSPLIT_PACK: return std::is_same<F, float>::value ? (const void*)&k_zpack : nullptr;
+    case ZSPLIT_PPACK:  // out_kind: FP32 input rows
+      return out_kind ? (std::is_same<F, float>::value ? (const void*)&k_zpair_pack<float, float> : nullptr)
+                      : (const void*)&k_zpair_pack<F, double>;
+    case ZSPLIT_PFWD: return (const void*)&k_zpair_split<F>;
+    case ZSPLIT_PINV: return (const void*)&k_zpair_merge<F>;
+    case ZSPLIT_PEPI:
+      if constexpr (std::is_same<F, double>::value) {
+        if (epi == EPI_STORE && out_kind == 0) return (const void*)&k_zpair_epi<double, double, EPI_STORE>;
+        if (epi == EPI_RESIDUAL && out_kind == 1) return (const void*)&k_zpair_epi<double, float, EPI_RESIDUAL>;
+        if (epi == EPI_RESIDUAL && out_kind == 0) return (const void*)&k_zpair_epi<double, double, EPI_RESIDUAL>;
+      } else {
+        if (epi == EPI_STORE && out_kind == 0) return (const void*)&k_zpair_epi<float, double, EPI_STORE>;
+        if (epi == EPI_ACCUM && out_kind == 0) return (const void*)&k_zpair_epi<float, double, EPI_ACCUM>;
+      }
+      return nullptr;
     case ZSPLIT_EPI:
       // the epilogues the passes use: FP64 stored or residual (FP64 or FP32
       // output), FP32 stored or accumulated into FP64 phi
@@ -166,9 +288,11 @@ hipError_t launch_zsplit(int step, int f32, int out_kind, int epi, const RowArgs
   if (launch_recording()) return hipErrorInvalidValue;  // never recorded
   const void* fn = f32 ? zsplit_fn<float>(step, out_kind, epi) : zsplit_fn<double>(step, out_kind, epi);
   if (!fn) return hipErrorInvalidValue;
-  // items: the split's pairs, else the rows' complex values (real values for the pack)
-  const long long total = step == ZSPLIT_FWD ? a.nrows * (a.nline / 2 + 1)
-                                             : (step == ZSPLIT_PACK ? 2 : 1) * a.nrows * a.nline;
+  // items: the split's pairs of k, else the rows' complex values (real values
+  // for the pack)
+  long long total = (step == ZSPLIT_PACK ? 2 : 1) * a.nrows * a.nline;
+  if (step == ZSPLIT_FWD) total = a.nrows * (a.nline / 2 + 1);
+  if (step == ZSPLIT_PFWD) total = a.nrows * ((a.nh + 1) / 2);
   void* args[] = {const_cast<RowArgs*>(&a)};
   return hipLaunchKernel(fn, dim3(zsplit_blocks(total)), dim3(kZThreads), args, 0, s);
 }

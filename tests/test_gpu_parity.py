@@ -568,8 +568,11 @@ def test_four_step_lines_in_a_batch():
 
 # even nz whose packed rows (nz/2 complex values) no workgroup transforms
 # whole: 5-smooth (10000, 16384) and with Bluestein sub-lines (8500 = 85 *
-# 100, 8195 = 5 * 11 * 149, 10001 = 73 * 137)
-LONG_Z_SHAPES = [(3, 2, 20000), (2, 2, 32768), (4, 3, 17000), (2, 5, 16390), (5, 2, 20002)]
+# 100, 8195 = 5 * 11 * 149, 10001 = 73 * 137); odd nz above a workgroup's
+# line (two real rows per complex row, nx*ny even): 16875 = 3^3 5^4, 10125 =
+# 3^4 5^3, 9801 = 99^2 (Bluestein sub-lines), 16385 = 5 * 29 * 113
+LONG_Z_SHAPES = [(3, 2, 20000), (2, 2, 32768), (4, 3, 17000), (2, 5, 16390), (5, 2, 20002),
+                 (2, 2, 16875), (6, 5, 10125), (4, 2, 9801), (2, 3, 16385)]
 
 
 @pytest.mark.parametrize("n", LONG_Z_SHAPES)
@@ -588,7 +591,7 @@ def test_four_step_z_rows_match_oracle(n):
     plan, nb, a, b = make(n)
     fs = json.loads(plan.describe())["four_step"]
     say("four-step z", n, fs)
-    assert fs["z"][0] * fs["z"][1] == n[2] // 2 and fs["z"][0] > 1
+    assert fs["z"][0] * fs["z"][1] == (n[2] if n[2] % 2 else n[2] // 2) and fs["z"][0] > 1
     put(a, f)
     plan.solve(nb, a, b)
     torch.cuda.synchronize()
@@ -652,6 +655,11 @@ def test_four_step_z_limits():
             ff.ParisPencil(n, lo, hi, **kw)
         assert e.value.status == ff.FF_ERR_UNSUPPORTED, kw
         assert "four steps" in str(e.value)
+    # odd nz above a workgroup's line needs row pairs: nx*ny odd has no plan
+    lo3, hi3, _ = ff.benchmark_grid((3, 3, 16875))
+    with pytest.raises(ff.FishFryError) as e:
+        ff.ParisPencil((3, 3, 16875), lo3, hi3)
+    assert e.value.status == ff.FF_ERR_UNSUPPORTED and "nx*ny must be even" in str(e.value)
     items = [{"n": n, "lo": lo, "hi": hi}, {"n": (8, 8, 8)}]
     for flags in (ff.FF_BATCH_GROUPED, ff.FF_BATCH_STEPS):
         with pytest.raises(ff.FishFryError) as e:
