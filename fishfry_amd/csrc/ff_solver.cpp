@@ -469,7 +469,7 @@ static RowArgs row_args(const ff_plan* p, int prec, bool inverse) {
   return a;
 }
 
-// z rows in four steps (ParisPencil.cpp:188-189 and 402-403 transform z
+// z rows in four steps (ParisPencil.cpp:80-81, 165 and 380 transform z
 // rows of any length): the packed row z[t] = x[2t] + i x[2t+1], t = t1 +
 // m1*t2 < M = nz/2 = m1*m2, in rows of M complex values, is transformed as
 //   A: for each t1, the m2-point DFT over t2 (stride m1), in place at t1 +

@@ -3,7 +3,7 @@
 // transform): the real-to-complex split after the forward M-point transform,
 // the complex-to-real merge before the inverse one, and the inverse's
 // epilogue.  The whole-row kernels (k_r2c, k_c2r) do the same arithmetic on
-// rows held in LDS (ParisPencil.cpp:188-189 and 402-403: D2Z and Z2D along z).
+// rows held in LDS (ParisPencil.cpp:165 and 380: D2Z and Z2D along z).
 #include <algorithm>
 #include <type_traits>
 

@@ -578,7 +578,7 @@ LONG_Z_SHAPES = [(3, 2, 20000), (2, 2, 32768), (4, 3, 17000), (2, 5, 16390), (5,
 @pytest.mark.parametrize("n", LONG_Z_SHAPES)
 def test_four_step_z_rows_match_oracle(n):
     """The reference transforms z rows of any length (D2Z / Z2D plans,
-    ParisPencil.cpp:74-77).  Packed rows too long for one workgroup run as
+    ParisPencil.cpp:80-81, 165, 380).  Packed rows too long for one workgroup run as
     four-step sub-line passes through the plan's work array, with the
     real-to-complex split and the complex-to-real merge as row-local steps:
     the solve, the z forward pass (pass 0) and the z inverse pass (pass 4)
