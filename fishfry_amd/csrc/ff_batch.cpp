@@ -103,6 +103,10 @@ namespace {
 
 constexpr int kBatchStreams = 4;
 
+// Instances the grouped and step modes solve on their own: grids whose z
+// rows run in four steps (their row-local steps are not recorded launches).
+bool solo(const ff_batch* b, int i) { return b->plans[i]->fz.n != 0; }
+
 void drop_graph(ff_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -136,6 +140,7 @@ ff_status prepare_groups(ff_batch* b, double* const* d, double* const* p) {
   b->groups.clear();
   std::vector<std::vector<PassLaunch>> rec(b->count);
   for (int i = 0; i < b->count; ++i) {
+    if (solo(b, i)) continue;
     set_launch_recorder(&rec[i]);
     const ff_status st = solve_enqueue(b->plans[i], d[i], p[i], nullptr, nullptr);
     set_launch_recorder(nullptr);
@@ -143,6 +148,7 @@ ff_status prepare_groups(ff_batch* b, double* const* d, double* const* p) {
   }
   std::vector<BatchGroup> groups;
   for (int i = 0; i < b->count; ++i) {
+    if (solo(b, i)) continue;
     const ff_batch_item& it = b->items[i];
     BatchGroup* home = nullptr;
     for (BatchGroup& g : groups) {
@@ -296,6 +302,7 @@ ff_status prepare_steps(ff_batch* b, double* const* d, double* const* p) {
   b->fams.clear();
   std::vector<std::vector<PassLaunch>> rec(b->count);
   for (int i = 0; i < b->count; ++i) {
+    if (solo(b, i)) continue;
     set_launch_recorder(&rec[i]);
     const ff_status st = solve_enqueue(b->plans[i], d[i], p[i], nullptr, nullptr);
     set_launch_recorder(nullptr);
@@ -303,6 +310,7 @@ ff_status prepare_steps(ff_batch* b, double* const* d, double* const* p) {
   }
   std::vector<StepFamily> fams;
   for (int i = 0; i < b->count; ++i) {
+    if (solo(b, i)) continue;
     StepFamily* home = nullptr;
     for (StepFamily& f : fams) {
       const int l = f.inst[0];
@@ -521,8 +529,9 @@ ff_status enqueue_all(ff_batch* b, double* const* d, double* const* p, hipStream
       }
     }
   }
-  for (int i = 0; i < b->count && st == FF_OK && !grouped && !steps; ++i)
-    st = solve_enqueue(b->plans[i], d[i], p[i], b->streams[i % b->streams.size()], nullptr);
+  for (int i = 0; i < b->count && st == FF_OK; ++i)
+    if ((!grouped && !steps) || solo(b, i))
+      st = solve_enqueue(b->plans[i], d[i], p[i], b->streams[i % b->streams.size()], nullptr);
   for (size_t k = 0; k < b->streams.size() && st == FF_OK; ++k) {
     st = hip_check(hipEventRecord(b->joins[k], b->streams[k]), "batch join");
     if (st == FF_OK) st = hip_check(hipStreamWaitEvent(s, b->joins[k], 0), "batch join wait");
@@ -556,15 +565,6 @@ ff_status ff_batch_create(int count, const ff_batch_item* items, int flags, ff_b
   b->flags = flags;
   b->items.assign(items, items + count);
   ff_status st = make_plans(b);
-  // the grouped and step modes replay recorded pass launches; four-step z
-  // rows also run row-local steps that are not recorded
-  if (st == FF_OK && (flags & (FF_BATCH_GROUPED | FF_BATCH_STEPS)))
-    for (const ff_plan* p : b->plans)
-      if (p->fz.n) {
-        st = set_error(FF_ERR_UNSUPPORTED,
-                       "ff_batch_create: grids whose z rows run in four steps take the stream-parallel mode");
-        break;
-      }
   const int ns = std::min(kBatchStreams, count);
   for (int k = 0; k < ns && st == FF_OK; ++k) {
     hipStream_t s = nullptr;

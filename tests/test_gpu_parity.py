@@ -644,9 +644,10 @@ def test_four_step_z_with_four_step_columns(n):
 
 def test_four_step_z_limits():
     """Four-step z rows run in the plain layout: the pencil, blocked and slab
-    layouts and the batch's recorded modes are refused with a message; the
-    stream-parallel batch runs them and matches the single-plan solve bit for
-    bit."""
+    layouts are refused with a message, and so is an odd nz above a
+    workgroup's line with nx*ny odd.  Every batch mode runs them (the grouped
+    and step modes solve such a grid on its own) and matches the single-plan
+    solve bit for bit, also on a second solve and under graph capture."""
     n = (3, 2, 20000)
     lo, hi, _ = ff.benchmark_grid(n)
     for kw in ({"flags": ff.FF_PENCIL_LAYOUT}, {"flags": ff.FF_BLOCKED_LAYOUT}, {"flags": ff.FF_SLAB_PIPELINE},
@@ -660,27 +661,32 @@ def test_four_step_z_limits():
     with pytest.raises(ff.FishFryError) as e:
         ff.ParisPencil((3, 3, 16875), lo3, hi3)
     assert e.value.status == ff.FF_ERR_UNSUPPORTED and "nx*ny must be even" in str(e.value)
-    items = [{"n": n, "lo": lo, "hi": hi}, {"n": (8, 8, 8)}]
-    for flags in (ff.FF_BATCH_GROUPED, ff.FF_BATCH_STEPS):
-        with pytest.raises(ff.FishFryError) as e:
-            ff.Batch(items, flags)
-        assert e.value.status == ff.FF_ERR_UNSUPPORTED
+    items = [{"n": n, "lo": lo, "hi": hi}, {"n": (8, 8, 8)}, {"n": (8, 8, 8)}]
     rng = np.random.default_rng(12)
-    fs = [rng.standard_normal(n), rng.standard_normal((8, 8, 8))]
+    fs = [rng.standard_normal(n), rng.standard_normal((8, 8, 8)), rng.standard_normal((8, 8, 8))]
     outs = [gpu_solve(f) for f in fs]
-    bt = ff.Batch(items, 0)
-    dens, pots = [], []
-    for k, f in enumerate(fs):
-        nb = bt.bytes(k)
-        d = torch.zeros(nb // 8, dtype=torch.float64, device="cuda")
-        put(d, f)
-        dens.append(d)
-        pots.append(torch.zeros(nb // 8, dtype=torch.float64, device="cuda"))
-    bt.solve(dens, pots)
-    torch.cuda.synchronize()
-    for k, f in enumerate(fs):
-        assert np.array_equal(get_real(pots[k], f.shape), outs[k])
-    bt.close()
+    # every batch mode runs them: the grouped and step modes solve such a
+    # grid on its own, beside their grouped or multi-shape launches
+    for flags in (0, ff.FF_BATCH_GROUPED, ff.FF_BATCH_STEPS, ff.FF_BATCH_STEPS | ff.FF_BATCH_GRAPH):
+        bt = ff.Batch(items, flags)
+        dens, pots = [], []
+        for k, f in enumerate(fs):
+            nb = bt.bytes(k)
+            d = torch.zeros(nb // 8, dtype=torch.float64, device="cuda")
+            put(d, f)
+            dens.append(d)
+            pots.append(torch.zeros(nb // 8, dtype=torch.float64, device="cuda"))
+        s = torch.cuda.Stream()
+        for _ in range(2):
+            torch.cuda.synchronize()
+            bt.solve(dens, pots, s)
+            bt.wait(s)
+            torch.cuda.synchronize()
+            for k, f in enumerate(fs):
+                assert np.array_equal(get_real(pots[k], f.shape), outs[k]), (flags, k)
+            for k, f in enumerate(fs):
+                put(dens[k], f)  # the solve may use density as work space
+        bt.close()
 
 
 @pytest.mark.parametrize("n", [(512, 512, 512), (768, 768, 768), (1000, 1000, 1000), (1024, 1024, 1024)])
