@@ -1,11 +1,14 @@
 // Pass kernels of the periodic Poisson solve.
 //
-// Data layout in HBM (all passes): the half spectrum is complex [i][j][kh],
-// kh fastest, row pitch nh = nz/2+1 -- the layout the reference has after its
-// D2Z (ParisPencil.cpp:165) -- so no pass ever transposes.  The z passes work
-// on contiguous rows; the y and x passes work on strided columns, a workgroup
-// owning a tile of T neighbouring kh columns so every row of the tile is one
-// contiguous T*sizeof(complex) segment in HBM.
+// Data layout in HBM: the half spectrum leaves the z forward pass as complex
+// [i][j][kh], kh fastest, row pitch nh = nz/2+1 -- the layout the reference
+// has after its D2Z (ParisPencil.cpp:165).  In the plain layout every pass
+// keeps it; in the pencil layout each column pass reads strided columns and
+// writes its lines contiguously in a transposed order ([kh][i][j], [kh][j][i],
+// [i][kh][j]; SpecSide describes each).  The z passes work on rows; the y and
+// x passes on strided columns, a workgroup owning a tile of T neighbouring
+// columns so every row of the tile is one contiguous T*sizeof(complex)
+// segment in HBM.
 //
 //   k_r2c   z forward, real rows -> half-spectrum rows       (ParisPencil.cpp:165)
 //   k_col   y forward / inverse, or x forward*filter*inverse (:212, :259-288, :334)
