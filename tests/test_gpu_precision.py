@@ -65,8 +65,8 @@ def test_precision_workload(n):
     assert errs[0][0] < 1e-6
     assert errs[1][0] < 1e-10
     # Two steps reach the floor that the FP64 residual's own rounding sets
-    # (measured 1.2-1.3x the FP64 solve's Linf at 1000^3 and 1024^3); the
-    # north star's 1.1x is not reached with the spectral symbol.
+    # (measured 1.2-1.3x the FP64 solve's Linf at 1000^3 and 1024^3); a 1.1x
+    # target is not reached with the spectral symbol.
     assert errs[2][0] < 1e-14 and errs[2][1] < 1.4 * l64[2]
     assert errs[3][0] < 1e-14 and errs[3][1] < 1.4 * l64[2]
     del a, ref, phi

@@ -1,7 +1,7 @@
-"""The drop-in binding of INTEGRATION.md section 1, compiled against the
-reference's own driver: the reference's main.cpp and FishFry.hpp (which
-instantiate FishFry<ParisPencil>, main.cpp:48) are compiled for gfx950 with
-their ParisPencil.hpp replaced by the binding stub printed in INTEGRATION.md,
+"""The drop-in binding printed in section 1 of the integration guide, compiled
+against the reference's own driver: the reference's main.cpp and FishFry.hpp
+(which instantiate FishFry<ParisPencil>, main.cpp:48) are compiled for gfx950
+with their ParisPencil.hpp replaced by that binding stub,
 through a clang file overlay (the reference tree is read, never modified or
 copied).  Runs only where a reference checkout and an MPI header exist (the
 build container); syntax and template instantiation only, no link, no GPU."""
