@@ -106,6 +106,8 @@ class ServingStats(ctypes.Structure):
                 ("grouped_solves", ctypes.c_longlong)]
 
 
+FF_SERVE_LARGE_ALONE = 1
+
 FF_BATCH_GRAPH = 1
 FF_BATCH_PERSISTENT = 4
 FF_BATCH_GROUPED = 8
@@ -423,8 +425,8 @@ class Server:
     groups, continuous dispatch over several streams."""
 
     def __init__(self, hbm_budget: int = 8 << 30, streams: int = 4, max_batch: int = 16,
-                 batch_window_us: float = 200.0):
-        cfg = ServerConfig(hbm_budget, streams, max_batch, batch_window_us, 0)
+                 batch_window_us: float = 200.0, flags: int = 0):
+        cfg = ServerConfig(hbm_budget, streams, max_batch, batch_window_us, flags)
         h = ctypes.c_void_p()
         check(lib().ff_server_create(ctypes.byref(cfg), ctypes.byref(h)), "Server")
         self._h = h

@@ -61,6 +61,7 @@ struct Group {
   hipEvent_t ev = nullptr;
   double work = 0;          // estimated cost of the group's solves (see work_of)
   std::vector<int> tables;  // per-instance table slots of its grouped solves
+  bool large = false;       // FF_SERVE_LARGE_ALONE: runs with the GPU to itself
 };
 
 // Estimated cost of a request's solve, in point-passes: five passes per
@@ -68,6 +69,18 @@ struct Group {
 double work_of(const ff_request& r) {
   const double N = double(r.n[0]) * r.n[1] * r.n[2];
   return r.precision == FF_FP32_REFINE ? N * (2.5 + 7.5 * r.refine_steps) : N * 5.0;
+}
+
+// FF_SERVE_LARGE_ALONE: the work above which a group runs with the GPU to
+// itself, and below which a group may still run beside it (point-passes;
+// 6e7 is FP64 up to about 230^3, FP32 + 2 steps up to about 150^3).
+constexpr double kLargeWork = 3e8;
+constexpr double kSmallWork = 6e7;
+
+double group_work(const std::vector<Req*>& reqs, int max_batch) {
+  double w = 0;
+  for (size_t i = 0; i < reqs.size() && (int)i < max_batch; ++i) w += work_of(reqs[i]->r);
+  return w;
 }
 
 }  // namespace
@@ -80,6 +93,7 @@ struct ff_server {
   std::vector<hipStream_t> streams;
   std::vector<int> stream_load;
   std::vector<double> stream_work;  // estimated work queued on each stream
+  int large_inflight = 0;           // FF_SERVE_LARGE_ALONE groups on the device
   std::vector<hipEvent_t> free_events;
   // plans by (extents, precision, refinement steps, lo, hi), with the tick of
   // their last use; at most kMaxPlans are kept (least recently used evicted)
@@ -376,6 +390,8 @@ void dispatch(ff_server* s, std::vector<Req*>& group, int k) {
   (void)hipEventRecord(g.ev, hs);
   s->stream_load[k]++;
   for (Req* q : g.reqs) g.work += work_of(q->r);
+  g.large = (s->cfg.flags & FF_SERVE_LARGE_ALONE) && g.work > kLargeWork;
+  if (g.large) s->large_inflight++;
   s->stream_work[k] += g.work;
   s->inflight.push_back(g);
   std::lock_guard<std::mutex> lk(s->mu);
@@ -408,6 +424,7 @@ void complete(ff_server* s, Group& g) {
     finish(s, q, st, true);
   }
   for (int t : g.tables) s->free_tables.push_back(t);
+  if (g.large) s->large_inflight--;
   s->stream_load[g.stream]--;
   s->stream_work[g.stream] = std::max(0.0, s->stream_work[g.stream] - g.work);
   s->free_events.push_back(g.ev);
@@ -448,11 +465,18 @@ void worker_main(ff_server* s) {
     std::vector<std::pair<double, int>> order;
     for (auto& c : classes) order.push_back({c.second.front()->t_submit, c.first});
     std::sort(order.begin(), order.end());
+    const bool alone = (s->cfg.flags & FF_SERVE_LARGE_ALONE) != 0;
     for (auto& oc : order) {
       std::vector<Req*>& reqs = classes[oc.second];
+      const double w = alone ? group_work(reqs, s->cfg.max_batch) : 0.0;
+      // beside a large group only small ones run
+      if (alone && s->large_inflight && w >= kSmallWork) break;
       const bool full = (int)reqs.size() >= s->cfg.max_batch;
       const bool due = t - reqs.front()->t_submit >= window || s->inflight.empty();
       if (!full && !due) continue;
+      // a large group waits for the device to drain, and holds back newer
+      // classes meanwhile (so that it is not starved)
+      if (alone && w > kLargeWork && !s->inflight.empty()) break;
       // the stream with the least work queued among those with fewer than two
       // groups, so a small request does not wait behind a large one
       int k = -1;
@@ -518,9 +542,10 @@ ff_status ff_server_create(const ff_server_config* cfg, ff_server** out) {
   ff_server_config c{size_t(8) << 30, 4, 16, 200.0, 0};
   if (cfg) c = *cfg;
   if (c.streams < 1 || c.streams > 16 || c.max_batch < 1 || c.hbm_budget < (size_t(1) << 20) ||
-      c.batch_window_us < 0) {
+      c.batch_window_us < 0 || (c.flags & ~FF_SERVE_LARGE_ALONE)) {
     delete s;
-    return set_error(FF_ERR_INVALID, "ff_server_create: streams 1..16, max_batch >= 1, budget >= 1 MiB");
+    return set_error(FF_ERR_INVALID,
+                     "ff_server_create: streams 1..16, max_batch >= 1, budget >= 1 MiB, flags FF_SERVE_LARGE_ALONE or 0");
   }
   s->cfg = c;
   ff_status st = hip_check(hipGetDevice(&s->device), "hipGetDevice");

@@ -289,8 +289,17 @@ typedef struct ff_server_config {
   int streams;            /* device streams groups are spread over (1..16)            */
   int max_batch;          /* requests dispatched together per size class              */
   double batch_window_us; /* longest wait for same-class companions while busy        */
-  int flags;              /* reserved, 0                                              */
+  int flags;              /* FF_SERVE_LARGE_ALONE or 0                                */
 } ff_server_config;
+
+/* ff_server_config.flags: a group whose estimated work is large (about 3e8
+ * point-passes: FP32 + 2 steps from about 260^3, FP64 from about 390^3)
+ * runs with the GPU nearly to itself -- it is dispatched once nothing is in
+ * flight, and while it runs only small groups (below 6e7 point-passes: FP64
+ * up to about 230^3, FP32 + 2 steps up to about 150^3) are dispatched --
+ * instead of sharing the GPU with the other streams' groups, which stretches
+ * its latency. */
+#define FF_SERVE_LARGE_ALONE 1
 
 struct ff_field;
 

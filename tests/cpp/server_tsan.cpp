@@ -146,6 +146,46 @@ int main() {
     }
     check(ff_server_destroy(s2) == FF_OK, "destroy 2");
   }
+  // FF_SERVE_LARGE_ALONE: a flag the server does not know is refused; large
+  // requests (FP32 + 8 steps at 170^3, about 3e8 point-passes each) complete
+  // among small ones, each run with the device to itself
+  {
+    ff_server_config bad = cfg;
+    bad.flags = 2;
+    ff_server* s3 = nullptr;
+    check(ff_server_create(&bad, &s3) == FF_ERR_INVALID, "unknown flag refused");
+    ff_server_config alone{size_t(1) << 30, 4, 2, 100.0, FF_SERVE_LARGE_ALONE};
+    check(ff_server_create(&alone, &s3) == FF_OK, "create 3");
+    if (s3) {
+      struct Job {
+        std::vector<double> f, phi;
+        uint64_t id = 0;
+      };
+      std::vector<Job> jobs(12);
+      for (int k = 0; k < 12; ++k) {
+        const bool large = k % 4 == 1;
+        const int e = large ? 170 : 16;
+        Job& j = jobs[k];
+        j.f.assign(size_t(e) * e * e, 0.25 * k);
+        j.phi.assign(j.f.size(), 0.0);
+        ff_request r{};
+        r.n[0] = r.n[1] = r.n[2] = e;
+        r.precision = large ? FF_FP32_REFINE : FF_FP64;
+        r.refine_steps = large ? 8 : 0;
+        r.f_host = j.f.data();
+        r.phi_host = j.phi.data();
+        check(ff_server_submit(s3, &r, &j.id) == FF_OK, "submit 3");
+      }
+      for (Job& j : jobs) {
+        ff_result res{};
+        check(ff_server_wait(s3, j.id, 60.0, &res) == FF_OK && res.status == FF_OK, "wait 3");
+        bool ok = true;
+        for (size_t i = 0; i < j.f.size(); ++i) ok = ok && j.phi[i] == 0.5 * j.f[i] + 1.0;
+        check(ok, "large-alone result");
+      }
+      check(ff_server_destroy(s3) == FF_OK, "destroy 3");
+    }
+  }
   if (g_fail) return 1;
   std::printf("server ok\n");
   return 0;

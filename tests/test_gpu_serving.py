@@ -69,6 +69,37 @@ def test_alone_equals_in_mix_bitwise():
     srv.close()
 
 
+def test_large_alone_bitwise():
+    """FF_SERVE_LARGE_ALONE: large requests (FP32 + 2 steps at 264^3, FP64 at
+    400^3: above 3e8 point-passes) run with the device to themselves among small
+    ones, and every result equals the same request solved on a default server."""
+    rng = np.random.default_rng(23)
+    reqs = []
+    for k in range(12):
+        if k in (2, 7):
+            n, prec = ((264, 264, 264), ff.FF_FP32_REFINE) if k == 2 else ((400, 400, 400), ff.FF_FP64)
+        else:
+            n = tuple(int(v) for v in rng.integers(24, 80, size=3))
+            prec = ff.FF_FP64 if k % 2 else ff.FF_FP32_REFINE
+        reqs.append((n, prec, random_field(n, rng)))
+    outs = {}
+    for flags in (0, ff.FF_SERVE_LARGE_ALONE):
+        srv = ff.Server(hbm_budget=16 << 30, streams=4, max_batch=2, flags=flags)
+        ids = []
+        for n, prec, fld in reqs:
+            phi = np.empty(n)
+            ids.append((srv.submit(n, prec, 2, field=fld, phi_host=phi), phi))
+        for rid, _ in ids:
+            srv.wait(rid, timeout=120.0)
+        say("flags", flags, srv.stats())
+        outs[flags] = [phi for _, phi in ids]
+        srv.close()
+    for a, b in zip(outs[0], outs[ff.FF_SERVE_LARGE_ALONE]):
+        assert np.array_equal(a, b)
+    with pytest.raises(ff.FishFryError):
+        ff.Server(flags=2)
+
+
 def test_grouped_solves_bitwise():
     """Requests of one shape, precision and step count that meet in one
     dispatch group run each pass step as one launch for all of them (their
