@@ -75,7 +75,7 @@ ff_status ff_solve_noalloc(const ff_plan* plan, size_t bytes, double* density, d
   hipStream_t s = (hipStream_t)stream;
   size_t need = 0;
   if (plan->opt.precision == FF_FP32_REFINE) need = plan_scratch_bytes(plan);
-  else if (plan->work) need = (size_t)plan->Nh * 16;
+  else need = plan_work_bytes(plan);
   if (need > kStaticWorkBytes) {
     char buf[200];
     snprintf(buf, sizeof buf,

@@ -550,6 +550,8 @@ struct RowArgs {
   const void *chirp, *bsf, *bsi;  // Bluestein tables (see line_dft)
   const PassInst* inst;           // grouped launch: per-instance fields (null: one solve)
   int m1;                         // four-step z rows: the stride-1 sub-line length (ff_zsplit.hip)
+  int lone;                       // odd nz, nx*ny odd: the launch's one pair is the last row alone
+                                  // (no second row: zero on the way in, not written on the way out)
 };
 
 __device__ __forceinline__ PassInst inst_of(const RowArgs& a) {

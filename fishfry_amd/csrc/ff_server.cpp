@@ -158,8 +158,7 @@ size_t refine_bytes(const ff_request& r) {
   return r.precision == FF_FP32_REFINE ? align_up(Nh * 16 + align_up(N * 4, 256) + 256, 256) : 0;
 }
 size_t work_bytes(const ff_request& r) {
-  const size_t Nh = (size_t)r.n[0] * r.n[1] * (r.n[2] / 2 + 1);
-  return refine_bytes(r) + (z_rows_four_step(r.n, r.precision) ? Nh * 16 : 0);
+  return refine_bytes(r) + (z_rows_four_step(r.n, r.precision) ? z_work_bytes(r.n) : 0);
 }
 
 ff_plan* plan_for(ff_server* s, const Req& q, ff_status& st) {

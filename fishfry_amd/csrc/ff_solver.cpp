@@ -177,7 +177,7 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   // even nz whose packed rows (nz/2 complex values) no workgroup transforms
   // whole, in any of the plan's precisions: four-step rows in all of them
   // (odd nz: the rows are transformed whole as complex rows, two real rows
-  // packed per complex row, which needs an even row count nx*ny)
+  // packed per complex row; with nx*ny odd the last row runs alone)
   bool zfour = false;
   for (int prec = 0; prec < nprec; ++prec) {
     RowCfg r0, r1;
@@ -185,7 +185,7 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     if (!make_row_cfg(n[2], eb, r0, 0, prec) || !make_row_cfg(n[2], eb, r1, 3, prec)) zfour = true;
   }
   p->zodd = zfour && n[2] % 2 ? 1 : 0;
-  if (p->zodd && ((long long)n[0] * n[1]) % 2) zfour = false;  // no plan: reported below
+  p->zlone = p->zodd && ((long long)n[0] * n[1]) % 2 ? 1 : 0;
   for (int prec = 0; prec < nprec && zfour; ++prec)
     if (!four_step_cfg(p->zodd ? n[2] : n[2] / 2, 0, prec ? 8 : 16, 1, prec, p->fz, true)) p->fz.c1[prec].threads = 0;
   const bool four = p->fy.n || p->fx.n || p->fz.n;
@@ -215,7 +215,7 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
       char buf[256];
       snprintf(buf, sizeof buf,
                "no transform plan for %dx%dx%d: an extent (nz/2 for even nz) is too long for one "
-               "workgroup's line transform and has no four-step split (odd nz: nx*ny must be even)",
+               "workgroup's line transform and has no four-step split",
                n[0], n[1], n[2]);
       return set_error(FF_ERR_UNSUPPORTED, buf);
     }
@@ -326,9 +326,9 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
     plan_bind_scratch(p, p->scratch);
   }
   // (four-step z rows: own_scratch false leaves the work array to the caller,
-  // who points the plan, or a copy of it, at 16*Nh bytes of its own)
+  // who points the plan, or a copy of it, at z_work_bytes of its own)
   if ((o.precision == FF_FP64 && (o.flags & FF_PRESERVE_INPUT)) || (p->fz.n && own_scratch)) {
-    st = hip_check(hipMalloc(&p->work, (size_t)p->Nh * 16), "work array hipMalloc");
+    st = hip_check(hipMalloc(&p->work, p->fz.n ? z_work_bytes(p->n) : (size_t)p->Nh * 16), "work array hipMalloc");
     if (st != FF_OK) {
       plan_destroy(p);
       return st;
@@ -338,8 +338,21 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
   return FF_OK;
 }
 
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+size_t z_work_bytes(const int n[3]) {
+  const size_t nh = (size_t)n[2] / 2 + 1;
+  size_t b = align256((size_t)n[0] * n[1] * nh * 16);
+  if (n[2] % 2 && ((long long)n[0] * n[1]) % 2) b += align256(16 * (size_t)n[2]) + align256(32 * nh);
+  return b;
+}
+
+size_t plan_work_bytes(const ff_plan* p) {
+  if (!p->work) return 0;
+  return p->fz.n ? z_work_bytes(p->n) : (size_t)p->Nh * 16;
+}
+
 bool z_rows_four_step(const int n[3], int precision) {
-  if (n[2] % 2 && ((long long)n[0] * n[1]) % 2) return false;  // odd nz needs row pairs
   for (int prec = 0; prec < (precision == FF_FP64 ? 1 : 2); ++prec) {
     RowCfg r0, r1;
     const int eb = prec ? 8 : 16;
@@ -483,13 +496,15 @@ static RowArgs row_args(const ff_plan* p, int prec, bool inverse) {
 // plain layout; the plan's work array (16*Nh bytes) holds the rows of M or
 // the merged rows.  prec: 0 FP64, 1 FP32.
 static ff_status run_zsub(const ff_plan* p, int prec, int mode, bool stepA, const void* src, void* dst,
-                          hipStream_t s) {
+                          hipStream_t s, int nrows = -1) {
   const ff_plan::FourStep& fs = p->fz;
   const long long m1 = fs.n1, m2 = fs.n2, M = fs.n;
   const ColCfg& c = stepA ? fs.c2[prec] : fs.c1[prec];
   // (odd nz: rows are the row pairs, and the natural order is written with a
-  // pair stride of nz + 1 = 2*nh, the pair's two spectrum rows)
-  const int rows = p->zodd ? p->n[0] * p->n[1] / 2 : p->n[0] * p->n[1];
+  // pair stride of nz + 1 = 2*nh, the pair's two spectrum rows; nrows: the
+  // lone last row's one pair)
+  const int rows = nrows >= 0 ? nrows : p->zodd ? p->n[0] * p->n[1] / 2 : p->n[0] * p->n[1];
+  if (!rows) return FF_OK;
   const long long nat = p->zodd ? 2LL * p->nh : p->nh;
   auto side = [](long long KS, long long fo, long long fr) {
     SpecSide sd;
@@ -555,7 +570,21 @@ static RowArgs zsplit_args(const ff_plan* p, int prec, const void* src, void* ds
 }
 
 static ff_status zsplit(int step, int prec, int out_kind, int epi, const RowArgs& a, hipStream_t s) {
+  if (!a.nrows) return FF_OK;  // (odd nz, one row: only the lone row's launches)
   return hip_check(launch_zsplit(step, prec, out_kind, epi, a, s), "z four-step row-local launch");
+}
+
+// The lone last row of odd nz with nx*ny odd, in the work array's tail: its
+// packed row (nz complex values) and its pair region (2*nh), one pair.
+static unsigned char* zlone_packed(const ff_plan* p) {
+  return static_cast<unsigned char*>(p->work) + align256((size_t)p->Nh * 16);
+}
+static unsigned char* zlone_natural(const ff_plan* p) { return zlone_packed(p) + align256(16 * (size_t)p->n[2]); }
+static RowArgs zlone_args(const ff_plan* p, int prec, const void* src, void* dst) {
+  RowArgs a = zsplit_args(p, prec, src, dst);
+  a.nrows = 1;
+  a.lone = 1;
+  return a;
 }
 
 // The buffers of a four-step z pass must differ, except where noted.
@@ -588,7 +617,23 @@ static ff_status zfwd_four(const ff_plan* p, int prec, int in_f32, const void* s
   if ((st = run_zsub(p, prec, COL_FWD, true, a_in, w, s))) return st;
   if ((st = zsplit(ZSPLIT_TWF, prec, 0, 0, zsplit_args(p, prec, w, w), s))) return st;
   if ((st = run_zsub(p, prec, COL_FWD, false, w, dst, s))) return st;
-  return zsplit(p->zodd ? ZSPLIT_PFWD : ZSPLIT_FWD, prec, 0, 0, zsplit_args(p, prec, dst, dst), s);
+  if ((st = zsplit(p->zodd ? ZSPLIT_PFWD : ZSPLIT_FWD, prec, 0, 0, zsplit_args(p, prec, dst, dst), s))) return st;
+  if (!p->zlone) return FF_OK;
+  // the lone last row: the same steps on its own pair in the work array's
+  // tail, then its spectrum row (the pair's first) into dst
+  const long long R = (long long)p->n[0] * p->n[1];
+  const size_t csz = prec ? 8 : 16;
+  unsigned char* tp = zlone_packed(p);
+  unsigned char* tn = zlone_natural(p);
+  const void* row = static_cast<const unsigned char*>(src) + (R - 1) * p->n[2] * (in_f32 ? 4 : 8);
+  if ((st = zsplit(ZSPLIT_PPACK, prec, in_f32, 0, zlone_args(p, prec, row, tp), s))) return st;
+  if ((st = run_zsub(p, prec, COL_FWD, true, tp, tp, s, 1))) return st;
+  if ((st = zsplit(ZSPLIT_TWF, prec, 0, 0, zlone_args(p, prec, tp, tp), s))) return st;
+  if ((st = run_zsub(p, prec, COL_FWD, false, tp, tn, s, 1))) return st;
+  if ((st = zsplit(ZSPLIT_PFWD, prec, 0, 0, zlone_args(p, prec, tn, tn), s))) return st;
+  return hip_check(hipMemcpyAsync(static_cast<unsigned char*>(dst) + (R - 1) * p->nh * csz, tn, p->nh * csz,
+                                  hipMemcpyDeviceToDevice, s),
+                   "z four-step lone row copy");
 }
 
 // Inverse.  The stored FP64 pass keeps src: the merge into dst, B^-1 into the
@@ -605,6 +650,16 @@ static ff_status zinv_four(const ff_plan* p, int prec, int out_kind, int epi, co
   RowArgs e;
   void* rows;
   const int merge = p->zodd ? ZSPLIT_PINV : ZSPLIT_INV;
+  // the lone last row (odd nz, nx*ny odd) runs through the work array's
+  // tail: its merge first, while src is whole (src may be the work array)
+  const long long R = (long long)p->n[0] * p->n[1];
+  const size_t csz = prec ? 8 : 16;
+  unsigned char* tp = p->zlone ? zlone_packed(p) : nullptr;
+  unsigned char* tn = p->zlone ? zlone_natural(p) : nullptr;
+  if (p->zlone) {
+    const void* row = static_cast<const unsigned char*>(src) + (R - 1) * p->nh * csz;
+    if ((st = zsplit(ZSPLIT_PINV, prec, 0, 0, zlone_args(p, prec, row, tn), s))) return st;
+  }
   if (!prec && !out_kind && epi == EPI_STORE && !f && !shiftp) {
     if ((st = zsplit(merge, 0, 0, 0, zsplit_args(p, 0, src, dst), s))) return st;
     if ((st = run_zsub(p, 0, COL_INV, false, dst, w, s))) return st;
@@ -623,7 +678,18 @@ static ff_status zinv_four(const ff_plan* p, int prec, int out_kind, int epi, co
     e.shiftp = shiftp;
     e.shift = e.scale;  // shift = (sum of phi) / N
   }
-  return zsplit(p->zodd ? ZSPLIT_PEPI : ZSPLIT_EPI, prec, out_kind, epi, e, s);
+  if ((st = zsplit(p->zodd ? ZSPLIT_PEPI : ZSPLIT_EPI, prec, out_kind, epi, e, s))) return st;
+  if (!p->zlone) return FF_OK;
+  if ((st = run_zsub(p, prec, COL_INV, false, tn, tp, s, 1))) return st;
+  if ((st = zsplit(ZSPLIT_TWI, prec, 0, 0, zlone_args(p, prec, tp, tp), s))) return st;
+  if ((st = run_zsub(p, prec, COL_INV, true, tp, tp, s, 1))) return st;
+  // its epilogue writes (and reads f at) the last real row only
+  const long long off = (R - 1) * p->n[2];
+  RowArgs el = zlone_args(p, prec, tp, static_cast<unsigned char*>(dst) + off * (out_kind ? 4 : 8));
+  el.aux = f ? f + off : nullptr;
+  el.shiftp = e.shiftp;
+  el.shift = e.shift;
+  return zsplit(ZSPLIT_PEPI, prec, out_kind, epi, el, s);
 }
 
 ff_status run_zfwd(const ff_plan* p, int prec, int in_f32, const void* src, void* dst, hipStream_t s,

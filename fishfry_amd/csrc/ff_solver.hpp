@@ -64,6 +64,9 @@ struct ff_plan {
   // four-step z rows of odd nz: the complex row of two real rows (2p, 2p+1),
   // nz long, is transformed (fz.n = nz) and the two spectra separated
   int zodd;
+  // odd nz with nx*ny odd: the last row has no partner and runs alone, with
+  // a zero second row, through the work array's tail (z_work_bytes)
+  int zlone;
 };
 
 namespace ff {
@@ -101,9 +104,15 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
                       ff_plan** out, bool own_scratch = true);
 void plan_destroy(ff_plan* p);
 // Whether an FP64 grid of extents n runs its z rows in four steps, which
-// needs the work array (16*Nh bytes; the plan's own unless created with
+// needs the work array (z_work_bytes; the plan's own unless created with
 // own_scratch false).
 bool z_rows_four_step(const int n[3], int precision);
+// The work array of four-step z rows: the rows of M (16*Nh bytes) and, for
+// odd nz with nx*ny odd, a tail for the lone last row (its packed row, 16*nz
+// bytes, and its pair region, 32*nh bytes), each part 256-byte aligned.
+size_t z_work_bytes(const int n[3]);
+// The size of a plan's work array (0: none).
+size_t plan_work_bytes(const ff_plan* p);
 // FP32 refinement work area: its size, and pointing a plan (or a copy of one)
 // at a work area of that size.
 size_t plan_scratch_bytes(const ff_plan* p);

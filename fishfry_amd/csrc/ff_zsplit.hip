@@ -140,6 +140,8 @@ __global__ __launch_bounds__(kZThreads) void k_zepi(const RowArgs a) {
 // Pair p holds rows (2p, 2p+1): z[t] = x[2p][t] + i x[2p+1][t], t < n = nz
 // (packed rows of n, pair stride n).  Its transform Z has the two spectra
 // X1[k] = (Z[k] + conj Z[n-k])/2 and X2[k] = (Z[k] - conj Z[n-k])/(2i).
+// With nx*ny odd the last row is alone: a launch of its own (a.lone, one
+// pair) takes its second row as zero and never reads or writes that row.
 
 // Packing: src real rows (IN) -> dst packed pairs (F).
 template <typename F, typename IN>
@@ -152,7 +154,7 @@ __global__ __launch_bounds__(kZThreads) void k_zpair_pack(const RowArgs a) {
   for (long long it = (long long)blockIdx.x * kZThreads + threadIdx.x; it < total;
        it += (long long)gridDim.x * kZThreads) {
     const long long p = it / n, t = it - p * n;
-    z[it] = cx<F>{F(x[2 * p * n + t]), F(x[(2 * p + 1) * n + t])};
+    z[it] = cx<F>{F(x[2 * p * n + t]), a.lone ? F(0) : F(x[(2 * p + 1) * n + t])};
   }
 }
 
@@ -206,17 +208,17 @@ __global__ __launch_bounds__(kZThreads) void k_zpair_merge(const RowArgs a) {
     const int k = (int)(it - p * n);
     const FF_GLB cx<F>* r1 = x + 2 * p * nh;
     const FF_GLB cx<F>* r2 = r1 + nh;
-    cx<F> u, v;
+    cx<F> u, v{F(0), F(0)};
     if (k < nh) {
       u = r1[k];
-      v = r2[k];
+      if (!a.lone) v = r2[k];
       if (k == 0) {
         u.y = F(0);
         v.y = F(0);
       }
     } else {
       u = conj(r1[n - k]);
-      v = conj(r2[n - k]);
+      if (!a.lone) v = conj(r2[n - k]);
     }
     z[p * (long long)(n + 1) + k] = cx<F>{u.x - v.y, u.y + v.x};
   }
@@ -239,7 +241,7 @@ __global__ __launch_bounds__(kZThreads) void k_zpair_epi(const RowArgs a) {
     const long long p = it / n, t = it - p * n;
     const cx<F> v = z[it];
     c2r_out1<EPI, OUT>(out, f, 2 * p * n + t, to_d(v.x), a.scale, shift);
-    c2r_out1<EPI, OUT>(out, f, (2 * p + 1) * n + t, to_d(v.y), a.scale, shift);
+    if (!a.lone) c2r_out1<EPI, OUT>(out, f, (2 * p + 1) * n + t, to_d(v.y), a.scale, shift);
   }
 }
 

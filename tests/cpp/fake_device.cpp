@@ -150,6 +150,7 @@ ff_status plan_create(const int n[3], const double lo[3], const double hi[3], co
 }
 void plan_destroy(ff_plan* p) { delete p; }
 bool z_rows_four_step(const int[3], int) { return false; }
+size_t z_work_bytes(const int[3]) { return 0; }
 void plan_bind_scratch(ff_plan* p, void* base) { p->scratch = base; }
 
 // Launch recording: the stand-in solve records nothing, so the server's

@@ -569,10 +569,11 @@ def test_four_step_lines_in_a_batch():
 # even nz whose packed rows (nz/2 complex values) no workgroup transforms
 # whole: 5-smooth (10000, 16384) and with Bluestein sub-lines (8500 = 85 *
 # 100, 8195 = 5 * 11 * 149, 10001 = 73 * 137); odd nz above a workgroup's
-# line (two real rows per complex row, nx*ny even): 16875 = 3^3 5^4, 10125 =
-# 3^4 5^3, 9801 = 99^2 (Bluestein sub-lines), 16385 = 5 * 29 * 113
+# line (two real rows per complex row): 16875 = 3^3 5^4, 10125 = 3^4 5^3,
+# 9801 = 99^2 (Bluestein sub-lines), 16385 = 5 * 29 * 113, and with nx*ny odd
+# (the last row alone, through the work array's tail)
 LONG_Z_SHAPES = [(3, 2, 20000), (2, 2, 32768), (4, 3, 17000), (2, 5, 16390), (5, 2, 20002),
-                 (2, 2, 16875), (6, 5, 10125), (4, 2, 9801), (2, 3, 16385)]
+                 (2, 2, 16875), (6, 5, 10125), (4, 2, 9801), (2, 3, 16385), (3, 3, 16875), (3, 5, 9801)]
 
 
 @pytest.mark.parametrize("n", LONG_Z_SHAPES)
@@ -644,10 +645,10 @@ def test_four_step_z_with_four_step_columns(n):
 
 def test_four_step_z_limits():
     """Four-step z rows run in the plain layout: the pencil, blocked and slab
-    layouts are refused with a message, and so is an odd nz above a
-    workgroup's line with nx*ny odd.  Every batch mode runs them (the grouped
-    and step modes solve such a grid on its own) and matches the single-plan
-    solve bit for bit, also on a second solve and under graph capture."""
+    layouts are refused with a message.  Every batch mode runs them (the
+    grouped and step modes solve such a grid on its own), an odd nz with
+    nx*ny odd included, and matches the single-plan solve bit for bit, also
+    on a second solve and under graph capture."""
     n = (3, 2, 20000)
     lo, hi, _ = ff.benchmark_grid(n)
     for kw in ({"flags": ff.FF_PENCIL_LAYOUT}, {"flags": ff.FF_BLOCKED_LAYOUT}, {"flags": ff.FF_SLAB_PIPELINE},
@@ -656,14 +657,12 @@ def test_four_step_z_limits():
             ff.ParisPencil(n, lo, hi, **kw)
         assert e.value.status == ff.FF_ERR_UNSUPPORTED, kw
         assert "four steps" in str(e.value)
-    # odd nz above a workgroup's line needs row pairs: nx*ny odd has no plan
-    lo3, hi3, _ = ff.benchmark_grid((3, 3, 16875))
-    with pytest.raises(ff.FishFryError) as e:
-        ff.ParisPencil((3, 3, 16875), lo3, hi3)
-    assert e.value.status == ff.FF_ERR_UNSUPPORTED and "nx*ny must be even" in str(e.value)
-    items = [{"n": n, "lo": lo, "hi": hi}, {"n": (8, 8, 8)}, {"n": (8, 8, 8)}]
+    n3 = (3, 3, 16875)
+    lo3, hi3, _ = ff.benchmark_grid(n3)
+    items = [{"n": n, "lo": lo, "hi": hi}, {"n": n3, "lo": lo3, "hi": hi3}, {"n": (8, 8, 8)}, {"n": (8, 8, 8)}]
     rng = np.random.default_rng(12)
-    fs = [rng.standard_normal(n), rng.standard_normal((8, 8, 8)), rng.standard_normal((8, 8, 8))]
+    fs = [rng.standard_normal(n), rng.standard_normal(n3), rng.standard_normal((8, 8, 8)),
+          rng.standard_normal((8, 8, 8))]
     outs = [gpu_solve(f) for f in fs]
     # every batch mode runs them: the grouped and step modes solve such a
     # grid on its own, beside their grouped or multi-shape launches
@@ -687,6 +686,37 @@ def test_four_step_z_limits():
             for k, f in enumerate(fs):
                 put(dens[k], f)  # the solve may use density as work space
         bt.close()
+
+
+def test_four_step_z_lone_row_only():
+    """nx = ny = 1 with an odd nz above a workgroup's line: no row pairs at all,
+    only the lone row's launches run (the pair launches have no rows).  The
+    solve (on a domain of its own: the benchmark grid has hi = lo at n = 1),
+    the z passes and FP32 + 2 steps match the oracle."""
+    n = (1, 1, 16875)
+    lo, hi = [0.0, 0.0, -math.pi], [1.0, 1.0, math.pi - 2.0 * math.pi / n[2]]
+    rng = np.random.default_rng(31)
+    f = rng.standard_normal(n)
+    want = oracle.solve(f, lo, hi)
+    for kw, tol in (({}, TOL), ({"precision": ff.FF_FP32_REFINE, "refine_steps": 2}, 1e-12)):
+        plan = ff.ParisPencil(n, lo, hi, **kw)
+        nb = plan.bytes()
+        a = torch.zeros(nb // 8, dtype=torch.float64, device="cuda")
+        b = torch.zeros(nb // 8, dtype=torch.float64, device="cuda")
+        put(a, f)
+        plan.solve(nb, a, b)
+        torch.cuda.synchronize()
+        assert rel(get_real(b, n), want) < tol, (kw, rel(get_real(b, n), want))
+        if not kw:
+            plan.solve_pass(0, a, b)
+            torch.cuda.synchronize()
+            assert rel(get_half(b, n), oracle.rfft(f)) < TOL
+            spec = rng.standard_normal((1, 1, n[2] // 2 + 1)) + 1j * rng.standard_normal((1, 1, n[2] // 2 + 1))
+            put_half(a, spec)
+            plan.solve_pass(4, a, b)
+            torch.cuda.synchronize()
+            assert rel(get_real(b, n), oracle.irfft(spec, n[2]) / n[2]) < TOL
+        plan.close()
 
 
 @pytest.mark.parametrize("n", [(512, 512, 512), (768, 768, 768), (1000, 1000, 1000), (1024, 1024, 1024)])

@@ -155,7 +155,9 @@ def test_four_step_z_requests():
     """Grids whose z rows run in four steps get their z work array from the
     arena per request (the footprint counts it, after the FP32 refinement
     work area), so same-shape requests on different streams do not share
-    one: every result matches the oracle, FP64 and FP32 + 2 steps."""
+    one: every result matches the oracle, FP64 and FP32 + 2 steps, an odd nz
+    with nx*ny odd included (its work array has a tail for the lone last
+    row)."""
     rng = np.random.default_rng(14)
     n = (3, 2, 20000)
     srv = ff.Server(hbm_budget=1 << 30, streams=4, max_batch=1)
@@ -180,6 +182,23 @@ def test_four_step_z_requests():
     for f, p in zip(fs, phis32):
         want = oracle.solve(f)
         assert np.linalg.norm(p - want) / np.linalg.norm(want) < 1e-12
+    # odd nz with nx*ny odd: the work array has a tail for the lone last row
+    n3 = (3, 3, 16875)
+    nh3 = n3[2] // 2 + 1
+    Nh3 = n3[0] * n3[1] * nh3
+    assert srv.footprint(n3) == 2 * block(16 * Nh3) + block(16 * Nh3) + block(16 * n3[2]) + block(32 * nh3)
+    fs3 = [rng.standard_normal(n3) for _ in range(3)]
+    phis3 = [np.empty(n3) for _ in fs3]
+    ids = [srv.submit(n3, f_host=f, phi_host=p) for f, p in zip(fs3, phis3)]
+    p32 = np.empty(n3)
+    ids.append(srv.submit(n3, ff.FF_FP32_REFINE, 2, f_host=fs3[0], phi_host=p32))
+    for rid in ids:
+        assert srv.wait(rid)["status"] == 0
+    for f, p in zip(fs3, phis3):
+        want = oracle.solve(f)
+        assert np.linalg.norm(p - want) / np.linalg.norm(want) < 1e-13
+    want = oracle.solve(fs3[0])
+    assert np.linalg.norm(p32 - want) / np.linalg.norm(want) < 1e-12
     srv.close()
 
 

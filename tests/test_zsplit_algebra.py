@@ -7,7 +7,8 @@ four-step z rows (fishfry_amd/csrc/ff_zsplit.hip), restated here in numpy:
 * odd nz (row pairs): the in-place separation of a pair's transform into two
   spectrum rows -- each item (k <= k' = nh-1-k) reads and writes the same four
   places, the items' places are disjoint and cover both rows -- and the merge
-  of two half spectra into the pair's complex spectrum.
+  of two half spectra into the pair's complex spectrum; with nx*ny odd, the
+  lone last row as a pair with a zero second row.
 
 A plausible slip (a conjugate on the wrong term, a sign of the 1/(2i), an
 index k against n-k, an overlap between items) fails one of these."""
@@ -130,6 +131,22 @@ def test_odd_pair_separation_and_merge(n):
     assert np.allclose(X2, np.fft.rfft(b), rtol=0, atol=1e-12 * n)
     z = np.fft.ifft(merge_pair(X1, X2, n))
     assert np.allclose(z.real, a, rtol=0, atol=1e-12) and np.allclose(z.imag, b, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [1, 9, 243, 1001])
+def test_odd_lone_row(n):
+    """nx*ny odd: the last row runs alone, its pair's second row zero on the
+    way in (k_zpair_pack with lone) and its second spectrum taken as zero on
+    the way back (k_zpair_merge with lone): the separation then gives the
+    row's spectrum and zero, and the merge's inverse gives the row back with a
+    zero imaginary part."""
+    rng = np.random.default_rng(n + 7)
+    a = rng.standard_normal(n)
+    X1, X2 = separate_pair(np.fft.fft(a + 0j), n)
+    assert np.allclose(X1, np.fft.rfft(a), rtol=0, atol=1e-12 * n)
+    assert np.allclose(X2, 0, rtol=0, atol=1e-12 * n)
+    z = np.fft.ifft(merge_pair(X1, np.zeros_like(X1), n))
+    assert np.allclose(z.real, a, rtol=0, atol=1e-12) and np.allclose(z.imag, 0, rtol=0, atol=1e-12)
 
 
 def test_planted_slips_are_caught():
