@@ -24,7 +24,9 @@ Prints one JSON line on stdout (progress goes to stderr):
   extras       the other configurations, measured beside the headline: FP64
                solves at 512^3, 768^3 and 1000^3; the 64-grid batching
                workload (FP64 and FP32 + 2 steps); the Poisson serving
-               workload (p50/p99, points/s)
+               workload (p50/p99, points/s) with the default policy and,
+               at the same arrivals, with large requests nearly alone;
+               same-shape bursts through the server
 """
 import argparse
 import json
@@ -316,11 +318,11 @@ def bench_serving_grouped(ff, nreq=128):
     return out
 
 
-def bench_serving(ff, nreq=96, seed=20261202):
+def bench_serving(ff, nreq=96, seed=20261202, flags=0, rate=None):
     """The serving configuration: Poisson arrivals, edges log-uniform in
     [32, 512] (any integer; half cubic), half FP64 and half FP32 + 2
     refinement steps, f = the analytic field plus 16 random modes; the arrival
-    rate is 75 % of the throughput a closed-loop burst of the same mix
+    rate (unless given) is 75 % of the throughput a closed-loop burst of the same mix
     reaches.  Reports p50/p99 latency, points/s and the worst Linf."""
     import numpy as np
 
@@ -340,10 +342,10 @@ def bench_serving(ff, nreq=96, seed=20261202):
 
     # size-class groups of at most two requests, gathered for 100 us: each
     # request runs its own solve, so larger groups only queue their later
-    # members behind the earlier ones on one stream.  The default policy: with
-    # FF_SERVE_LARGE_ALONE the burst capacity is about 1.5x higher, but at 75 %
-    # of that capacity under Poisson arrivals requests queue (p50 47 ms)
-    cfg = dict(hbm_budget=48 << 30, streams=4, max_batch=2, batch_window_us=100.0, flags=0)
+    # members behind the earlier ones on one stream.  flags 0: the default
+    # policy; FF_SERVE_LARGE_ALONE is measured at the default policy's rate
+    # (at 75 % of its own, higher, burst capacity requests queue)
+    cfg = dict(hbm_budget=48 << 30, streams=4, max_batch=2, batch_window_us=100.0, flags=flags)
     srv = ff.Server(**cfg)
     # closed-loop burst: capacity in requests/s for this mix
     burst = [request(k) for k in range(32)]
@@ -353,7 +355,7 @@ def bench_serving(ff, nreq=96, seed=20261202):
         srv.wait(rid, timeout=300.0)
     cap = len(burst) / (time.perf_counter() - t0)
     srv.close()
-    rate = 0.75 * cap
+    rate = rate or 0.75 * cap
     srv = ff.Server(**cfg)
     reqs = [request(k) for k in range(nreq)]
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
@@ -369,12 +371,13 @@ def bench_serving(ff, nreq=96, seed=20261202):
         worst[prec] = max(worst[prec], srv.wait(rid, timeout=300.0)["norms"][2])
     st = srv.stats()
     srv.close()
-    res = {"requests": nreq, "policy": "default", "arrival_rate_per_s": round(rate, 2),
+    res = {"requests": nreq, "policy": "large_alone" if flags else "default", "arrival_rate_per_s": round(rate, 2),
            "burst_capacity_per_s": round(cap, 2),
            "p50_ms": round(1e3 * st["p50_s"], 4), "p99_ms": round(1e3 * st["p99_s"], 4),
            "points_per_s": st["points_per_s"], "groups": st["groups"], "arena_peak_bytes": st["arena_peak_bytes"],
            "worst_linf_fp64": worst[ff.FF_FP64], "worst_linf_fp32_refine2": worst[ff.FF_FP32_REFINE]}
-    log(f"serving: {nreq} requests at {rate:.1f}/s (75% of {cap:.1f}/s): p50 {res['p50_ms']:.2f} ms, "
+    log(f"serving ({res['policy']}): {nreq} requests at {rate:.1f}/s (burst capacity {cap:.1f}/s): "
+        f"p50 {res['p50_ms']:.2f} ms, "
         f"p99 {res['p99_ms']:.2f} ms, {st['points_per_s']:.4e} points/s, worst Linf {worst}")
     return res
 
@@ -546,6 +549,8 @@ def main():
             "serving": bench_serving(ff),
             "serving_same_shape": bench_serving_grouped(ff),
         }
+        extras["serving_large_alone"] = bench_serving(ff, flags=ff.FF_SERVE_LARGE_ALONE,
+                                                      rate=extras["serving"]["arrival_rate_per_s"])
 
     cpu = None
     if not args.no_cpu:
