@@ -529,7 +529,7 @@ def field_norms(field: Field, phi, stream=None) -> Tuple[float, float, float]:
 from .driver import FishFry  # noqa: E402  (needs the names above)
 
 __all__ = [
-    "ParisPencil", "FishFry", "Batch", "Server", "FF_BATCH_GRAPH", "FF_BATCH_PERSISTENT", "FF_BATCH_GROUPED", "FF_BATCH_STEPS", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
+    "ParisPencil", "FishFry", "Batch", "Server", "FF_BATCH_GRAPH", "FF_BATCH_PERSISTENT", "FF_BATCH_GROUPED", "FF_BATCH_STEPS", "FF_SERVE_LARGE_ALONE", "Field", "field_default", "field_set_modes", "field_init", "field_diff",
     "field_norms", "reference_bytes", "benchmark_grid", "FishFryError", "lib", "LIB_PATH",
     "FF_FP64", "FF_FP32_REFINE", "FF_PRESERVE_INPUT", "FF_BLOCKED_LAYOUT", "FF_SLAB_PIPELINE", "FF_SYMBOL_3PT", "FF_SYMBOL_5PT", "FF_PENCIL_LAYOUT", "FF_PLAIN_LAYOUT", "FF_TUNE_SPLIT_LDS", "FF_TUNE_TWO_GROUPS", "FF_TUNE_GENERIC", "FF_TUNE_LDS_TWIDDLES",
 ]

@@ -35,6 +35,22 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_python_constants_match_the_header():
+    """Every FF_* integer constant the Python package defines has the value
+    the C header gives it (flags, precisions, status codes)."""
+    text = open(os.path.join(ROOT, "include", "fishfry.h")).read()
+    defines = {m.group(1): int(m.group(2), 0)
+               for m in re.finditer(r"#define\s+(FF_[A-Z0-9_]+)\s+\(?(-?(?:0x[0-9a-fA-F]+|\d+))\)?", text)}
+    enums = {m.group(1): int(m.group(2)) for m in re.finditer(r"\b(FF_[A-Z0-9_]+)\s*=\s*(-?\d+)", text)}
+    header = {**enums, **defines}
+    checked = 0
+    for name in dir(ff):
+        if name.startswith("FF_") and isinstance(getattr(ff, name), int) and name in header:
+            assert getattr(ff, name) == header[name], name
+            checked += 1
+    assert checked >= 15 and "FF_SERVE_LARGE_ALONE" in header
+
+
 @pytest.mark.parametrize("n,m", [
     ((64, 64, 64), (1, 1, 1)), ((1024, 1024, 1024), (1, 1, 1)), ((12, 10, 9), (1, 1, 1)),
     ((128, 256, 384), (1, 2, 3)), ((100, 90, 80), (2, 2, 2)), ((33, 17, 65), (3, 1, 4)),
