@@ -343,8 +343,8 @@ def bench_serving(ff, nreq=96, seed=20261202, flags=0, rate=None):
     # size-class groups of at most two requests, gathered for 100 us: each
     # request runs its own solve, so larger groups only queue their later
     # members behind the earlier ones on one stream.  flags 0: the default
-    # policy; FF_SERVE_LARGE_ALONE is measured at the default policy's rate
-    # (at 75 % of its own, higher, burst capacity requests queue)
+    # policy; FF_SERVE_LARGE_ALONE is measured at the default policy's rate,
+    # so that both see the same arrivals
     cfg = dict(hbm_budget=48 << 30, streams=4, max_batch=2, batch_window_us=100.0, flags=flags)
     srv = ff.Server(**cfg)
     # closed-loop burst: capacity in requests/s for this mix

@@ -468,8 +468,9 @@ void worker_main(ff_server* s) {
     for (auto& oc : order) {
       std::vector<Req*>& reqs = classes[oc.second];
       const double w = alone ? group_work(reqs, s->cfg.max_batch) : 0.0;
-      // beside a large group only small ones run
-      if (alone && s->large_inflight && w >= kSmallWork) break;
+      // beside a large group only small ones run (a larger group waits
+      // for it to complete; newer small classes go ahead meanwhile)
+      if (alone && s->large_inflight && w >= kSmallWork) continue;
       const bool full = (int)reqs.size() >= s->cfg.max_batch;
       const bool due = t - reqs.front()->t_submit >= window || s->inflight.empty();
       if (!full && !due) continue;
