@@ -71,13 +71,16 @@ def test_alone_equals_in_mix_bitwise():
 
 def test_large_alone_bitwise():
     """FF_SERVE_LARGE_ALONE: large requests (FP32 + 2 steps at 264^3, FP64 at
-    400^3: above 3e8 point-passes) run with the device to themselves among small
-    ones, and every result equals the same request solved on a default server."""
+    400^3: above 3e8 point-passes) run with only small ones beside them, a
+    medium one (FP32 + 2 at 200^3, 1.4e8 point-passes) waits while a large
+    one runs, and every result equals the same request solved on a default
+    server."""
     rng = np.random.default_rng(23)
     reqs = []
     for k in range(12):
-        if k in (2, 7):
-            n, prec = ((264, 264, 264), ff.FF_FP32_REFINE) if k == 2 else ((400, 400, 400), ff.FF_FP64)
+        if k in (2, 3, 7):
+            n, prec = {2: ((264, 264, 264), ff.FF_FP32_REFINE), 3: ((200, 200, 200), ff.FF_FP32_REFINE),
+                       7: ((400, 400, 400), ff.FF_FP64)}[k]
         else:
             n = tuple(int(v) for v in rng.integers(24, 80, size=3))
             prec = ff.FF_FP64 if k % 2 else ff.FF_FP32_REFINE
