@@ -495,14 +495,40 @@ __device__ __forceinline__ void col_twiddles_to_lds(const ColArgs& a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<F>* twl = reinterpret_cast<cx<F>*>(smem_raw + col_exchange_bytes<F, SPLIT, SN, ST>());
   const FF_GLB cx<F>* twg = glb<const cx<F>>(a.tw);
-  for (int i = threadIdx.x; i < SN / 2; i += THREADS) twl[tw_swizzle(i)] = twg[i];
-  if constexpr (MODE == COL_FILTER_FD) {
-    // the x axis's finite-difference factors (SN doubles) follow the
-    // twiddles: the filter then reads them from LDS instead of through the
-    // vector memory path (launch LDS: SN * 8 bytes more)
-    double* sym = reinterpret_cast<double*>(twl + SN / 2);
+  // every load is issued before the first LDS store, so the copy costs one
+  // memory round trip: a loop that stores each value as it arrives waits
+  // for each load in turn, behind the tile's loads
+  // (indexes clamped into the table: every load is unconditional, which
+  // keeps the values in registers)
+  constexpr int NT = (SN / 2 + THREADS - 1) / THREADS;
+  cx<F> tv[NT];
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    const int i = min((int)threadIdx.x + k * THREADS, SN / 2 - 1);
+    tv[k] = cx<F>{twg[i].x, twg[i].y};
+  }
+  // (finite-difference x pass: the x axis's factors, SN doubles, follow the
+  // twiddles, and the filter reads them from LDS instead of through the
+  // vector memory path; launch LDS: SN * 8 bytes more)
+  constexpr int NS = MODE == COL_FILTER_FD ? (SN + THREADS - 1) / THREADS : 0;
+  double sv[NS > 0 ? NS : 1];
+  if constexpr (NS > 0) {
     const FF_GLB double* symg = glb<const double>(a.symx);
-    for (int i = threadIdx.x; i < SN; i += THREADS) sym[i] = symg[i];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) sv[k] = symg[min((int)threadIdx.x + k * THREADS, SN - 1)];
+  }
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    const int i = threadIdx.x + k * THREADS;
+    if (i < SN / 2) twl[tw_swizzle(i)] = tv[k];
+  }
+  if constexpr (NS > 0) {
+    double* sym = reinterpret_cast<double*>(twl + SN / 2);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const int i = threadIdx.x + k * THREADS;
+      if (i < SN) sym[i] = sv[k];
+    }
   }
   __syncthreads();
 }
